@@ -1,0 +1,48 @@
+#!/usr/bin/env bash
+# Build the reference Openpipeflow core (test oracle) for one resolution.
+#
+# Test infrastructure only.  Compiles the reference's own core modules from
+# the sources where they lie (REF, default /root/reference) with amdflang and
+# links them with oracle/ref_driver.f90 against MKL's LAPACK and FFTW3
+# wrappers (/opt/conda/lib/libmkl_rt.so).  The reference keeps its sizes as
+# compile-time parameters (parameters.f90:17-20), so a configured copy of
+# parameters.f90 with i_N/i_K/i_M/i_Mp replaced is written to a temporary
+# directory outside the repository; the copy, the preprocessed sources and the
+# objects are deleted afterwards.  Only the executable lands in oracle/_ref/.
+#
+# usage: oracle/build_ref.sh NAME N K M [Mp]
+set -euo pipefail
+NAME=$1; N=$2; K=$3; M=$4; MP=${5:-1}
+REF=${REF:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=${FC:-amdflang}
+[ -f "$REF/velocity.f90" ] || { echo "reference sources not found in $REF" >&2; exit 2; }
+mkdir -p "$OUT"
+TMP=$(mktemp -d /tmp/opf_ref_build.XXXXXX)
+trap 'rm -rf "$TMP"' EXIT
+mkdir -p "$TMP/program" "$TMP/pp"
+# The sources include "../parallel.h" relative to their own directory.
+sed -e '/^#define _CPUTIME/d' "$REF/parallel.h" > "$TMP/parallel.h"
+sed -e "s/\(i_N *= *\)[0-9][0-9]*/\1$N/" \
+    -e "s/\(i_K *= *\)[0-9][0-9]*/\1$K/" \
+    -e "s/\(i_M *= *\)[0-9][0-9]*/\1$M/" \
+    -e "s/\(i_Mp *= *\)[0-9][0-9]*/\1$MP/" \
+    "$REF/parameters.f90" > "$TMP/program/parameters.f90"
+grep -E "i_(N|K|M|Mp) +=" "$TMP/program/parameters.f90" | head -4 >&2
+cd "$TMP/program"
+for f in mpi meshs variables transform.fftw3 timestep velocity; do
+  cpp -traditional-cpp -P -I"$TMP/program" "$REF/$f.f90" > "$TMP/pp/$f.f90"
+done
+cpp -traditional-cpp -P -I"$TMP/program" parameters.f90 > "$TMP/pp/parameters.f90"
+cd "$TMP/pp"
+FFLAGS="-O2"
+$FC $FFLAGS -c "$REF/cheby.f" -o cheby.o
+for f in parameters mpi meshs variables transform.fftw3 timestep velocity; do
+  $FC $FFLAGS -c $f.f90 2>/dev/null
+done
+$FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
+$FC -o "$OUT/opf_ref_$NAME" ref_driver.o parameters.o mpi.o cheby.o meshs.o \
+    variables.o transform.fftw3.o timestep.o velocity.o \
+    -L/opt/conda/lib -lmkl_rt -Wl,-rpath,/opt/conda/lib
+echo "built $OUT/opf_ref_$NAME (N=$N K=$K M=$M Mp=$MP)" >&2

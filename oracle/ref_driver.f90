@@ -1,0 +1,163 @@
+!*************************************************************************
+! Harness driver for the compiled reference (oracle/_ref).
+!
+! Test infrastructure only.  It is linked against the reference's own core
+! modules (parameters, mpif, meshs, variables, transform, timestep,
+! velocity), compiled from the sources in place by oracle/build_ref.sh, and
+! replaces only main.f90 / io.f90: the time loop below follows main.f90:18-46
+! call for call, and the diagnostics follow io.f90:501-524 (energy) and
+! io.f90:627-639 (friction velocity).  netCDF-Fortran is absent, so the
+! state is exchanged as raw little-endian float64 in the reference's own
+! coll layout: Re(1:N,0:H-1) then Im(1:N,0:H-1) for ur, ut, uz.
+!
+! usage: opf_ref_<cfg> state_in out_prefix Re alpha dt nsteps [dump steps...]
+!   writes out_prefix_series.txt   one line per step:
+!          step t E Ek0 Em0 Pr0(=Ub) Uc Ufr ncorr dterr_last
+!   out_prefix_stage_*.bin         step-1 intermediates (N after each
+!                                  nonlinear evaluation, u after predictor
+!                                  and after each corrector)
+!   out_prefix_state_XXXXXX.bin    state after the listed steps
+!   out_prefix_timing.txt          wall seconds for the time loop
+!*************************************************************************
+ program opf_ref_driver
+   use velocity
+   implicit none
+   character(len=512) :: infile, prefix, arg
+   double precision   :: re_in, alpha_in, dt_in
+   integer            :: nsteps, ndump, i, ncorr, nstage
+   integer, allocatable :: dumps(:)
+   integer(8)         :: c0, c1, crate
+   double precision   :: dterr_last, E, Ek0, Em0, Uc, Ufr, Ub
+   integer, parameter :: useries = 41, utim = 42
+
+   if(command_argument_count() < 6) stop 'usage: state_in prefix Re alpha dt nsteps [dumps]'
+   call get_command_argument(1, infile)
+   call get_command_argument(2, prefix)
+   call get_command_argument(3, arg);  read(arg,*) re_in
+   call get_command_argument(4, arg);  read(arg,*) alpha_in
+   call get_command_argument(5, arg);  read(arg,*) dt_in
+   call get_command_argument(6, arg);  read(arg,*) nsteps
+   ndump = command_argument_count() - 6
+   allocate(dumps(max(ndump,1)))
+   dumps = -1
+   do i = 1, ndump
+      call get_command_argument(6+i, arg);  read(arg,*) dumps(i)
+   end do
+
+   call mpi_precompute()
+   call par_precompute()
+   d_Re    = re_in
+   d_alpha = alpha_in
+   call mes_precompute()
+   call var_precompute()
+   call tra_precompute()
+   call tim_precompute()
+   call vel_precompute()
+   tim_dt = dt_in
+   call read_state(trim(infile))
+   call vel_matrices()
+
+   open(useries, file=trim(prefix)//'_series.txt', status='replace')
+   nstage = 0
+   call system_clock(c0, crate)
+
+   do while(tim_step < nsteps)
+      call vel_imposesym()
+      call vel_transform()
+      call vel_nonlinear()
+      if(tim_step==0) call dump_stage('N', vel_Nr, vel_Nt, vel_Nz)
+      call diagnostics(E, Ek0, Em0, Uc, Ufr)
+      Ub = vel_Pr0            ! io_write_friction reads it here (io.f90:631)
+      call vel_predictor()
+      if(tim_step==0) call dump_stage('u', vel_ur, vel_ut, vel_uz)
+      tim_it = 1
+      ncorr  = 0
+      do while(tim_it/=0)
+         call vel_transform()
+         call vel_nonlinear()
+         if(tim_step==0) call dump_stage('N', vel_Nr, vel_Nt, vel_Nz)
+         call vel_corrector()
+         if(tim_step==0) call dump_stage('u', vel_ur, vel_ut, vel_uz)
+         ncorr = ncorr + 1
+         dterr_last = tim_dterr
+         call tim_check_cgce()
+      end do
+      if(tim_step < 0) stop 'corrector failure'
+      write(useries,'(I8,7ES26.17E3,I4,ES26.17E3)') tim_step, tim_t, E, Ek0, Em0,  &
+         Ub, Uc, Ufr, ncorr, dterr_last
+      tim_t    = tim_t    + tim_dt
+      tim_step = tim_step + 1
+      do i = 1, ndump
+         if(dumps(i)==tim_step) call write_state(tim_step)
+      end do
+   end do
+
+   call system_clock(c1)
+   close(useries)
+   open(utim, file=trim(prefix)//'_timing.txt', status='replace')
+   write(utim,'(ES26.17E3,I8)') dble(c1-c0)/dble(crate), nsteps
+   close(utim)
+
+ contains
+
+   subroutine diagnostics(E, Ek0, Em0, Uc, Ufr)
+      ! io_write_energy (io.f90:501-524) and io_write_friction (io.f90:627-639)
+      double precision, intent(out) :: E, Ek0, Em0, Uc, Ufr
+      double precision :: E_, Ek(0:i_K1), Em(0:i_M1)
+      call var_coll_norm(vel_ur, E,Ek,Em)
+      Ek0 = Ek(0)
+      Em0 = Em(0)
+      call var_coll_norm(vel_ut, E_,Ek,Em)
+      E   = E + E_
+      Ek0 = Ek0 + Ek(0)
+      Em0 = Em0 + Em(0)
+      call var_coll_norm(vel_uz, E_,Ek,Em)
+      E   = E + E_
+      Ek0 = Ek0 + Ek(0)
+      Em0 = Em0 + Em(0)
+      Uc  = 1d0 + dot_product(vel_uz%Re(1:1+i_KL,0),mes_D%dr0(:,0))
+      Ufr = dot_product(vel_uz%Re(i_N-i_KL:i_N,0),mes_D%dr1(:,1))
+      Ufr = dsqrt(dabs((Ufr-2d0)/d_Re))
+   end subroutine diagnostics
+
+   subroutine read_state(fn)
+      character(*), intent(in) :: fn
+      integer :: u
+      open(newunit=u, file=fn, access='stream', form='unformatted', status='old')
+      call var_coll_init(vel_ur)
+      call var_coll_init(vel_ut)
+      call var_coll_init(vel_uz)
+      read(u) vel_ur%Re(:,0:i_H1), vel_ur%Im(:,0:i_H1)
+      read(u) vel_ut%Re(:,0:i_H1), vel_ut%Im(:,0:i_H1)
+      read(u) vel_uz%Re(:,0:i_H1), vel_uz%Im(:,0:i_H1)
+      close(u)
+   end subroutine read_state
+
+   subroutine write_coll3(fn, a, b, c)
+      character(*), intent(in) :: fn
+      type (coll),  intent(in) :: a, b, c
+      integer :: u
+      open(newunit=u, file=fn, access='stream', form='unformatted', status='replace')
+      write(u) a%Re(:,0:i_H1), a%Im(:,0:i_H1)
+      write(u) b%Re(:,0:i_H1), b%Im(:,0:i_H1)
+      write(u) c%Re(:,0:i_H1), c%Im(:,0:i_H1)
+      close(u)
+   end subroutine write_coll3
+
+   subroutine write_state(step)
+      integer, intent(in) :: step
+      character(6) :: cnum
+      write(cnum,'(I6.6)') step
+      call write_coll3(trim(prefix)//'_state_'//cnum//'.bin', vel_ur, vel_ut, vel_uz)
+   end subroutine write_state
+
+   subroutine dump_stage(tag, a, b, c)
+      character(*), intent(in) :: tag
+      type (coll),  intent(in) :: a, b, c
+      character(2) :: cnum
+      nstage = nstage + 1
+      write(cnum,'(I2.2)') nstage
+      call write_coll3(trim(prefix)//'_stage_'//cnum//tag//'.bin', a, b, c)
+   end subroutine dump_stage
+
+ end program opf_ref_driver

@@ -1,0 +1,90 @@
+"""Write the golden fixtures from the compiled reference (oracle/_ref).
+
+Run in the build container only (the reference binaries are built there by
+oracle/build_ref.sh).  For each workload the seeded initial state is produced
+by the package's generator on the oracle's mesh, written in the reference's
+raw coll layout, and integrated by the reference executable; the fixture keeps
+the initial state, the reference's states after chosen steps, its step-1 stage
+dumps (mini only) and its per-step series (t, E, Ek0, Em0, Pr0, Uc, Ufr,
+ncorr, dterr).  Fixtures are data only: inputs and the reference's outputs.
+
+usage: python tests/golden/make_golden.py [mini c0 s48 c1 ...]
+"""
+from __future__ import annotations
+
+import os
+import resource
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.join(REPO, "softx-d-17-00015_amd"))
+import opf_oracle as O  # noqa: E402
+from opf import ic  # noqa: E402
+
+# name: (N, K, M, Re, alpha, dt, E0, seed, nsteps, state steps saved, keep states fully)
+WORKLOADS = {
+    "mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 100, (1, 2, 10, 100), True),
+    "c0": (32, 24, 12, 2400.0, 1.25, 0.01, 1e-3, 1, 100, (1, 100), True),
+    "s48": (48, 48, 24, 3000.0, 1.25, 0.01, 1e-3, 7, 50, (1,), False),
+    "c1": (64, 64, 48, 5300.0, 0.5, 0.005, 1e-2, 1, 1000, (1,), False),
+}
+SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
+
+
+def _unlimited_stack():
+    # The reference keeps large automatic coll temporaries (e.g. velocity.f90:417)
+    # on the stack; its users run it with an unlimited stack.
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def make(name):
+    N, K, M, Re, alpha, dt, E0, seed, nsteps, saves, full = WORKLOADS[name]
+    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{name}")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_ref.sh {name} {N} {K} {M}")
+    mesh = O.build_mesh(N)
+    ur, ut, uz = ic.seeded_perturbation(mesh.r, mesh.intrdr, K, M, alpha, seed, E0)
+    H = ur.shape[1]
+    out = dict(N=N, K=K, M=M, Re=Re, alpha=alpha, dt=dt, E0=E0, seed=seed,
+               nsteps=nsteps, r=mesh.r)
+    with tempfile.TemporaryDirectory() as td:
+        icf = os.path.join(td, "ic.bin")
+        O.write_coll3(icf, ur, ut, uz)
+        pre = os.path.join(td, name)
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+        subprocess.run([exe, icf, pre, repr(Re), repr(alpha), repr(dt), str(nsteps)]
+                       + [str(s) for s in saves], check=True, cwd=td, env=env,
+                       stdout=subprocess.DEVNULL, preexec_fn=_unlimited_stack)
+        ser = np.loadtxt(pre + "_series.txt", ndmin=2)
+        out["series"] = ser
+        sel = np.arange(0, H, SAMPLE_STRIDE) if not full else np.arange(H)
+        out["sel"] = sel
+        if full:
+            out["ic"] = np.stack([ur, ut, uz])
+        else:
+            out["ic_sel"] = np.stack([ur, ut, uz])[:, :, sel]
+        for s in saves:
+            st = np.stack(O.read_coll3(f"{pre}_state_{s:06d}.bin", N, H))
+            out[f"state_{s}"] = st[:, :, sel] if not full else st
+            out[f"state_{s}_norm"] = np.array([np.abs(st[c]).max() for c in range(3)])
+        if name == "mini":
+            stages = sorted(f for f in os.listdir(td) if "_stage_" in f)
+            for f in stages:
+                tag = f.split("_stage_")[1].split(".")[0]
+                out[f"stage_{tag}"] = np.stack(O.read_coll3(os.path.join(td, f), N, H))
+        with open(pre + "_timing.txt") as fh:
+            out["ref_seconds"] = float(fh.read().split()[0])
+    path = os.path.join(HERE, f"golden_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+if __name__ == "__main__":
+    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1"]):
+        make(nm)

@@ -1,0 +1,117 @@
+"""The oracle against the compiled reference's golden vectors.
+
+The fixtures were written by the reference Fortran itself (oracle/_ref, via
+tests/golden/make_golden.py) from the same seeded initial states.  Bars:
+one-step spectral coefficients <= 1e-12 normwise relative per field; energy
+and friction velocity series <= 1e-10 relative; identical corrector counts.
+"""
+import numpy as np
+import pytest
+
+import opf_oracle as O
+from conftest import load_golden, rel_err
+
+TOL_STEP = 1e-12
+TOL_DIAG = 1e-10
+
+
+def _ctx(g):
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    return O.Ctx(N, K, M, Re=float(g["Re"]), alpha=float(g["alpha"]), dt=float(g["dt"]))
+
+
+def test_mesh_matches_reference_points():
+    g = load_golden("c0")
+    ctx = _ctx(g)
+    assert np.array_equal(ctx.mesh.r, g["r"])
+
+
+@pytest.mark.parametrize("name", ["mini", "c0"])
+def test_one_step_parity(name):
+    g = load_golden(name)
+    ctx = _ctx(g)
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    ctx.run(1)
+    ref = g["state_1"]
+    for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
+        assert rel_err(u, ref[c]) <= TOL_STEP, (name, c)
+
+
+def test_stage_parity_mini():
+    """Step-1 intermediates: N after each nonlinear evaluation, u after the
+    predictor and the corrector (localises a mismatch to a stage)."""
+    g = load_golden("mini")
+    ctx = _ctx(g)
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    stages = {}
+
+    ctx.imposesym()
+    ctx.transform()
+    ctx.nonlinear()
+    stages["01N"] = (ctx.Nr, ctx.Nt, ctx.Nz)
+    ctx.predictor()
+    stages["02u"] = (ctx.ur, ctx.ut, ctx.uz)
+    ctx.transform()
+    ctx.nonlinear()
+    stages["03N"] = (ctx.Nr, ctx.Nt, ctx.Nz)
+    ctx.dterr = 0.0
+    ctx.corrector()
+    stages["04u"] = (ctx.ur, ctx.ut, ctx.uz)
+    for tag, fields in stages.items():
+        ref = g[f"stage_{tag}"]
+        for c in range(3):
+            assert rel_err(fields[c], ref[c]) <= TOL_STEP, (tag, c)
+
+
+def test_multi_step_mini():
+    g = load_golden("mini")
+    ctx = _ctx(g)
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    states = {}
+    ser = ctx.run(100, record=lambda c: states.__setitem__(c.step, (c.ur.copy(), c.ut.copy(), c.uz.copy())))
+    for s in (1, 2, 10, 100):
+        ref = g[f"state_{s}"]
+        for c in range(3):
+            assert rel_err(states[s][c], ref[c]) <= 1e-11, (s, c)
+    rs = g["series"]
+    E = np.array([d["E"] for d in ser])
+    Ufr = np.array([d["Ufr"] for d in ser])
+    nc = np.array([d["ncorr"] for d in ser])
+    assert np.max(np.abs(E - rs[:, 2]) / rs[:, 2]) <= TOL_DIAG
+    assert np.max(np.abs(Ufr - rs[:, 7]) / rs[:, 7]) <= TOL_DIAG
+    assert np.array_equal(nc, rs[:, 8].astype(int))
+
+
+def test_series_c0_first_steps():
+    g = load_golden("c0")
+    ctx = _ctx(g)
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    ser = ctx.run(10)
+    rs = g["series"][:10]
+    for i, d in enumerate(ser):
+        assert abs(d["E"] - rs[i, 2]) <= TOL_DIAG * rs[i, 2]
+        assert abs(d["Ufr"] - rs[i, 7]) <= TOL_DIAG * rs[i, 7]
+        assert abs(d["Pr0"] - rs[i, 5]) <= 1e-10 * max(abs(rs[i, 5]), 1e-6)
+        assert d["ncorr"] == int(rs[i, 8])
+
+
+def test_one_step_s48_sampled():
+    g = load_golden("s48")
+    ctx = _ctx(g)
+    # The fixture keeps every 37th harmonic; regenerate the full initial state
+    # with the package's generator and check it reproduces the stored sample.
+    from opf import ic as icgen
+    ur, ut, uz = icgen.seeded_perturbation(ctx.mesh.r, ctx.mesh.intrdr, ctx.K, ctx.M,
+                                           ctx.alpha, int(g["seed"]), float(g["E0"]))
+    sel = g["sel"]
+    assert rel_err(np.stack([ur, ut, uz])[:, :, sel], g["ic_sel"]) <= 1e-15
+    ctx.set_state(ur, ut, uz)
+    ctx.run(1)
+    ref = g["state_1"]
+    for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
+        den = g["state_1_norm"][c]
+        assert np.abs(u[:, sel] - ref[c]).max() / den <= TOL_STEP

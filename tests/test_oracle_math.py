@@ -1,0 +1,239 @@
+"""Pins of the oracle that do not need the reference: closed forms, exactness
+of the finite-difference weights, DFT by brute force, the banded LU against a
+dense solve and against scipy's LAPACK, and the linear decay of an
+axisymmetric swirl mode.  Each check is chosen so that a plausible mistake
+(dropped term, wrong sign, transposed band index) fails it."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg.lapack as lapack
+
+import opf_oracle as O
+
+N = 24
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    return O.build_mesh(N)
+
+
+def test_default_mesh_shape(mesh):
+    r = mesh.r
+    assert abs(r[-1] - 1.0) < 1e-14
+    assert r[0] > 0.0
+    assert np.all(np.diff(r) > 0)
+    # clustering at the wall: last spacing smaller than the middle spacing
+    assert r[-1] - r[-2] < r[N // 2] - r[N // 2 - 1]
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 4])
+def test_fd_first_derivative_exact_low_degree(mesh, p):
+    r = mesh.r
+    d1 = O.band_to_dense(mesh.dr[1])
+    d2 = O.band_to_dense(mesh.dr[2])
+    f = r ** p
+    exact1 = p * r ** (p - 1) if p > 0 else 0 * r
+    exact2 = p * (p - 1) * r ** (p - 2) if p > 1 else 0 * r
+    assert np.allclose(d1 @ f, exact1, atol=1e-8 * max(1, p * p), rtol=1e-10)
+    assert np.allclose(d2 @ f, exact2, atol=1e-6 * max(1, p ** 3), rtol=1e-9)
+
+
+def test_fd_interior_exact_degree_8(mesh):
+    r = mesh.r
+    d1 = O.band_to_dense(mesh.dr[1])
+    f = r ** 8
+    interior = slice(4, N - 4)
+    assert np.allclose((d1 @ f)[interior], (8 * r ** 7)[interior], rtol=1e-8, atol=1e-9)
+
+
+def test_radlap_is_d2_plus_d1_over_r(mesh):
+    r = mesh.r
+    L = O.band_to_dense(mesh.radLap)
+    f = r ** 4          # (1/r) d/dr + d2/dr2 of r^4 = 16 r^2
+    assert np.allclose(L @ f, 16 * r ** 2, rtol=1e-9, atol=1e-8)
+
+
+def test_axis_and_wall_weights(mesh):
+    r = mesh.r
+    x = np.concatenate([[0.0], r])
+    for p in range(0, 5):
+        f = x ** p
+        # dr0(:,i): i-th derivative at r=0 from r_1..r_5
+        v0 = mesh.dr0[:, 0] @ f[1:6]
+        assert abs(v0 - (1.0 if p == 0 else 0.0)) < 1e-10
+        v1 = mesh.dr0[:, 1] @ f[1:6]
+        assert abs(v1 - (1.0 if p == 1 else 0.0)) < 1e-9
+        # dr1(:,1): derivative at r=1
+        w1 = mesh.dr1[:, 1] @ f[N - 4:N + 1]
+        assert abs(w1 - p) < 1e-9 * max(1, p)
+    # dr1(:,0): value at the wall point itself is the identity weight
+    assert np.allclose(mesh.dr1[:, 0], [0, 0, 0, 0, 1], atol=1e-13)
+
+
+def test_intrdr_quadrature_order():
+    """intrdr (meshs.f90:176-192) weights a Taylor integral of f over the half
+    cells around r_n by r_n: exact for f = 1 (the rule is then the trapezoid
+    rule for r, which is linear) and second order otherwise."""
+    m1, m2 = O.build_mesh(24), O.build_mesh(48)
+    assert abs(np.dot(m1.intrdr, np.ones(24)) - 0.5) < 1e-14
+    for p in range(1, 7):
+        e1 = np.dot(m1.intrdr, m1.r ** p) - 1.0 / (p + 2)
+        e2 = np.dot(m2.intrdr, m2.r ** p) - 1.0 / (p + 2)
+        assert abs(e1) < 5e-4
+        assert 3.5 < e1 / e2 < 4.5
+
+
+def test_laminar_energy_normalisation():
+    """E of U = 1 - r^2 stored in harmonic 0 is pi^2/(3 alpha) (the laminar
+    energy io.f90:548 normalises by), up to intrdr's second-order error."""
+    E = []
+    for n in (32, 64):
+        ctx = O.Ctx(n, 2, 2, alpha=1.25)
+        uz = ctx.zeros()
+        uz[:, 0] = 1.0 - ctx.mesh.r ** 2
+        E.append(ctx.norm(uz)[0])
+    exact = math.pi ** 2 / (3 * 1.25)
+    e1, e2 = E[0] - exact, E[1] - exact
+    assert abs(e1) / exact < 2e-3
+    assert 3.5 < e1 / e2 < 4.5
+
+
+def test_friction_velocity_laminar():
+    ctx = O.Ctx(16, 3, 2, Re=2400.0)
+    ctx.set_state(ctx.zeros(), ctx.zeros(), ctx.zeros())
+    d = ctx.diagnostics()
+    assert abs(d["Ufr"] - math.sqrt(2.0 / 2400.0)) < 1e-15
+
+
+def _brute_spec2phys(ctx, a):
+    """Direct double sum of the Fourier series on the 3/2 grid."""
+    K, M = ctx.K, ctx.M
+    Z, T = 3 * K, 3 * M
+    zj = np.arange(Z)[:, None]
+    tj = np.arange(T)[None, :]
+    out = np.zeros((a.shape[0], Z, T))
+    for h in range(ctx.H):
+        k, m = int(ctx.harm.k[h]), int(ctx.harm.m[h])
+        ph = np.exp(2j * np.pi * (k * zj / Z + m * tj / T))
+        term = a[:, h][:, None, None] * ph[None]
+        if m == 0 and k == 0:
+            out += term.real
+        elif m == 0:
+            out += 2 * term.real        # k<0 partner is the conjugate
+        else:
+            out += 2 * term.real        # m<0 partner is the conjugate
+    return out
+
+
+def test_spec2phys_brute_force_and_roundtrip():
+    ctx = O.Ctx(6, 3, 3)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((6, ctx.H)) + 1j * rng.standard_normal((6, ctx.H))
+    a[:, 0] = a[:, 0].real
+    p = ctx.spec2phys(a)
+    assert np.allclose(p, _brute_spec2phys(ctx, a), atol=1e-12)
+    back = ctx.phys2spec(p)
+    assert np.allclose(back, a, atol=1e-13)
+
+
+def test_single_mode_grid_values():
+    ctx = O.Ctx(6, 4, 3)
+    a = ctx.zeros()
+    h = int(np.nonzero((ctx.harm.k == -2) & (ctx.harm.m == 1))[0][0])
+    a[:, h] = 0.5
+    p = ctx.spec2phys(a)
+    Z, T = 12, 9
+    zz, tt = np.meshgrid(np.arange(Z), np.arange(T), indexing="ij")
+    expect = np.cos(2 * np.pi * (-2 * zz / Z + tt / T))
+    assert np.allclose(p[0], expect, atol=1e-13)
+
+
+def test_dealiased_product_is_exact_convolution():
+    """The 3/2 grid makes the quadratic product alias-free: phys2spec of a
+    product equals the truncated spectral convolution."""
+    ctx = O.Ctx(6, 3, 2)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((6, ctx.H)) + 1j * rng.standard_normal((6, ctx.H))
+    b = rng.standard_normal((6, ctx.H)) + 1j * rng.standard_normal((6, ctx.H))
+    a[:, 0] = a[:, 0].real
+    b[:, 0] = b[:, 0].real
+    c = ctx.phys2spec(ctx.spec2phys(a) * ctx.spec2phys(b))
+
+    def full(x):
+        d = {}
+        for h in range(ctx.H):
+            k, m = int(ctx.harm.k[h]), int(ctx.harm.m[h])
+            d[(k, m)] = x[:, h]
+            d[(-k, -m)] = np.conj(x[:, h])
+        return d
+    fa, fb = full(a), full(b)
+    expect = np.zeros_like(c)
+    for h in range(ctx.H):
+        k, m = int(ctx.harm.k[h]), int(ctx.harm.m[h])
+        s = 0
+        for (k1, m1), v in fa.items():
+            w = fb.get((k - k1, m - m1))
+            if w is not None:
+                s = s + v * w
+        expect[:, h] = s
+    assert np.allclose(c, expect, atol=1e-12)
+
+
+def _random_band(rng, S, N):
+    AB = np.zeros((S, 13, N))
+    AB[:, 4:, :] = rng.standard_normal((S, 9, N))
+    return AB
+
+
+def test_band_lu_matches_lapack_and_dense():
+    rng = np.random.default_rng(11)
+    S, n = 7, 20
+    AB = _random_band(rng, S, n)
+    b = rng.standard_normal((n, S))
+    ABo = AB.copy()
+    ipiv = O.Ctx.dgbtf2(ABo)
+    x = O.Ctx.dgbtrs(ABo, ipiv, b)
+    for s in range(S):
+        lu, piv, info = lapack.dgbtrf(AB[s].copy(), 4, 4)
+        assert info == 0
+        assert np.array_equal(piv + 1, ipiv[s])
+        assert np.allclose(lu, ABo[s], rtol=0, atol=1e-12 * np.abs(lu).max())
+        xs, info = lapack.dgbtrs(lu, 4, 4, b[:, s], piv)
+        assert np.allclose(x[:, s], xs, rtol=1e-13, atol=1e-13 * np.abs(xs).max())
+        dense = np.zeros((n, n))
+        for j in range(n):
+            for i in range(max(0, j - 4), min(n, j + 5)):
+                dense[i, j] = AB[s, 8 + i - j, j]
+        assert np.allclose(dense @ x[:, s], b[:, s], atol=1e-10)
+
+
+def test_mat_invert_4x4():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((4, 4))
+    Ai = np.array(O.mat_invert(A.tolist()))
+    assert np.allclose(Ai @ A, np.eye(4), atol=1e-12)
+
+
+def test_swirl_mode_decays_at_stokes_rate():
+    """u_theta = J1(j11 r) at k = m = 0 is an exact solution decaying like
+    exp(-j11^2 t / Re); the nonlinear centrifugal term only feeds u_r, which
+    the step discards.  The theta-scheme factor per step is
+    (1/dt - (1-th) lam/Re) / (1/dt + th lam/Re), lam = j11^2."""
+    from scipy.special import j1
+    j11 = 3.8317059702075125
+    Re, dt = 100.0, 0.01
+    ctx = O.Ctx(32, 2, 2, Re=Re, dt=dt)
+    ut = ctx.zeros()
+    ut[:, 0] = j1(j11 * ctx.mesh.r)
+    ctx.set_state(ctx.zeros(), ut, ctx.zeros())
+    ctx.run(30)
+    a = ctx.ut[:, 0].real
+    ctx.run(40)
+    b = ctx.ut[:, 0].real
+    ratio = np.dot(b, a) / np.dot(a, a)
+    lam = j11 * j11
+    g = (1 / dt - (1 - 0.51) * lam / Re) / (1 / dt + 0.51 * lam / Re)
+    assert abs(ratio - g ** 10) < 1e-6
+    assert np.abs(ctx.ur).max() == 0.0
