@@ -1,0 +1,227 @@
+"""Benchmark of the Openpipeflow time step on one MI355X.
+
+Workload (default): the plain single-state run at configs[1] size,
+(N, K, M) = (64, 64, 48), Mp = 1, alpha = 0.5, Re = 5300, dt = 0.005,
+implicitness 0.51, constant flux, seeded perturbation of energy 1e-2, with
+the corrector iterated to convergence as main.f90:35-41 does.  The timed
+region is `--steps` consecutive steps of the device-resident loop after
+`--warmup` steps; the state is in HBM before timing starts.
+
+Printed: one JSON line with the metric (state-steps/s), the parity check of
+the energy / friction-velocity series against the reference's own run
+(tests/golden/golden_c1.npz, |dE|/E <= 1e-10), the HBM roofline of the
+dominant kernel (HIP events on the library's stream), and the reference CPU
+code timed on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "softx-d-17-00015_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+
+WORKLOADS = {
+    # name: N, K, M, Re, alpha, dt, E0, seed, golden fixture
+    "c1": (64, 64, 48, 5300.0, 0.5, 0.005, 1e-2, 1, "golden_c1.npz"),
+    "c0": (32, 24, 12, 2400.0, 1.25, 0.01, 1e-3, 1, "golden_c0.npz"),
+}
+
+
+def alg_bytes(N, K, M):
+    """Algorithmic (compulsory) bytes per launch of each kernel: every array
+    the kernel must read or write once.  C = one complex coll field."""
+    H = (2 * K - 1) * M - (K - 1)
+    P = K * M
+    Z = 3 * K
+    C = 16 * N * H
+    Y = 16 * N * Z * M                       # one field of the z-transformed intermediates
+    F = 4 * P * N * (13 * 8 + 1) + 6 * P * N * 8 + 16 * H * 8 + 8 * N
+    return dict(C=C, F=F,
+                zfwd=6 * C + 6 * Y, theta=9 * Y, zbwd=3 * Y + 3 * C,
+                step_pred=9 * C + F, step_corr=15 * C + F, curl=6 * C)
+
+
+def survey_bytes_per_step(N, K, M, ncorr):
+    """B_alg = (21 + 27 n_corr) C + (1 + n_corr) F, the two-phase compulsory
+    traffic model (factors once per +-k pair, influence vectors per harmonic)."""
+    H = (2 * K - 1) * M - (K - 1)
+    C = 16 * N * H
+    F = 4 * K * M * (8 * 13 * N + 4 * N) + H * (48 * N + 128)
+    return (21 + 27 * ncorr) * C + (1 + ncorr) * F
+
+
+def make_ic(ctx, wl):
+    from opf import ic
+    N, K, M, Re, alpha, dt, E0, seed, _ = wl
+    me = ctx.mesh()
+    return ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
+
+
+def cpu_baseline(name, wl, ic_fields, target_s):
+    """Reference Fortran (oracle/_ref, compiled from the reference sources
+    with MKL's LAPACK/FFTW3) on one host core, on a bounded number of steps
+    of the same workload; falls back to the numpy oracle."""
+    N, K, M, Re, alpha, dt, E0, seed, _ = wl
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import opf_oracle as O
+    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{name}")
+    per_step_guess = {"c1": 0.7, "c0": 0.025}.get(name, 0.5)
+    nsteps = max(3, int(target_s / per_step_guess))
+    if os.path.exists(exe):
+        try:
+            with tempfile.TemporaryDirectory() as td:
+                icf = os.path.join(td, "ic.bin")
+                O.write_coll3(icf, *ic_fields)
+                import resource
+
+                def lim():
+                    resource.setrlimit(resource.RLIMIT_STACK,
+                                       (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+                env = dict(os.environ, MKL_NUM_THREADS="1", MKL_THREADING_LAYER="SEQUENTIAL",
+                           OMP_NUM_THREADS="1")
+                subprocess.run([exe, icf, os.path.join(td, "b"), repr(Re), repr(alpha), repr(dt),
+                                str(nsteps)], check=True, cwd=td, env=env, preexec_fn=lim,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=300)
+                secs = float(open(os.path.join(td, "b_timing.txt")).read().split()[0])
+            return dict(value=nsteps / secs, unit="state-steps/s", cores=1, kind="reference",
+                        sample=f"{nsteps} steps of the same workload, reference Fortran "
+                               f"(amdflang -O2 + MKL sequential), 1 core, {secs:.1f} s")
+        except Exception as e:                       # noqa: BLE001
+            print(f"cpu_baseline: reference run failed ({e}); timing the oracle port",
+                  file=sys.stderr)
+    oc = O.Ctx(N, K, M, Re=Re, alpha=alpha, dt=dt)
+    oc.set_state(*ic_fields)
+    n = max(1, nsteps // 10)
+    t0 = time.perf_counter()
+    oc.run(n)
+    secs = time.perf_counter() - t0
+    return dict(value=n / secs, unit="state-steps/s", cores=1, kind="port",
+                sample=f"{n} steps of the same workload, numpy oracle, 1 process, {secs:.1f} s")
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design, not a code path)"}))
+        sys.exit(2)
+    import opf
+
+    name = args.workload
+    wl = WORKLOADS[name]
+    N, K, M, Re, alpha, dt, E0, seed, golden = wl
+    ctx = opf.Context(N, K, M)
+    ctx.setup(0, Re, alpha, dt)
+    fields = make_ic(ctx, wl)
+    ctx.put_state(0, *fields)
+
+    # warmup, then the timed region (the library synchronises its stream on
+    # both sides of opf_run)
+    ctx.run(args.warmup, diag_every=10)
+    t0 = time.perf_counter()
+    ctx.run(args.steps, diag_every=10)
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = args.steps / secs
+    diag = ctx.diag(0)
+
+    # parity of the energy and friction velocity series with the reference
+    parity = {"reference": golden, "checked_records": 0}
+    gpath = os.path.join(REPO, "tests", "golden", golden)
+    ncorr_mean = 1.0
+    if os.path.exists(gpath):
+        with np.load(gpath) as g:
+            rs = g["series"]
+        rows = [r for r in diag if int(r[0]) < rs.shape[0]]
+        if rows:
+            d = np.array(rows)
+            ref = rs[d[:, 0].astype(int)]
+            dE = np.abs(d[:, 2] - ref[:, 2]) / ref[:, 2]
+            dU = np.abs(d[:, 7] - ref[:, 7]) / ref[:, 7]
+            parity.update(checked_records=len(rows), max_rel_dE=float(dE.max()),
+                          max_rel_dutau=float(dU.max()),
+                          ncorr_equal=bool(np.array_equal(d[:, 8].astype(int), ref[:, 8].astype(int))),
+                          ok=bool(dE.max() <= 1e-10 and dU.max() <= 1e-10))
+    if len(diag):
+        ncorr_mean = float(np.mean(diag[:, 8][diag[:, 8] > 0])) if np.any(diag[:, 8] > 0) else 1.0
+
+    # per-kernel durations with HIP events on the library's stream
+    ctx.kernel_timing(True)
+    ctx.run(args.timing_steps)
+    kt = {}
+    for k in ("zfwd", "theta", "zbwd", "step", "curl", "control"):
+        ms, n = ctx.kernel_time(k)
+        if n:
+            kt[k] = dict(ms_total=ms, launches=n, us_avg=1000.0 * ms / n)
+    ctx.kernel_timing(False)
+    ab = alg_bytes(N, K, M)
+    ab["step"] = (ab["step_pred"] + ncorr_mean * ab["step_corr"]) / (1.0 + ncorr_mean)
+    dom = max(kt, key=lambda k: kt[k]["ms_total"])
+    achieved = ab[dom] / (kt[dom]["us_avg"] * 1e-6) / 1e9
+    ms_step = 1000.0 * secs / args.steps
+    B_alg = survey_bytes_per_step(N, K, M, ncorr_mean)
+    step_gbs = B_alg / (secs / args.steps) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(name, wl, fields, args.cpu_seconds)
+
+    out = {
+        "metric": "state-steps/s plain vs batched, p50/p99 request latency, at dE<=1e-10",
+        "value": value,
+        "unit": "state-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded divergence-free perturbation of Hagen-Poiseuille flow)",
+        "config": {"workload": f"plain single-state time step, configs[1] size "
+                               f"(N,K,M)=({N},{K},{M}), Re={Re}, alpha={alpha}, dt={dt}, E0={E0}",
+                   "N": N, "K": K, "M": M, "Re": Re, "alpha": alpha, "dt": dt,
+                   "corrector": "iterated to convergence (main.f90:35-41)",
+                   "mean_ncorr": ncorr_mean},
+        "parity": parity,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(dom, name),
+                     "alg_bytes_per_launch": ab[dom], "us_per_launch": kt[dom]["us_avg"]},
+        "step_roofline": {"B_alg_bytes": B_alg, "achieved_GBs": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_us": {k: round(v["us_avg"], 3) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * opf.h -- C ABI of the MI355X Openpipeflow time-step library (libopf.so).
+ *
+ * The reference's interface for this path is Fortran module procedures over
+ * module state (velocity.f90, timestep.f90, transform.fftw3.f90,
+ * variables.f90); these entry points are what a Fortran host binds with
+ * iso_c_binding to keep those procedures as thin wrappers.  Each comment
+ * names the reference procedure the entry point replaces.
+ *
+ * Conventions
+ *  - A context (opf_ctx) is one resolution class (N, K, M, Mp): the radial
+ *    mesh, FFT plans and device arrays for `capacity` independent states
+ *    ("slots").  A plain run uses one slot.
+ *  - Spectral fields cross the boundary in the reference's coll layout:
+ *    separate real and imaginary arrays Re(1:N, 0:H-1), Im(1:N, 0:H-1),
+ *    radial index fastest (variables.f90:37-40), H = (2K-1)M - (K-1),
+ *    harmonics ordered as parallel.h:47-56.
+ *  - Physical fields use the phys layout Re(0:3K-1, 0:3M-1, 1:N), z fastest
+ *    (variables.f90:42-44).
+ *  - Every call returns OPF_OK (0) or an error code; opf_last_error() gives a
+ *    message.  Calls on one context are not re-entrant.
+ *  - State stays in device memory between calls; only the explicit put/get
+ *    calls move fields across PCIe.
+ */
+#ifndef OPF_H
+#define OPF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct opf_ctx opf_ctx;
+
+enum {
+  OPF_OK = 0,
+  OPF_ERR_ARG = 1,        /* bad argument */
+  OPF_ERR_HIP = 2,        /* HIP runtime error / no device */
+  OPF_ERR_SINGULAR = 3,   /* 'tim_lumesh_init' / 'matrix inversion error' */
+  OPF_ERR_CORRECTOR = 4,  /* tim_check_cgce failure (too many its, ...) */
+  OPF_ERR_STATE = 5,      /* slot not set up / no state */
+  OPF_ERR_NOMEM = 6       /* device memory exhausted / admission refused */
+};
+
+/* Message of the last failing call on this thread. */
+const char* opf_last_error(void);
+int opf_version(void);
+
+/* ---- resolution class -------------------------------------------------- */
+
+/* Create a context: mes_precompute + var_precompute + tra_precompute
+ * (meshs.f90:66-103, variables.f90:60-88, transform.fftw3.f90:35-59) for a
+ * radial grid of N points (default mesh when r == NULL, else the points of
+ * mesh.in) and K axial / M azimuthal modes, with device storage for
+ * `capacity` states. */
+int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r);
+int opf_ctx_destroy(opf_ctx* ctx);
+/* H = number of harmonics; bytes = device memory held by the context. */
+int opf_ctx_dims(const opf_ctx* ctx, int* N, int* K, int* M, int* Mp, int* H, int* capacity,
+                 size_t* bytes);
+/* Device bytes a context of this shape would hold (admission control). */
+size_t opf_ctx_bytes_needed(int N, int K, int M, int capacity);
+
+/* mes_D of meshs.f90:48-58: r(1:N), intrdr(1:N), dr0(1:5,0:4), dr1(1:5,0:4)
+ * (column-major), dr(1)%M and radLap%M in mesh band storage M(9, N)
+ * (M(KL+1+n-j, j) = A(n, j), column-major).  Any pointer may be NULL. */
+int opf_mesh_get(const opf_ctx* ctx, double* r, double* intrdr, double* dr0, double* dr1,
+                 double* dr1band, double* radlap);
+
+/* Host-only mesh precompute (mes_precompute + mes_rdom_init, meshs.f90:66-194)
+ * without a context or device, same outputs as opf_mesh_get. */
+int opf_mesh_build(int N, const double* r_in, double* r, double* intrdr, double* dr0, double* dr1,
+                   double* dr1band, double* radlap);
+
+/* ---- per-state set-up ---------------------------------------------------- */
+
+/* vel_matrices (velocity.f90:120-146) for one slot: tim_lumesh_init of LNp,
+ * LDp, LDm, LDz (timestep.f90:67-96), the influence vectors and 4x4
+ * inverses of vel_adjPPE(0) (velocity.f90:191-225) and vel_adjustFlux(0)
+ * (:160-168), for Re, alpha, dt and the implicitness (d_implicit).  Also
+ * sets tim_dt.  Factorisation runs on the device. */
+int opf_slot_setup(opf_ctx* ctx, int slot, double Re, double alpha, double dt, double implicit,
+                   int const_flux);
+
+/* Load / read a state (vel_ur, vel_ut, vel_uz and tim_t); the layout of
+ * io_load_state after radial interpolation (io.f90:98-165).  put applies
+ * vel_imposesym (velocity.f90:61-74) and resets the step counter. */
+int opf_put_state(opf_ctx* ctx, int slot, const double* ur_re, const double* ur_im,
+                  const double* ut_re, const double* ut_im, const double* uz_re,
+                  const double* uz_im, double t);
+int opf_get_state(opf_ctx* ctx, int slot, double* ur_re, double* ur_im, double* ut_re,
+                  double* ut_im, double* uz_re, double* uz_im, double* t);
+
+/* ---- the time loop (main.f90:18-46), device resident ------------------------
+ * Advance every slot that holds a state by `nsteps` steps.  corr_fixed = 0
+ * iterates the corrector to convergence as main.f90:35-41 does
+ * (tim_check_cgce, timestep.f90:214-253); corr_fixed = n runs exactly n
+ * correctors per step (steporbit, newton.f90:743-787, uses 1).  diag_every
+ * > 0 records io_write_energy / io_write_friction quantities at the start of
+ * every diag_every-th step (io.f90:76-83 uses 10).  Predictor/corrector
+ * decisions are taken on the device; the host only enqueues iterations. */
+int opf_run(opf_ctx* ctx, int nsteps, int corr_fixed, int diag_every);
+/* Same for a subset of slots (slots[0..nslots)). */
+int opf_run_slots(opf_ctx* ctx, const int* slots, int nslots, int nsteps, int corr_fixed,
+                  int diag_every);
+
+/* tim_step, tim_t, failure status (0 ok), correctors of the last step,
+ * tim_dterr of the last corrector, vel_Pr0. */
+int opf_slot_status(opf_ctx* ctx, int slot, int* step, double* t, int* status, int* last_ncorr,
+                    double* last_dterr, double* Pr0);
+/* Diagnostics records, oldest first, 10 doubles each:
+ * step, t, E, Ek0, Em0, Ub (= vel_Pr0 with constant flux), Uc, u_tau, ncorr, dterr
+ * (vel_energy.dat / vel_friction.dat, io.f90:517,637). */
+int opf_diag_get(opf_ctx* ctx, int slot, double* recs, int maxrec, int* nrec);
+
+/* ---- reference procedures, one call each (Fortran drop-in) --------------- */
+
+int opf_vel_imposesym(opf_ctx* ctx, int slot);      /* vel_imposesym, velocity.f90:61-74 */
+int opf_vel_transform(opf_ctx* ctx, int slot);      /* vel_transform, velocity.f90:304-317 */
+int opf_vel_nonlinear(opf_ctx* ctx, int slot);      /* vel_nonlinear, velocity.f90:323-335 */
+int opf_vel_predictor(opf_ctx* ctx, int slot);      /* vel_predictor, velocity.f90:400-410 */
+/* vel_corrector (velocity.f90:416-432); returns sqrt(max|du|^2) as
+ * tim_measurecorr computes it (timestep.f90:190-207). */
+int opf_vel_corrector(opf_ctx* ctx, int slot, double* dterr);
+/* vel_Nr, vel_Nt, vel_Nz after vel_nonlinear, base-flow terms included
+ * (vel_addHPF, velocity.f90:338-361); coll layout. */
+int opf_get_nonlinear(opf_ctx* ctx, int slot, double* nr_re, double* nr_im, double* nt_re,
+                      double* nt_im, double* nz_re, double* nz_im);
+
+/* tra_coll2phys = var_coll2spec + tra_spec2phys (variables.f90:184-197,
+ * transform.fftw3.f90:65-102) and tra_phys2coll = tra_phys2spec +
+ * var_spec2coll (transform.fftw3.f90:108-147, variables.f90:281-294), on
+ * host arrays of one field. */
+int opf_tra_coll2phys(opf_ctx* ctx, const double* c_re, const double* c_im, double* phys);
+int opf_tra_phys2coll(opf_ctx* ctx, const double* phys, double* c_re, double* c_im);
+
+/* tim_lumesh_init result of a slot (timestep.f90:67-96): which = 0 LDp,
+ * 1 LDm, 2 LDz, 3 LNp.  ab: lumesh%M(13, N) per harmonic (13*N*H doubles,
+ * column-major per harmonic), ipiv: lumesh%ipiv(N) per harmonic (1-based).
+ * Rows 1..KL (fill-in workspace) are returned as the factorisation left them. */
+int opf_tim_lumesh_get(opf_ctx* ctx, int slot, int which, double* ab, int* ipiv);
+/* tim_lumesh_invert (timestep.f90:102-123) with the slot's factor `which`,
+ * BC as for that matrix, on a host coll field (in place). */
+int opf_tim_lumesh_invert(opf_ctx* ctx, int slot, int which, double* b_re, double* b_im);
+
+/* ---- timing hooks --------------------------------------------------------- */
+
+/* Record HIP events around every launch of the named kernel on the
+ * context's stream ("step", "zfwd", "theta", "zbwd", "curl", "control");
+ * 0 disables.  opf_kernel_time returns the summed milliseconds and the
+ * launch count since the last reset. */
+int opf_kernel_timing(opf_ctx* ctx, int enable);
+int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPF_H */

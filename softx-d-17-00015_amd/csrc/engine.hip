@@ -1,0 +1,808 @@
+// Host orchestration and the C ABI (include/opf.h).
+//
+// One opf_ctx holds a resolution class: the host mesh, FFT plans, and the
+// device arrays of `capacity` slots.  The time loop of main.f90:18-46 runs as
+// a sequence of "iterations", each one nonlinear evaluation plus one
+// predictor or corrector (per slot, as the slot's control block says):
+//   k_zfwd -> k_theta -> k_zbwd -> k_step -> k_curl_next -> k_control
+// The corrector test (tim_check_cgce) runs in k_control on the device, so the
+// host only enqueues iterations and checks progress between chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/opf.h"
+#include "kernels.hpp"
+
+using namespace opf;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct OpfError : std::runtime_error {
+  int code;
+  OpfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      throw OpfError(OPF_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+template <typename F> int guard(F&& f) {
+  try {
+    f();
+    return OPF_OK;
+  } catch (const OpfError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return OPF_ERR_ARG;
+  }
+}
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Timing {
+  bool on = false;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+};
+
+}  // namespace
+
+struct opf_ctx {
+  int N, K, M, Mp, H, Hp, P, Z, T, cap;
+  int diag_cap = 256;
+  HostMesh mesh;
+  hipStream_t stream = nullptr;
+  double* d_mesh = nullptr;
+  Bufs B{};
+  Geo G{};
+  FftTiles tiles{};
+  size_t smem_z = 0, smem_t = 0;
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  std::vector<int> occupied;   // slot holds a state
+  std::vector<int> setup;      // slot matrices built
+  // granular transform temporaries (allocated on first use)
+  double2* tY = nullptr;
+  double* tP = nullptr;
+  double2* tC = nullptr;
+  Timing tm;
+
+  template <typename T> T* alloc(size_t n) {
+    void* p = nullptr;
+    const size_t b = round_up(std::max<size_t>(n, 1) * sizeof(T), 256);
+    HIPCHK(hipMalloc(&p, b));
+    HIPCHK(hipMemset(p, 0, b));
+    allocs.push_back(p);
+    bytes += b;
+    return (T*)p;
+  }
+};
+
+namespace {
+
+// -------------------------------------------------------------------------
+// shapes
+// -------------------------------------------------------------------------
+struct Shape {
+  int H, Hp, P, Z, T;
+  size_t coll;
+};
+Shape shape_of(int N, int K, int M) {
+  Shape s;
+  s.H = (2 * K - 1) * M - (K - 1);
+  s.Hp = (int)round_up((size_t)s.H, 64);
+  s.P = K * M;
+  s.Z = 3 * K;
+  s.T = 3 * M;
+  s.coll = (size_t)N * s.Hp;
+  return s;
+}
+
+size_t bytes_needed(int N, int K, int M, int cap, int diag_cap) {
+  const Shape s = shape_of(N, K, M);
+  size_t b = 0;
+  const size_t c3 = (size_t)cap * 3 * s.coll * sizeof(double2);
+  b += c3 * 7;                                               // U[3], Nraw[2], om, R
+  b += (size_t)cap * s.coll * sizeof(double2);               // S
+  b += (size_t)cap * 4 * N * NLU * s.P * sizeof(double);     // LU
+  b += (size_t)cap * 4 * N * s.P;                            // PV
+  b += (size_t)cap * 6 * N * s.P * sizeof(double);           // Uinf
+  b += (size_t)cap * 16 * s.H * sizeof(double);              // Ainv
+  b += (size_t)cap * 3 * s.Hp * sizeof(double);              // diagb
+  b += (size_t)cap * diag_cap * sizeof(DiagRec);
+  b += (size_t)cap * N * s.Z * 9 * M * sizeof(double2);      // Yf + Yb
+  return b;
+}
+
+FftTiles choose_tiles(int Z, int T, int M) {
+  FftTiles t{};
+  t.pz = make_plan(Z);
+  t.pt = make_plan(T);
+  // LDS budget 48 KiB per workgroup: tm*Z and 3*tz*T complex values
+  t.tm = std::max(1, std::min(M, 3072 / Z));
+  t.tz = std::max(1, std::min(Z, 1024 / T));
+  // every Stockham pass must fit FFT_MAXB butterflies per thread (radix 2 worst)
+  const int cap = FFT_MAXB * FFT_THREADS;
+  if (t.tm * Z / 2 > cap || 3 * t.tz * T / 2 > cap || 2 * T / 2 > cap)
+    throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
+  return t;
+}
+
+void check_slot(opf_ctx* c, int slot) {
+  if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+  if (slot < 0 || slot >= c->cap) throw OpfError(OPF_ERR_ARG, "slot out of range");
+}
+
+std::vector<SlotCtl> read_ctl(opf_ctx* c) {
+  std::vector<SlotCtl> v(c->cap);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(v.data(), c->B.ctl, sizeof(SlotCtl) * c->cap, hipMemcpyDeviceToHost));
+  return v;
+}
+void write_ctl(opf_ctx* c, const std::vector<SlotCtl>& v) {
+  HIPCHK(hipMemcpy(c->B.ctl, v.data(), sizeof(SlotCtl) * c->cap, hipMemcpyHostToDevice));
+}
+
+// Launch helper with optional event timing.
+template <typename Kern, typename... Args>
+void launch(opf_ctx* c, const char* name, Kern k, dim3 grid, dim3 block, size_t smem, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->tm.on) {
+    a = c->tm.get();
+    b = c->tm.get();
+    HIPCHK(hipEventRecord(a, c->stream));
+  }
+  hipLaunchKernelGGL(k, grid, block, smem, c->stream, args...);
+  HIPCHK(hipGetLastError());
+  if (c->tm.on) {
+    HIPCHK(hipEventRecord(b, c->stream));
+    c->tm.recs.push_back({name, a, b});
+  }
+}
+
+dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); }
+
+// One evaluation (vel_transform + vel_nonlinear without the base-flow
+// terms) for slots [s0, s0+ns).
+void launch_eval(opf_ctx* c, int s0, int ns) {
+  const FftTiles& t = c->tiles;
+  launch(c, "zfwd", k_zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, ns), dim3(FFT_THREADS), c->smem_z,
+         c->G, c->B, t, s0);
+  launch(c, "theta", k_theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
+         c->G, c->B, t, s0);
+  launch(c, "zbwd", k_zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, ns), dim3(FFT_THREADS), c->smem_z,
+         c->G, c->B, t, s0);
+}
+
+void launch_step(opf_ctx* c, int s0, int ns) {
+  launch(c, "step", k_step, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0);
+  launch(c, "curl", k_curl_next, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0);
+  launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
+}
+
+void launch_iteration(opf_ctx* c, int s0, int ns) {
+  launch_eval(c, s0, ns);
+  launch_step(c, s0, ns);
+}
+
+// coll (reference layout) <-> device rows
+void coll_to_dev(opf_ctx* c, const double* re, const double* im, double2* dst) {
+  std::vector<double2> h((size_t)c->N * c->Hp, make_double2(0.0, 0.0));
+  for (int hh = 0; hh < c->H; ++hh)
+    for (int n = 0; n < c->N; ++n)
+      h[(size_t)n * c->Hp + hh] = make_double2(re[(size_t)hh * c->N + n], im ? im[(size_t)hh * c->N + n] : 0.0);
+  HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+void dev_to_coll(opf_ctx* c, const double2* src, double* re, double* im) {
+  std::vector<double2> h((size_t)c->N * c->Hp);
+  HIPCHK(hipMemcpyAsync(h.data(), src, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int hh = 0; hh < c->H; ++hh)
+    for (int n = 0; n < c->N; ++n) {
+      const double2 v = h[(size_t)n * c->Hp + hh];
+      if (re) re[(size_t)hh * c->N + n] = v.x;
+      if (im) im[(size_t)hh * c->N + n] = v.y;
+    }
+}
+
+void ensure_temps(opf_ctx* c) {
+  if (c->tY) return;
+  c->tY = c->alloc<double2>((size_t)c->N * c->Z * c->M);
+  c->tP = c->alloc<double>((size_t)c->N * c->Z * c->T);
+  c->tC = c->alloc<double2>(c->G.coll);
+}
+
+std::vector<double2> twiddles(int L) {
+  std::vector<double2> tw(L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int q = 0; q < L; ++q) {
+    const long double a = 2.0L * pi * (long double)q / (long double)L;
+    tw[q] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return tw;
+}
+
+void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
+  if (N < 9 || N > MAXN) throw OpfError(OPF_ERR_ARG, "N must be in [9, 128]");
+  if (K < 2 || M < 1 || Mp < 1 || cap < 1) throw OpfError(OPF_ERR_ARG, "bad K, M, Mp or capacity");
+  if (M > 256) throw OpfError(OPF_ERR_ARG, "M must be <= 256");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw OpfError(OPF_ERR_HIP, "no HIP device");
+  c->N = N; c->K = K; c->M = M; c->Mp = Mp; c->cap = cap;
+  const Shape s = shape_of(N, K, M);
+  c->H = s.H; c->Hp = s.Hp; c->P = s.P; c->Z = s.Z; c->T = s.T;
+  build_mesh(N, r, c->mesh);
+  c->tiles = choose_tiles(c->Z, c->T, M);
+  c->smem_z = (size_t)c->tiles.tm * c->Z * sizeof(double2);
+  c->smem_t = (size_t)3 * c->tiles.tz * c->T * sizeof(double2);
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // device mesh block
+  const MeshOff mo = MeshOff::make(N);
+  std::vector<double> hm(mo.total, 0.0);
+  const HostMesh& me = c->mesh;
+  for (int n = 0; n < N; ++n) {
+    hm[mo.r + n] = me.r[n];
+    hm[mo.rinv + n] = me.rinv[n];
+    hm[mo.rinv2 + n] = me.rinv2[n];
+    hm[mo.Ubase + n] = 1.0 - me.r2[n];        // vel_U = 1 - r^2 (velocity.f90:53)
+    hm[mo.twor + n] = -(-2.0 * me.r[n]);      // -vel_Up, vel_Up = -2r (velocity.f90:54, 342)
+    hm[mo.intrdr + n] = me.intrdr[n];
+  }
+  std::copy(me.dr1b.begin(), me.dr1b.end(), hm.begin() + mo.dr1b);
+  std::copy(me.lapb.begin(), me.lapb.end(), hm.begin() + mo.lapb);
+  std::copy(me.dr0.begin(), me.dr0.end(), hm.begin() + mo.dr0);
+  std::copy(me.dr1.begin(), me.dr1.end(), hm.begin() + mo.dr1);
+  c->d_mesh = c->alloc<double>(mo.total);
+  HIPCHK(hipMemcpy(c->d_mesh, hm.data(), hm.size() * sizeof(double), hipMemcpyHostToDevice));
+
+  Bufs& B = c->B;
+  const size_t c3 = (size_t)cap * 3 * s.coll;
+  for (int i = 0; i < 3; ++i) B.U[i] = c->alloc<double2>(c3);
+  for (int i = 0; i < 2; ++i) B.Nraw[i] = c->alloc<double2>(c3);
+  B.om = c->alloc<double2>(c3);
+  B.R = c->alloc<double2>(c3);
+  B.S = c->alloc<double2>((size_t)cap * s.coll);
+  B.LU = c->alloc<double>((size_t)cap * 4 * N * NLU * s.P);
+  B.PV = c->alloc<unsigned char>((size_t)cap * 4 * N * s.P);
+  B.Uinf = c->alloc<double>((size_t)cap * 6 * N * s.P);
+  B.Ainv = c->alloc<double>((size_t)cap * 16 * s.H);
+  B.Ui = c->alloc<double>((size_t)cap * N);
+  B.diagb = c->alloc<double>((size_t)cap * 3 * s.Hp);
+  B.diag = c->alloc<DiagRec>((size_t)cap * c->diag_cap);
+  B.prm = c->alloc<SlotParams>(cap);
+  B.ctl = c->alloc<SlotCtl>(cap);
+  B.Yf = c->alloc<double2>((size_t)cap * N * s.Z * 6 * M);
+  B.Yb = c->alloc<double2>((size_t)cap * N * s.Z * 3 * M);
+  B.twZ = c->alloc<double2>(s.Z);
+  B.twT = c->alloc<double2>(s.T);
+  auto tz = twiddles(s.Z), tt = twiddles(s.T);
+  HIPCHK(hipMemcpy(B.twZ, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(B.twT, tt.data(), tt.size() * sizeof(double2), hipMemcpyHostToDevice));
+
+  Geo& G = c->G;
+  G.N = N; G.K = K; G.M = M; G.Mp = Mp; G.H = s.H; G.Hp = s.Hp; G.P = s.P; G.Z = s.Z; G.T = s.T;
+  G.coll = s.coll;
+  G.mesh = c->d_mesh;
+
+  std::vector<SlotCtl> ctl(cap);
+  for (int i = 0; i < cap; ++i) {
+    SlotCtl& k = ctl[i];
+    std::memset(&k, 0, sizeof(k));
+    k.ubase = 2; k.ucur = 0; k.unext = 1;
+    k.ncur = 0; k.npred = 1;
+    k.target_steps = -1;
+    k.diag_cap = c->diag_cap;
+    k.lasterr = 1e99;
+  }
+  write_ctl(c, ctl);
+  c->occupied.assign(cap, 0);
+  c->setup.assign(cap, 0);
+}
+
+void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
+  check_slot(c, slot);
+  if (!(dt > 0.0) || !(Re > 0.0) || alpha == 0.0) throw OpfError(OPF_ERR_ARG, "Re, dt must be > 0 and alpha != 0");
+  SlotParams p{};
+  p.Re = Re; p.alpha = alpha; p.dt = dt; p.implicit = implicit; p.nu = 1.0;
+  p.dterr_tol = 1e-5;   // d_dterr (parameters.f90:40)
+  p.courant = 0.5;      // d_courant (parameters.f90:41)
+  p.const_flux = cf;
+  HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
+  auto ctl = read_ctl(c);
+  ctl[slot].status = 0;
+  write_ctl(c, ctl);
+  const dim3 gp((c->P + 63) / 64, 4, 1);
+  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot);
+  launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
+  launch(c, "ainv", k_ainv, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
+  ctl = read_ctl(c);
+  if (ctl[slot].status != 0) {
+    const int st = ctl[slot].status;
+    ctl[slot].status = 0;
+    write_ctl(c, ctl);
+    throw OpfError(OPF_ERR_SINGULAR, st < 110 ? "tim_lumesh_init: singular factor"
+                                              : (st == 110 ? "vel_adjustFlux: err 2" : "matrix inversion error"));
+  }
+  c->setup[slot] = 1;
+}
+
+void put_state(opf_ctx* c, int slot, const double* const* f, double t) {
+  check_slot(c, slot);
+  if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (opf_slot_setup)");
+  auto ctl = read_ctl(c);
+  SlotCtl& k = ctl[slot];
+  double2* u = c->B.U[k.ucur] + (size_t)slot * 3 * c->G.coll;
+  for (int comp = 0; comp < 3; ++comp) coll_to_dev(c, f[2 * comp], f[2 * comp + 1], u + comp * c->G.coll);
+  k.step = 0;
+  k.t = t;
+  k.mode = MODE_PRED;
+  k.it = 0;
+  k.ncorr = 0;
+  k.status = 0;
+  k.ndiag = 0;
+  k.active = 1;
+  k.dterr_bits = 0;
+  write_ctl(c, ctl);
+  launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
+  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  ctl = read_ctl(c);
+  ctl[slot].active = 0;
+  write_ctl(c, ctl);
+  c->occupied[slot] = 1;
+}
+
+void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
+  if (nsteps < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
+  auto ctl = read_ctl(c);
+  int lo = c->cap, hi = -1;
+  for (int s : slots) {
+    check_slot(c, s);
+    if (!c->occupied[s]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    SlotCtl& k = ctl[s];
+    if (k.status != 0) throw OpfError(OPF_ERR_CORRECTOR, "slot failed earlier");
+    if (nsteps == 0) continue;
+    k.active = 1;
+    k.mode = MODE_PRED;
+    k.target_steps = k.step + nsteps;
+    k.corr_fixed = corr_fixed;
+    k.diag_every = diag_every;
+    k.host_driven = 0;
+    lo = std::min(lo, s);
+    hi = std::max(hi, s);
+  }
+  if (hi < 0) return;
+  write_ctl(c, ctl);
+  const int ns = hi - lo + 1;
+  // Most steps take one corrector; enqueue 2 iterations per step, then top
+  // up until every slot has reached its target.
+  long chunk = 2L * nsteps;
+  for (int round = 0; round < 64; ++round) {
+    for (long i = 0; i < chunk; ++i) launch_iteration(c, lo, ns);
+    ctl = read_ctl(c);
+    int remaining = 0;
+    bool any = false;
+    for (int s : slots) {
+      if (ctl[s].active) {
+        any = true;
+        remaining = std::max(remaining, ctl[s].target_steps - ctl[s].step);
+      }
+    }
+    if (!any) break;
+    chunk = std::max(2L, 2L * remaining);
+  }
+  for (int s : slots)
+    if (ctl[s].status != 0) {
+      char msg[128];
+      std::snprintf(msg, sizeof msg, "tim_check_cgce failure %d in slot %d at step %d", ctl[s].status, s, ctl[s].step);
+      throw OpfError(OPF_ERR_CORRECTOR, msg);
+    }
+}
+
+}  // namespace
+
+namespace {
+// run `fn` with the slot marked active (kernels skip inactive slots)
+template <typename F> void with_active(opf_ctx* c, int slot, int mode, F&& fn) {
+  check_slot(c, slot);
+  if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+  auto ctl = read_ctl(c);
+  ctl[slot].active = 1;
+  ctl[slot].target_steps = -1;
+  if (mode >= 0) ctl[slot].mode = mode;
+  ctl[slot].diag_every = 0;
+  ctl[slot].corr_fixed = 0;
+  ctl[slot].host_driven = 1;
+  write_ctl(c, ctl);
+  fn();
+  HIPCHK(hipStreamSynchronize(c->stream));
+  ctl = read_ctl(c);
+  ctl[slot].active = 0;
+  write_ctl(c, ctl);
+}
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+const char* opf_last_error(void) { return g_err.c_str(); }
+int opf_version(void) { return 100; }
+
+int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r) {
+  return guard([&] {
+    if (!out) throw OpfError(OPF_ERR_ARG, "null out");
+    *out = nullptr;
+    auto* c = new opf_ctx();
+    try {
+      create(c, N, K, M, Mp, capacity, r);
+    } catch (...) {
+      opf_ctx_destroy(c);
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int opf_ctx_destroy(opf_ctx* c) {
+  if (!c) return OPF_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  for (auto& r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : c->tm.pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return OPF_OK;
+}
+
+int opf_ctx_dims(const opf_ctx* c, int* N, int* K, int* M, int* Mp, int* H, int* cap, size_t* bytes) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    if (N) *N = c->N;
+    if (K) *K = c->K;
+    if (M) *M = c->M;
+    if (Mp) *Mp = c->Mp;
+    if (H) *H = c->H;
+    if (cap) *cap = c->cap;
+    if (bytes) *bytes = c->bytes;
+  });
+}
+
+size_t opf_ctx_bytes_needed(int N, int K, int M, int capacity) { return bytes_needed(N, K, M, capacity, 256); }
+
+namespace {
+void mesh_out(const HostMesh& me, double* r, double* intrdr, double* dr0, double* dr1, double* dr1band,
+              double* radlap) {
+  const int N = me.N;
+    if (r) std::copy(me.r.begin(), me.r.end(), r);
+    if (intrdr) std::copy(me.intrdr.begin(), me.intrdr.end(), intrdr);
+    if (dr0) std::copy(me.dr0.begin(), me.dr0.end(), dr0);
+    if (dr1) std::copy(me.dr1.begin(), me.dr1.end(), dr1);
+    // band [row][col] -> Fortran M(row+1, col+1) column-major: M[row + 9*col]
+    for (int row = 0; row < LDA; ++row)
+      for (int col = 0; col < N; ++col) {
+        if (dr1band) dr1band[row + LDA * col] = me.dr1b[(size_t)row * N + col];
+        if (radlap) radlap[row + LDA * col] = me.lapb[(size_t)row * N + col];
+      }
+}
+}  // namespace
+
+extern "C" int opf_mesh_get(const opf_ctx* c, double* r, double* intrdr, double* dr0, double* dr1,
+                            double* dr1band, double* radlap) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    mesh_out(c->mesh, r, intrdr, dr0, dr1, dr1band, radlap);
+  });
+}
+
+extern "C" int opf_mesh_build(int N, const double* r_in, double* r, double* intrdr, double* dr0, double* dr1,
+                              double* dr1band, double* radlap) {
+  return guard([&] {
+    HostMesh me;
+    try {
+      build_mesh(N, r_in, me);
+    } catch (const std::runtime_error& e) {
+      throw OpfError(OPF_ERR_ARG, e.what());
+    }
+    mesh_out(me, r, intrdr, dr0, dr1, dr1band, radlap);
+  });
+}
+
+int opf_slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
+  return guard([&] { slot_setup(c, slot, Re, alpha, dt, implicit, cf); });
+}
+
+int opf_put_state(opf_ctx* c, int slot, const double* urr, const double* uri, const double* utr,
+                  const double* uti, const double* uzr, const double* uzi, double t) {
+  return guard([&] {
+    const double* f[6] = {urr, uri, utr, uti, uzr, uzi};
+    for (auto p : f)
+      if (!p) throw OpfError(OPF_ERR_ARG, "null field");
+    put_state(c, slot, f, t);
+  });
+}
+
+int opf_get_state(opf_ctx* c, int slot, double* urr, double* uri, double* utr, double* uti, double* uzr,
+                  double* uzi, double* t) {
+  return guard([&] {
+    check_slot(c, slot);
+    auto ctl = read_ctl(c);
+    const double2* u = c->B.U[ctl[slot].ucur] + (size_t)slot * 3 * c->G.coll;
+    double* f[6] = {urr, uri, utr, uti, uzr, uzi};
+    for (int comp = 0; comp < 3; ++comp) dev_to_coll(c, u + comp * c->G.coll, f[2 * comp], f[2 * comp + 1]);
+    if (t) *t = ctl[slot].t;
+  });
+}
+
+int opf_run(opf_ctx* c, int nsteps, int corr_fixed, int diag_every) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    std::vector<int> slots;
+    for (int s = 0; s < c->cap; ++s)
+      if (c->occupied[s]) slots.push_back(s);
+    run_slots(c, slots, nsteps, corr_fixed, diag_every);
+  });
+}
+
+int opf_run_slots(opf_ctx* c, const int* slots, int nslots, int nsteps, int corr_fixed, int diag_every) {
+  return guard([&] {
+    if (!c || (!slots && nslots > 0)) throw OpfError(OPF_ERR_ARG, "null argument");
+    run_slots(c, std::vector<int>(slots, slots + nslots), nsteps, corr_fixed, diag_every);
+  });
+}
+
+int opf_slot_status(opf_ctx* c, int slot, int* step, double* t, int* status, int* last_ncorr, double* last_dterr,
+                    double* Pr0) {
+  return guard([&] {
+    check_slot(c, slot);
+    auto ctl = read_ctl(c);
+    const SlotCtl& k = ctl[slot];
+    if (step) *step = k.step;
+    if (t) *t = k.t;
+    if (status) *status = k.status;
+    if (last_ncorr) *last_ncorr = k.last_ncorr;
+    if (last_dterr) *last_dterr = k.last_dterr;
+    if (Pr0) *Pr0 = k.Pr0;
+  });
+}
+
+int opf_diag_get(opf_ctx* c, int slot, double* recs, int maxrec, int* nrec) {
+  return guard([&] {
+    check_slot(c, slot);
+    auto ctl = read_ctl(c);
+    const int nd = ctl[slot].ndiag;
+    const int have = std::min(nd, c->diag_cap);
+    std::vector<DiagRec> all(c->diag_cap);
+    HIPCHK(hipMemcpy(all.data(), c->B.diag + (size_t)slot * c->diag_cap, sizeof(DiagRec) * c->diag_cap,
+                     hipMemcpyDeviceToHost));
+    const int n = std::min(have, maxrec);
+    const int first = nd - have + (have - n);
+    for (int i = 0; i < n; ++i) {
+      const DiagRec& r = all[(first + i) % c->diag_cap];
+      std::memcpy(recs + 10 * i, &r, sizeof(DiagRec));
+    }
+    if (nrec) *nrec = n;
+  });
+}
+
+// ---- granular reference procedures -----------------------------------------
+
+int opf_vel_imposesym(opf_ctx* c, int slot) {
+  return guard([&] {
+    check_slot(c, slot);
+    launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+
+int opf_vel_transform(opf_ctx* c, int slot) {
+  return guard([&] {
+    with_active(c, slot, -1, [&] {
+      const FftTiles& t = c->tiles;
+      launch(c, "zfwd", k_zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+             c->B, t, slot);
+    });
+  });
+}
+
+int opf_vel_nonlinear(opf_ctx* c, int slot) {
+  return guard([&] {
+    with_active(c, slot, -1, [&] {
+      const FftTiles& t = c->tiles;
+      launch(c, "theta", k_theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, 1), dim3(FFT_THREADS), c->smem_t, c->G,
+             c->B, t, slot);
+      launch(c, "zbwd", k_zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+             c->B, t, slot);
+    });
+  });
+}
+
+int opf_vel_predictor(opf_ctx* c, int slot) {
+  return guard([&] { with_active(c, slot, MODE_PRED, [&] { launch_step(c, slot, 1); }); });
+}
+
+int opf_vel_corrector(opf_ctx* c, int slot, double* dterr) {
+  return guard([&] {
+    with_active(c, slot, MODE_CORR, [&] { launch_step(c, slot, 1); });
+    auto ctl = read_ctl(c);
+    if (dterr) *dterr = ctl[slot].last_dterr;
+
+  });
+}
+
+int opf_get_nonlinear(opf_ctx* c, int slot, double* nrr, double* nri, double* ntr, double* nti, double* nzr,
+                      double* nzi) {
+  // Nraw of the last evaluation plus the base-flow terms of vel_addHPF,
+  // added on the host from the current iterate (velocity.f90:338-361).
+  return guard([&] {
+    check_slot(c, slot);
+    auto ctl = read_ctl(c);
+    SlotParams prm;
+    HIPCHK(hipMemcpy(&prm, c->B.prm + slot, sizeof prm, hipMemcpyDeviceToHost));
+    const int N = c->N, H = c->H;
+    std::vector<double> f[12];
+    for (auto& v : f) v.assign((size_t)N * H, 0.0);
+    const double2* nr = c->B.Nraw[ctl[slot].ncur] + (size_t)slot * 3 * c->G.coll;
+    const double2* u = c->B.U[ctl[slot].ucur] + (size_t)slot * 3 * c->G.coll;
+    for (int comp = 0; comp < 3; ++comp) {
+      dev_to_coll(c, nr + comp * c->G.coll, f[2 * comp].data(), f[2 * comp + 1].data());
+      dev_to_coll(c, u + comp * c->G.coll, f[6 + 2 * comp].data(), f[6 + 2 * comp + 1].data());
+    }
+    const HostMesh& me = c->mesh;
+    double Pr0 = 0.0;
+    for (int hh = 0; hh < H; ++hh) {
+      int k, m;
+      if (hh < c->K) { m = 0; k = hh; }
+      else { const int q = hh - c->K; m = 1 + q / (2 * c->K - 1); k = q % (2 * c->K - 1) - (c->K - 1); }
+      (void)m;
+      for (int n = 0; n < N; ++n) {
+        const size_t o = (size_t)hh * N + n;
+        const double a = (prm.alpha * (double)k) * (1.0 - me.r2[n]);
+        const double b = 2.0 * me.r[n];
+        f[0][o] = f[0][o] + a * f[7][o];
+        f[1][o] = f[1][o] - a * f[6][o];
+        f[2][o] = f[2][o] + a * f[9][o];
+        f[3][o] = f[3][o] - a * f[8][o];
+        f[4][o] = (f[4][o] + a * f[11][o]) + b * f[6][o];
+        f[5][o] = (f[5][o] - a * f[10][o]) + b * f[7][o];
+      }
+    }
+    if (prm.const_flux) {
+      double s = 0.0;
+      for (int i = 0; i <= KL; ++i) s = s + f[10][(size_t)(N - 1 - KL + i)] * me.dr1[i + (KL + 1)];
+      Pr0 = s / (-2.0);
+      for (int n = 0; n < N; ++n) f[4][n] = f[4][n] + (4.0 * Pr0) / prm.Re;
+    }
+    double* outs[6] = {nrr, nri, ntr, nti, nzr, nzi};
+    for (int i = 0; i < 6; ++i)
+      if (outs[i]) std::copy(f[i].begin(), f[i].end(), outs[i]);
+  });
+}
+
+int opf_tra_coll2phys(opf_ctx* c, const double* cre, const double* cim, double* phys) {
+  return guard([&] {
+    if (!c || !cre || !phys) throw OpfError(OPF_ERR_ARG, "null argument");
+    ensure_temps(c);
+    coll_to_dev(c, cre, cim, c->tC);
+    const FftTiles& t = c->tiles;
+    launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+           c->B, t, (const double2*)c->tC, c->tY);
+    launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), c->T * sizeof(double2), c->G,
+           c->B, t, (const double2*)c->tY, c->tP);
+    HIPCHK(hipMemcpyAsync(phys, c->tP, sizeof(double) * c->N * c->Z * c->T, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int opf_tra_phys2coll(opf_ctx* c, const double* phys, double* cre, double* cim) {
+  return guard([&] {
+    if (!c || !phys) throw OpfError(OPF_ERR_ARG, "null argument");
+    ensure_temps(c);
+    HIPCHK(hipMemcpyAsync(c->tP, phys, sizeof(double) * c->N * c->Z * c->T, hipMemcpyHostToDevice, c->stream));
+    const FftTiles& t = c->tiles;
+    HIPCHK(hipMemsetAsync(c->tC, 0, sizeof(double2) * c->G.coll, c->stream));
+    launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), c->T * sizeof(double2), c->G,
+           c->B, t, (const double*)c->tP, c->tY);
+    launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+           c->B, t, (const double2*)c->tY, c->tC);
+    dev_to_coll(c, c->tC, cre, cim);
+  });
+}
+
+int opf_tim_lumesh_get(opf_ctx* c, int slot, int which, double* ab, int* ipiv) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (which < 0 || which > 3) throw OpfError(OPF_ERR_ARG, "which must be 0..3");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up");
+    const int N = c->N, P = c->P, H = c->H;
+    std::vector<double> F((size_t)N * NLU * P);
+    std::vector<unsigned char> pv((size_t)N * P);
+    HIPCHK(hipMemcpy(F.data(), c->B.LU + ((size_t)(slot * 4 + which) * N) * NLU * P, F.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pv.data(), c->B.PV + ((size_t)(slot * 4 + which) * N) * P, pv.size(), hipMemcpyDeviceToHost));
+    for (int hh = 0; hh < H; ++hh) {
+      int k, m;
+      if (hh < c->K) { m = 0; k = hh; }
+      else { const int q = hh - c->K; m = 1 + q / (2 * c->K - 1); k = q % (2 * c->K - 1) - (c->K - 1); }
+      const int p = m * c->K + std::abs(k);
+      for (int n = 0; n < N; ++n) {
+        for (int row = 0; row < NLU; ++row)
+          if (ab) ab[((size_t)hh * N + n) * NLU + row] = F[((size_t)n * NLU + row) * P + p];
+        if (ipiv) ipiv[(size_t)hh * N + n] = n + 1 + pv[(size_t)n * P + p];
+      }
+    }
+  });
+}
+
+}  // extern "C"
+
+// tim_lumesh_invert on a host field: solved on the device, lane per harmonic.
+namespace opf {
+__global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, double2* x);
+}
+
+extern "C" int opf_tim_lumesh_invert(opf_ctx* c, int slot, int which, double* bre, double* bim) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (which < 0 || which > 3) throw OpfError(OPF_ERR_ARG, "which must be 0..3");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up");
+    if (!bre || !bim) throw OpfError(OPF_ERR_ARG, "null field");
+    ensure_temps(c);
+    coll_to_dev(c, bre, bim, c->tC);
+    launch(c, "lumesh_invert", k_lumesh_invert, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, which, c->tC);
+    dev_to_coll(c, c->tC, bre, bim);
+  });
+}
+
+extern "C" int opf_kernel_timing(opf_ctx* c, int enable) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& r : c->tm.recs) { c->tm.pool.push_back(r.a); c->tm.pool.push_back(r.b); }
+    c->tm.recs.clear();
+    c->tm.on = enable != 0;
+  });
+}
+
+extern "C" int opf_kernel_time(opf_ctx* c, const char* name, double* ms, int* launches) {
+  return guard([&] {
+    if (!c || !name) throw OpfError(OPF_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int cnt = 0;
+    for (auto& r : c->tm.recs)
+      if (r.name == name) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        tot += t;
+        ++cnt;
+      }
+    if (ms) *ms = tot;
+    if (launches) *launches = cnt;
+  });
+}

@@ -1,0 +1,4 @@
+"""MI355X Openpipeflow time-step path: Python host mirror of the reference
+interface over the HIP library libopf.so (include/opf.h)."""
+from .engine import Context, bytes_needed, host_mesh, LDP, LDM, LDZ, LNP, DIAG_FIELDS  # noqa: F401
+from . import ic  # noqa: F401

@@ -1,0 +1,84 @@
+"""ctypes binding of libopf.so (include/opf.h).
+
+The library is loaded from this directory; if it is missing the import fails
+loudly - there is no CPU fallback for the time step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libopf.so")
+
+OPF_OK = 0
+ERRORS = {1: "OPF_ERR_ARG", 2: "OPF_ERR_HIP", 3: "OPF_ERR_SINGULAR", 4: "OPF_ERR_CORRECTOR",
+          5: "OPF_ERR_STATE", 6: "OPF_ERR_NOMEM"}
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+# name: (restype, argtypes)
+PROTOTYPES = {
+    "opf_last_error": (C.c_char_p, []),
+    "opf_version": (C.c_int, []),
+    "opf_ctx_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+    "opf_ctx_destroy": (C.c_int, [vp]),
+    "opf_ctx_dims": (C.c_int, [vp, ip, ip, ip, ip, ip, ip, C.POINTER(C.c_size_t)]),
+    "opf_ctx_bytes_needed": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "opf_mesh_get": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+    "opf_mesh_build": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+    "opf_slot_setup": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "opf_put_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double]),
+    "opf_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+    "opf_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "opf_run_slots": (C.c_int, [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "opf_slot_status": (C.c_int, [vp, C.c_int, ip, dp, ip, ip, dp, dp]),
+    "opf_diag_get": (C.c_int, [vp, C.c_int, dp, C.c_int, ip]),
+    "opf_vel_imposesym": (C.c_int, [vp, C.c_int]),
+    "opf_vel_transform": (C.c_int, [vp, C.c_int]),
+    "opf_vel_nonlinear": (C.c_int, [vp, C.c_int]),
+    "opf_vel_predictor": (C.c_int, [vp, C.c_int]),
+    "opf_vel_corrector": (C.c_int, [vp, C.c_int, dp]),
+    "opf_get_nonlinear": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
+    "opf_tra_coll2phys": (C.c_int, [vp, dp, dp, dp]),
+    "opf_tra_phys2coll": (C.c_int, [vp, dp, dp, dp]),
+    "opf_tim_lumesh_get": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
+    "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
+    "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
+    "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
+}
+
+
+class OpfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIBPATH):
+    """Load libopf.so and attach the prototypes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() "
+                          "(or python softx-d-17-00015_amd/opf/build.py)")
+    lib = C.CDLL(path)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != OPF_OK:
+        msg = _lib.opf_last_error().decode(errors="replace") if _lib else ""
+        raise OpfError(rc, msg)
+    return rc

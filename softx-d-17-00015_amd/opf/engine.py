@@ -1,0 +1,217 @@
+"""Python mirror of the reference's time-step interface over libopf.so.
+
+``Context`` is one resolution class (N, K, M, Mp) holding ``capacity``
+device-resident states.  Method names follow the reference procedures they
+stand for (velocity.f90, timestep.f90, transform.fftw3.f90, io.f90); spectral
+fields are complex numpy arrays ``a[n, h]`` of shape (N, H) (radial point,
+harmonic in parallel.h:47-56 order), physical fields real arrays
+``p[n, th, z]`` matching the reference's Re(0:3K-1, 0:3M-1, N) layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+LDP, LDM, LDZ, LNP = 0, 1, 2, 3
+DIAG_FIELDS = ("step", "t", "E", "Ek0", "Em0", "Ub", "Uc", "Ufr", "ncorr", "dterr")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _split(a, N, H):
+    a = np.asarray(a, dtype=np.complex128)
+    if a.shape != (N, H):
+        raise ValueError(f"field shape {a.shape}, expected {(N, H)}")
+    # coll layout Re(1:N, 0:H-1), radial index fastest
+    return np.ascontiguousarray(a.real.T), np.ascontiguousarray(a.imag.T)
+
+
+def _join(re, im):
+    return (re + 1j * im).T.copy()
+
+
+class Context:
+    def __init__(self, N, K, M, Mp=1, capacity=1, r=None):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rr = None
+        if r is not None:
+            rr = np.ascontiguousarray(r, dtype=np.float64)
+        check(self.lib.opf_ctx_create(C.byref(h), N, K, M, Mp, capacity,
+                                      _dptr(rr) if rr is not None else None))
+        self.h = h
+        dims = [C.c_int() for _ in range(6)]
+        nbytes = C.c_size_t()
+        check(self.lib.opf_ctx_dims(h, *[C.byref(d) for d in dims], C.byref(nbytes)))
+        self.N, self.K, self.M, self.Mp, self.H, self.capacity = [d.value for d in dims]
+        self.bytes = nbytes.value
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.opf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- mesh (mes_D) -------------------------------------------------------
+    def mesh(self):
+        N = self.N
+        r = np.zeros(N)
+        intrdr = np.zeros(N)
+        dr0 = np.zeros(25)
+        dr1 = np.zeros(25)
+        d1 = np.zeros(9 * N)
+        lap = np.zeros(9 * N)
+        check(self.lib.opf_mesh_get(self.h, _dptr(r), _dptr(intrdr), _dptr(dr0), _dptr(dr1),
+                                    _dptr(d1), _dptr(lap)))
+        return dict(r=r, intrdr=intrdr, dr0=dr0.reshape(5, 5).T, dr1=dr1.reshape(5, 5).T,
+                    dr1band=d1.reshape(N, 9).T, radlap=lap.reshape(N, 9).T)
+
+    # -- set-up and state ---------------------------------------------------------
+    def setup(self, slot, Re, alpha, dt, implicit=0.51, const_flux=True):
+        """vel_matrices (velocity.f90:120-146) for one slot."""
+        check(self.lib.opf_slot_setup(self.h, slot, float(Re), float(alpha), float(dt),
+                                      float(implicit), int(bool(const_flux))))
+
+    def put_state(self, slot, ur, ut, uz, t=0.0):
+        parts = []
+        for a in (ur, ut, uz):
+            parts.extend(_split(a, self.N, self.H))
+        check(self.lib.opf_put_state(self.h, slot, *[_dptr(p) for p in parts], float(t)))
+
+    def get_state(self, slot):
+        parts = [np.zeros((self.H, self.N)) for _ in range(6)]
+        t = C.c_double()
+        check(self.lib.opf_get_state(self.h, slot, *[_dptr(p) for p in parts], C.byref(t)))
+        return (_join(parts[0], parts[1]), _join(parts[2], parts[3]), _join(parts[4], parts[5]),
+                t.value)
+
+    # -- time loop ------------------------------------------------------------
+    def run(self, nsteps, corr_fixed=0, diag_every=0, slots=None):
+        """main.f90:18-46 for `nsteps` steps on the device."""
+        if slots is None:
+            check(self.lib.opf_run(self.h, int(nsteps), int(corr_fixed), int(diag_every)))
+        else:
+            arr = (C.c_int * len(slots))(*slots)
+            check(self.lib.opf_run_slots(self.h, arr, len(slots), int(nsteps), int(corr_fixed),
+                                         int(diag_every)))
+
+    def status(self, slot):
+        step = C.c_int()
+        t = C.c_double()
+        st = C.c_int()
+        nc = C.c_int()
+        de = C.c_double()
+        pr = C.c_double()
+        check(self.lib.opf_slot_status(self.h, slot, C.byref(step), C.byref(t), C.byref(st),
+                                       C.byref(nc), C.byref(de), C.byref(pr)))
+        return dict(step=step.value, t=t.value, status=st.value, last_ncorr=nc.value,
+                    last_dterr=de.value, Pr0=pr.value)
+
+    def diag(self, slot, maxrec=256):
+        buf = np.zeros((maxrec, 10))
+        n = C.c_int()
+        check(self.lib.opf_diag_get(self.h, slot, _dptr(buf), maxrec, C.byref(n)))
+        return buf[:n.value]
+
+    # -- reference procedures, one call each -------------------------------------
+    def imposesym(self, slot=0):
+        check(self.lib.opf_vel_imposesym(self.h, slot))
+
+    def transform(self, slot=0):
+        check(self.lib.opf_vel_transform(self.h, slot))
+
+    def nonlinear(self, slot=0):
+        check(self.lib.opf_vel_nonlinear(self.h, slot))
+
+    def predictor(self, slot=0):
+        check(self.lib.opf_vel_predictor(self.h, slot))
+
+    def corrector(self, slot=0):
+        d = C.c_double()
+        check(self.lib.opf_vel_corrector(self.h, slot, C.byref(d)))
+        return d.value
+
+    def get_nonlinear(self, slot=0):
+        parts = [np.zeros((self.H, self.N)) for _ in range(6)]
+        check(self.lib.opf_get_nonlinear(self.h, slot, *[_dptr(p) for p in parts]))
+        return (_join(parts[0], parts[1]), _join(parts[2], parts[3]), _join(parts[4], parts[5]))
+
+    def coll2phys(self, a):
+        """tra_coll2phys of one field -> phys[n, th, z]."""
+        re, im = _split(a, self.N, self.H)
+        Z, T = 3 * self.K, 3 * self.M
+        p = np.zeros((self.N, T, Z))
+        check(self.lib.opf_tra_coll2phys(self.h, _dptr(re), _dptr(im), _dptr(p)))
+        return p
+
+    def phys2coll(self, p):
+        Z, T = 3 * self.K, 3 * self.M
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        if p.shape != (self.N, T, Z):
+            raise ValueError("phys shape")
+        re = np.zeros((self.H, self.N))
+        im = np.zeros((self.H, self.N))
+        check(self.lib.opf_tra_phys2coll(self.h, _dptr(p), _dptr(re), _dptr(im)))
+        return _join(re, im)
+
+    def lumesh_get(self, slot, which):
+        """tim_lumesh_init result: (ab[h, 13, n], ipiv[h, n]) like the reference."""
+        ab = np.zeros((self.H, self.N, 13))
+        ipiv = np.zeros((self.H, self.N), dtype=np.int32)
+        check(self.lib.opf_tim_lumesh_get(self.h, slot, which, _dptr(ab),
+                                          ipiv.ctypes.data_as(C.POINTER(C.c_int))))
+        return ab.transpose(0, 2, 1).copy(), ipiv
+
+    def lumesh_invert(self, slot, which, b):
+        re, im = _split(b, self.N, self.H)
+        check(self.lib.opf_tim_lumesh_invert(self.h, slot, which, _dptr(re), _dptr(im)))
+        return _join(re, im)
+
+    # -- timing ------------------------------------------------------------------
+    def kernel_timing(self, enable=True):
+        check(self.lib.opf_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_int()
+        check(self.lib.opf_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def bytes_needed(N, K, M, capacity=1):
+    lib = _lib.load()
+    return int(lib.opf_ctx_bytes_needed(N, K, M, capacity))
+
+
+def host_mesh(N, r=None):
+    """mes_precompute on the host (no device needed): r, intrdr, dr0[q, i],
+    dr1[q, i], dr1band[row, col], radlap[row, col]."""
+    lib = _lib.load()
+    out = dict(r=np.zeros(N), intrdr=np.zeros(N), dr0=np.zeros(25), dr1=np.zeros(25),
+               dr1band=np.zeros(9 * N), radlap=np.zeros(9 * N))
+    rin = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    check(lib.opf_mesh_build(int(N), _dptr(rin) if rin is not None else None,
+                             *[_dptr(out[k]) for k in ("r", "intrdr", "dr0", "dr1", "dr1band", "radlap")]))
+    out["dr0"] = out["dr0"].reshape(5, 5).T.copy()
+    out["dr1"] = out["dr1"].reshape(5, 5).T.copy()
+    out["dr1band"] = out["dr1band"].reshape(N, 9).T.copy()
+    out["radlap"] = out["radlap"].reshape(N, 9).T.copy()
+    return out
