@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -78,6 +79,7 @@ struct opf_ctx {
   Bufs B{};
   Geo G{};
   FftTiles tiles{};
+  StepTiles st{};
   size_t smem_z = 0, smem_t = 0;
   std::vector<void*> allocs;
   size_t bytes = 0;
@@ -126,7 +128,7 @@ size_t bytes_needed(int N, int K, int M, int cap, int diag_cap) {
   const size_t c3 = (size_t)cap * 3 * s.coll * sizeof(double2);
   b += c3 * 7;                                               // U[3], Nraw[2], om, R
   b += (size_t)cap * s.coll * sizeof(double2);               // S
-  b += (size_t)cap * 4 * N * NLU * s.P * sizeof(double);     // LU
+  b += (size_t)cap * 4 * N * LUS * s.P * sizeof(double);     // LU
   b += (size_t)cap * 4 * N * s.P;                            // PV
   b += (size_t)cap * 6 * N * s.P * sizeof(double);           // Uinf
   b += (size_t)cap * 16 * s.H * sizeof(double);              // Ainv
@@ -140,13 +142,44 @@ FftTiles choose_tiles(int Z, int T, int M) {
   FftTiles t{};
   t.pz = make_plan(Z);
   t.pt = make_plan(T);
-  // LDS budget 48 KiB per workgroup: tm*Z and 3*tz*T complex values
-  t.tm = std::max(1, std::min(M, 3072 / Z));
-  t.tz = std::max(1, std::min(Z, 1024 / T));
+  // LDS budget 32 KiB per workgroup: tm*Z and 3*tz*T complex values, so that
+  // every radix-2 pass holds at most FFT_MAXB butterflies per thread
+  t.tm = std::max(1, std::min(M, 2048 / Z));
+  t.tz = std::max(1, std::min(Z, 2048 / (3 * T)));
   // every Stockham pass must fit FFT_MAXB butterflies per thread (radix 2 worst)
   const int cap = FFT_MAXB * FFT_THREADS;
   if (t.tm * Z / 2 > cap || 3 * t.tz * T / 2 > cap || 2 * T / 2 > cap)
     throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
+  return t;
+}
+
+size_t step_smem(int N, int gp, int cs) {
+  const int GH = 2 * gp;
+  const size_t FCP = (size_t)cs * LUS + 2;
+  size_t b = (size_t)7 * N * GH * sizeof(double2);          // V[4] + UB[3]
+  b += std::max((size_t)2 * 3 * gp * FCP * sizeof(double), (size_t)3 * N * GH * sizeof(double));
+  b += (size_t)GH * 8 * sizeof(double) + 8 * sizeof(double);
+  b += (size_t)4 * gp * N;
+  return round_up(b, 16);
+}
+
+StepTiles choose_step_tiles(int N, int P, int gp_req, int cs_req) {
+  StepTiles t{};
+  t.cs = cs_req > 0 ? cs_req : 16;
+  int gp = gp_req > 0 ? gp_req : 2;
+  gp = std::min(gp, 8);
+  auto fits = [&](int g) {
+    // LDS within the default 64 KiB and every loader thread within STEP_LOADMAX
+    const int nsolv = 2 * 2 * g * 3;
+    const int first = nsolv <= 64 ? 64 : ((nsolv + 63) / 64) * 64;
+    const int total = 3 * g * t.cs * (LUS / 2);
+    return step_smem(N, g, t.cs) <= 64 * 1024 && total <= STEP_LOADMAX * (STEP_THREADS - first);
+  };
+  while (gp > 1 && !fits(gp)) gp /= 2;
+  if (!fits(gp)) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit");
+  t.gp = gp;
+  t.ngroups = (P + gp - 1) / gp;
+  t.smem = step_smem(N, gp, t.cs);
   return t;
 }
 
@@ -197,8 +230,8 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
 }
 
 void launch_step(opf_ctx* c, int s0, int ns) {
-  launch(c, "step", k_step, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0);
-  launch(c, "curl", k_curl_next, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0);
+  const int per = (c->st.ngroups + 7) / 8;   // XCD-aware order inside the kernel
+  launch(c, "step", k_step, dim3(8 * per, 1, ns), dim3(STEP_THREADS), c->st.smem, c->G, c->B, c->st, s0);
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }
 
@@ -256,6 +289,11 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->H = s.H; c->Hp = s.Hp; c->P = s.P; c->Z = s.Z; c->T = s.T;
   build_mesh(N, r, c->mesh);
   c->tiles = choose_tiles(c->Z, c->T, M);
+  {
+    const char* e1 = std::getenv("OPF_STEP_GP");
+    const char* e2 = std::getenv("OPF_STEP_CS");
+    c->st = choose_step_tiles(N, c->P, e1 ? std::atoi(e1) : 0, e2 ? std::atoi(e2) : 0);
+  }
   c->smem_z = (size_t)c->tiles.tm * c->Z * sizeof(double2);
   c->smem_t = (size_t)3 * c->tiles.tz * c->T * sizeof(double2);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -285,7 +323,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   B.om = c->alloc<double2>(c3);
   B.R = c->alloc<double2>(c3);
   B.S = c->alloc<double2>((size_t)cap * s.coll);
-  B.LU = c->alloc<double>((size_t)cap * 4 * N * NLU * s.P);
+  B.LU = c->alloc<double>((size_t)cap * 4 * N * LUS * s.P);
   B.PV = c->alloc<unsigned char>((size_t)cap * 4 * N * s.P);
   B.Uinf = c->alloc<double>((size_t)cap * 6 * N * s.P);
   B.Ainv = c->alloc<double>((size_t)cap * 16 * s.H);
@@ -740,11 +778,12 @@ int opf_tim_lumesh_get(opf_ctx* c, int slot, int which, double* ab, int* ipiv) {
     if (which < 0 || which > 3) throw OpfError(OPF_ERR_ARG, "which must be 0..3");
     if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up");
     const int N = c->N, P = c->P, H = c->H;
-    std::vector<double> F((size_t)N * NLU * P);
-    std::vector<unsigned char> pv((size_t)N * P);
-    HIPCHK(hipMemcpy(F.data(), c->B.LU + ((size_t)(slot * 4 + which) * N) * NLU * P, F.size() * sizeof(double),
+    // per-pair blocks [pair][n][LUS] and [pair][n]
+    std::vector<double> F((size_t)P * N * LUS);
+    std::vector<unsigned char> pv((size_t)P * N);
+    HIPCHK(hipMemcpy(F.data(), lu_block(c->B, slot, which, 0, P, N), F.size() * sizeof(double),
                      hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(pv.data(), c->B.PV + ((size_t)(slot * 4 + which) * N) * P, pv.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pv.data(), pv_block(c->B, slot, which, 0, P, N), pv.size(), hipMemcpyDeviceToHost));
     for (int hh = 0; hh < H; ++hh) {
       int k, m;
       if (hh < c->K) { m = 0; k = hh; }
@@ -752,8 +791,8 @@ int opf_tim_lumesh_get(opf_ctx* c, int slot, int which, double* ab, int* ipiv) {
       const int p = m * c->K + std::abs(k);
       for (int n = 0; n < N; ++n) {
         for (int row = 0; row < NLU; ++row)
-          if (ab) ab[((size_t)hh * N + n) * NLU + row] = F[((size_t)n * NLU + row) * P + p];
-        if (ipiv) ipiv[(size_t)hh * N + n] = n + 1 + pv[(size_t)n * P + p];
+          if (ab) ab[((size_t)hh * N + n) * NLU + row] = F[((size_t)p * N + n) * LUS + row];
+        if (ipiv) ipiv[(size_t)hh * N + n] = n + 1 + pv[(size_t)p * N + n];
       }
     }
   });

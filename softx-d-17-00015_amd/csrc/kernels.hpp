@@ -13,8 +13,8 @@ struct Bufs {
   double2* om;          // curl of the current iterate [slot][3][N][Hp]
   double2* R;           // step scratch: right-hand sides [slot][3][N][Hp]
   double2* S;           // step scratch: pressure [slot][N][Hp]
-  double* LU;           // factors [slot][4][N][13][P]
-  unsigned char* PV;    // pivot offsets ipiv(j)-j [slot][4][N][P]
+  double* LU;           // factors [slot][4 mats][P pairs][N columns][LUS]
+  unsigned char* PV;    // pivot offsets ipiv(j)-j [slot][4][P][N]
   double* Uinf;         // influence vectors [slot][6][N][P]
   double* Ainv;         // inverted influence matrices [slot][16][H]
   double* Ui;           // flux correction vector [slot][N]
@@ -35,16 +35,34 @@ struct FftTiles {
   FftPlan pz, pt;
 };
 
+// Factor blocks: matrix `mat` of pair p, column n, LAPACK band row r at
+// lu_block(...)[n*LUS + r]; pivot offset at pv_block(...)[n].
+__host__ __device__ inline double* lu_block(const Bufs& B, int slot, int mat, int p, int P, int N) {
+  return B.LU + (((size_t)(slot * 4 + mat) * P + p) * N) * LUS;
+}
+__host__ __device__ inline unsigned char* pv_block(const Bufs& B, int slot, int mat, int p, int P, int N) {
+  return B.PV + ((size_t)(slot * 4 + mat) * P + p) * N;
+}
+
+// Shape of the fused step kernel: pairs per workgroup and factor chunk.
+struct StepTiles {
+  int gp;        // (|k|, m) pairs per workgroup (<= 8)
+  int cs;        // factor columns per LDS chunk
+  int ngroups;   // workgroups per slot
+  size_t smem;   // dynamic LDS bytes
+};
+constexpr int STEP_THREADS = 256;
+constexpr int STEP_LOADMAX = 4;  // factor double2 loaded per loader thread per chunk
+
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_MAXB = 8;      // butterflies held per thread per pass
+constexpr int FFT_MAXB = 4;      // butterflies held per thread per pass
 
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0);
 __global__ void k_influence(Geo G, Bufs B, int slot0);
 __global__ void k_ainv(Geo G, Bufs B, int slot0);
 __global__ void k_curl(Geo G, Bufs B, int slot0);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
-__global__ void k_step(Geo G, Bufs B, int slot0);
-__global__ void k_curl_next(Geo G, Bufs B, int slot0);
+__global__ void k_step(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
 // transforms

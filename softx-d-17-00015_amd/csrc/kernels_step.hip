@@ -59,10 +59,11 @@ __device__ __forceinline__ double2 band_row(const double* band, int N, int n,
 
 // ---------------------------------------------------------------------------
 // banded LU solve (LAPACK DGBTRS 'N', KL = KU = 4), one system per lane.
-// F: factor rows of this lane's matrix, element (n,row) at F[(n*13+row)*P];
-// pv: pivot offsets ipiv(j)-j at pv[n*P]; x: vector with stride xs, solved
-// in place.  T is double or double2 (real and imaginary parts solved with
-// the same factor, as tim_lumesh_invert does with two dgbtrs calls).
+// F: factor of this lane's matrix, column n at F[n*LUS] with the 13 rows of
+// LAPACK band storage AB(1:13, n) (rows 9..12 hold the multipliers); pv:
+// pivot offsets ipiv(j)-j; x: vector with stride xs, solved in place.  T is
+// double or double2 (real and imaginary parts solved with the same factor,
+// as tim_lumesh_invert does with two dgbtrs calls).
 // ---------------------------------------------------------------------------
 template <typename T> struct VT;
 template <> struct VT<double> {
@@ -87,14 +88,14 @@ template <> struct VT<double2> {
 };
 
 template <typename T>
-__device__ void lu_solve(const double* F, const unsigned char* pv, int P, T* x, size_t xs, int N) {
+__device__ void lu_solve(const double* F, const unsigned char* pv, T* x, size_t xs, int N) {
   using V = VT<T>;
   // L solve with row interchanges: window w[i] = b(j+i)
   T w[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) w[i] = i < N ? x[i * xs] : V::zero();
   for (int j = 0; j < N - 1; ++j) {
-    const int off = pv[(size_t)j * P];
+    const int off = pv[j];
     const T a0 = w[0];
     T s = w[0];
     s = off == 1 ? w[1] : s;
@@ -108,10 +109,10 @@ __device__ void lu_solve(const double* F, const unsigned char* pv, int P, T* x, 
     w[4] = off == 4 ? a0 : w[4];
     const T t = V::neg(w[0]);
     const int lm = N - 1 - j < KL ? N - 1 - j : KL;
-    const double* Fj = F + (size_t)j * NLU * P;
+    const double* Fj = F + (size_t)j * LUS;
 #pragma unroll
     for (int i = 1; i <= KL; ++i)
-      if (i <= lm) V::axpy(w[i], Fj[(size_t)(8 + i) * P], t);
+      if (i <= lm) V::axpy(w[i], Fj[8 + i], t);
     x[j * xs] = w[0];
 #pragma unroll
     for (int i = 0; i < 4; ++i) w[i] = w[i + 1];
@@ -127,13 +128,13 @@ __device__ void lu_solve(const double* F, const unsigned char* pv, int P, T* x, 
   }
   v[8] = w[0];
   for (int j = N - 1; j >= 0; --j) {
-    const double* Fj = F + (size_t)j * NLU * P;
-    v[8] = V::div(v[8], Fj[(size_t)8 * P]);
+    const double* Fj = F + (size_t)j * LUS;
+    v[8] = V::div(v[8], Fj[8]);
     const T t = v[8];
     x[j * xs] = t;
 #pragma unroll
     for (int d = 1; d <= 8; ++d)
-      if (j - d >= 0) V::sub_mul(v[8 - d], t, Fj[(size_t)(8 - d) * P]);
+      if (j - d >= 0) V::sub_mul(v[8 - d], t, Fj[8 - d]);
 #pragma unroll
     for (int q = 8; q >= 1; --q) v[q] = v[q - 1];
     v[0] = j - 9 >= 0 ? x[(j - 9) * xs] : V::zero();
@@ -258,9 +259,9 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
     c1 = 1.0 / prm.dt;
     c2 = -prm.implicit * prm.nu / prm.Re;
   }
-  double* F = B.LU + ((size_t)(slot * 4 + mat) * N) * NLU * P + p;
-  unsigned char* pv = B.PV + ((size_t)(slot * 4 + mat) * N) * P + p;
-  auto A = [&](int row, int col) -> double& { return F[((size_t)col * NLU + row) * P]; };
+  double* F = lu_block(B, slot, mat, p, P, N);
+  unsigned char* pv = pv_block(B, slot, mat, p, P, N);
+  auto A = [&](int row, int col) -> double& { return F[(size_t)col * LUS + row]; };
   for (int j = 0; j < N; ++j) {
     for (int row = 0; row < KL; ++row) A(row, j) = 0.0;
     for (int q = 0; q < LDA; ++q) A(KL + q, j) = c2 * me[mo.lapb + q * N + j];
@@ -270,7 +271,10 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
   const int S = (m * G.Mp + (PM < 0 ? -PM : PM)) % 2;
   for (int j1 = 1; j1 <= KL + 1; ++j1) A(2 * KL + 1 - j1, j1 - 1) = me[mo.dr0 + (j1 - 1) + (KL + 1) * (1 - S)];
   for (int j1 = N - KL; j1 <= N; ++j1) A(2 * KL + N - j1, j1 - 1) = me[mo.dr1 + (KL - N + j1) + (KL + 1) * BC];
-  for (int j = 0; j < N; ++j) pv[(size_t)j * P] = 0;
+  for (int j = 0; j < N; ++j) {
+    pv[j] = 0;
+    for (int row = NLU; row < LUS; ++row) A(row, j) = 0.0;
+  }
   if (BC == 1 && k == 0 && m == 0) return;   // pressure (0,0) is never factored
   // DGBTF2, KL = KU = 4, KV = 8 (0-based rows)
   const int KV = 2 * KL;
@@ -283,7 +287,7 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
       const double v = fabs(A(KV + i, j));
       if (v > best) { best = v; jp = i; }
     }
-    pv[(size_t)j * P] = (unsigned char)jp;
+    pv[j] = (unsigned char)jp;
     if (A(KV + jp, j) == 0.0) {
       atomicMax(&B.ctl[slot].status, 100 + mat);  // singular factor
       return;
@@ -324,19 +328,19 @@ __global__ void k_influence(Geo G, Bufs B, int slot0) {
   const MeshOff mo = MeshOff::make(N);
   const double* me = G.mesh;
   double* U = B.Uinf + (size_t)slot * 6 * N * P + p;
-  auto LUp = [&](int mat) { return B.LU + ((size_t)(slot * 4 + mat) * N) * NLU * P + p; };
-  auto PVp = [&](int mat) { return B.PV + ((size_t)(slot * 4 + mat) * N) * P + p; };
+  auto LUp = [&](int mat) { return (const double*)lu_block(B, slot, mat, p, P, N); };
+  auto PVp = [&](int mat) { return (const unsigned char*)pv_block(B, slot, mat, p, P, N); };
   for (int j = 0; j < 3; ++j) {
     double* x = U + (size_t)j * N * P;
     for (int n = 0; n < N; ++n) x[(size_t)n * P] = n == N - 1 ? 1.0 : 0.0;
-    lu_solve<double>(LUp(j), PVp(j), P, x, P, N);
+    lu_solve<double>(LUp(j), PVp(j), x, P, N);
   }
   double* pp = U + (size_t)5 * N * P;
   if (k == 0 && m == 0) {
     for (int n = 0; n < N; ++n) pp[(size_t)n * P] = 0.0;
   } else {
     for (int n = 0; n < N; ++n) pp[(size_t)n * P] = n == N - 1 ? -1.0 : 0.0;
-    lu_solve<double>(LUp(MAT_LNP), PVp(MAT_LNP), P, pp, P, N);
+    lu_solve<double>(LUp(MAT_LNP), PVp(MAT_LNP), pp, P, N);
   }
   double* U4 = U + (size_t)3 * N * P;
   double* U5 = U + (size_t)4 * N * P;
@@ -355,7 +359,7 @@ __global__ void k_influence(Geo G, Bufs B, int slot0) {
     double* Ui = B.Ui + (size_t)slot * N;
     for (int n = 0; n < N; ++n) Ui[n] = (n == 0 || n == N - 1) ? 0.0 : 1.0;
     if (prm.const_flux) {
-      lu_solve<double>(LUp(MAT_LDZ), PVp(MAT_LDZ), P, Ui, 1, N);
+      lu_solve<double>(LUp(MAT_LDZ), PVp(MAT_LDZ), Ui, 1, N);
       double s = 0.0;
       for (int n = 0; n < N; ++n) s = s + Ui[n] * me[mo.intrdr + n];
       B.ctl[slot].flux_d1 = 2.0 * s;
@@ -458,305 +462,6 @@ __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
   u[o] = make_double2(0.0, 0.0);
   u[G.coll + o].y = 0.0;
   u[2 * G.coll + o].y = 0.0;
-}
-
-// ---------------------------------------------------------------------------
-// The step kernel: vel_predictor / vel_corrector (velocity.f90:400-432) with
-// vel_addHPF (:338-361), tim_nlincorr (timestep.f90:174-184), vel_step
-// (velocity.f90:367-394) and tim_measurecorr (timestep.f90:190-207), for all
-// active slots, one lane per harmonic.  The raw nonlinear terms Nraw come
-// from the transform kernels without the base-flow terms; they are added
-// here from the same u the reference's vel_addHPF sees (the current iterate),
-// and the corrector rebuilds the predictor's N_ from its raw terms and u_.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void hpf_point(double2& Nr, double2& Nt, double2& Nz,
-                                          double2 ur, double2 ut, double2 uz, double a, double b) {
-  Nr = make_double2(Nr.x + a * ur.y, Nr.y - a * ur.x);
-  Nt = make_double2(Nt.x + a * ut.y, Nt.y - a * ut.x);
-  Nz = make_double2((Nz.x + a * uz.y) + b * ur.x, (Nz.y - a * uz.x) + b * ur.y);
-}
-
-__device__ __forceinline__ double pr0_of(const double2* uz, size_t hs, int N, const double* dr1c) {
-  // vel_Pr0 = dot_product(uz%Re(N-KL:N,0), dr1(:,1)) / (-2)
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i <= KL; ++i) s = s + uz[(size_t)(N - 1 - KL + i) * hs].x * dr1c[i];
-  return s / (-2.0);
-}
-
-__global__ void __launch_bounds__(64) k_step(Geo G, Bufs B, int slot0) {
-  const int slot = slot0 + blockIdx.z;
-  SlotCtl* ctl = &B.ctl[slot];
-  if (!ctl->active) return;
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = h < G.H;
-  const int N = G.N, P = G.P;
-  const size_t hs = G.Hp, cs = G.coll;
-  const MeshOff mo = MeshOff::make(N);
-  const double* me = G.mesh;
-  const double* dr1c = me + mo.dr1 + (KL + 1);
-  const SlotParams prm = B.prm[slot];
-  const int mode = ctl->mode;
-  const bool corr = mode == MODE_CORR;
-  const size_t s3 = (size_t)slot * 3 * cs;
-
-  // u_ (start of step): the predictor copies u -> u_ first (velocity.f90:402-404),
-  // so in predictor mode it is the current state itself
-  const double2* ub = B.U[corr ? ctl->ubase : ctl->ucur] + s3;
-  const double2* uc = B.U[ctl->ucur] + s3;    // current iterate (HPF source)
-  double2* un = B.U[ctl->unext] + s3;         // output
-  const double2* Nc = B.Nraw[ctl->ncur] + s3;
-  const double2* Np = B.Nraw[ctl->npred] + s3;
-  double2* R = B.R + s3;
-  double2* S = B.S + (size_t)slot * cs;
-
-  int k = 0, m = 0;
-  if (live) harmonic_km(h, G.K, k, m);
-  const int p = m * G.K + (k < 0 ? -k : k);
-  const double ak = prm.alpha * (double)k;
-  const bool h00 = live && h == 0;
-  // vel_Pr0 from the u seen by each vel_addHPF
-  const double pr_c = h00 ? pr0_of(uc + 2 * cs, hs, N, dr1c) : 0.0;
-  const double pr_b = h00 ? pr0_of(ub + 2 * cs, hs, N, dr1c) : 0.0;
-  const double th = prm.implicit, th1 = 1.0 - prm.implicit;
-  double dmax = 0.0;     // max |u_new - u_cur|^2 of this lane (corrector)
-  const double c1 = 1.0 / prm.dt;
-  const double c2 = (1.0 - prm.implicit) * prm.nu / prm.Re;
-
-  // diagnostics at the start of a step (io_write2files point): per-harmonic
-  // energy partials of the current state and the friction quantities
-  const bool want_diag = !corr && ctl->diag_every > 0 && (ctl->step % ctl->diag_every) == 0;
-
-  if (live) {
-    if (want_diag) {
-      const double w = 4.0 * D_PI * D_PI / prm.alpha;
-      double* db = B.diagb + (size_t)slot * 3 * hs;
-      for (int c = 0; c < 3; ++c) {
-        double b = 0.0;
-        for (int n = 0; n < N; ++n) {
-          const double2 v = uc[c * cs + (size_t)n * hs + h];
-          b = b + (w * (v.x * v.x + v.y * v.y)) * me[mo.intrdr + n];
-        }
-        if (h == 0) b = 0.5 * b;
-        db[c * hs + h] = b;
-      }
-      if (h00) {
-        double s = 0.0;
-        for (int i = 0; i <= KL; ++i) s = s + uc[2 * cs + (size_t)i * hs].x * me[mo.dr0 + i];
-        double f = 0.0;
-        for (int i = 0; i <= KL; ++i) f = f + uc[2 * cs + (size_t)(N - 1 - KL + i) * hs].x * dr1c[i];
-        DiagRec* rec = B.diag + (size_t)slot * ctl->diag_cap + (ctl->ndiag % ctl->diag_cap);
-        rec->step = ctl->step;
-        rec->t = ctl->t;
-        rec->Ub = pr_c;
-        rec->Uc = 1.0 + s;
-        rec->Ufr = sqrt(fabs((f - 2.0) / prm.Re));
-      }
-    }
-
-    // -- stage 1: effective N, RHS = Lt u_ + N (tim_meshmult) --------------
-    for (int n = 0; n < N; ++n) {
-      const size_t o = (size_t)n * hs + h;
-      const double a = ak * me[mo.Ubase + n];
-      const double b = me[mo.twor + n];
-      double2 nr = Nc[o], nt = Nc[cs + o], nz = Nc[2 * cs + o];
-      hpf_point(nr, nt, nz, uc[o], uc[cs + o], uc[2 * cs + o], a, b);
-      if (h00) nz.x = nz.x + (4.0 * pr_c) / prm.Re;
-      if (corr) {
-        double2 pr = Np[o], pt = Np[cs + o], pz = Np[2 * cs + o];
-        hpf_point(pr, pt, pz, ub[o], ub[cs + o], ub[2 * cs + o], a, b);
-        if (h00) pz.x = pz.x + (4.0 * pr_b) / prm.Re;
-        nr = make_double2(th * nr.x + th1 * pr.x, th * nr.y + th1 * pr.y);
-        nt = make_double2(th * nt.x + th1 * pt.x, th * nt.y + th1 * pt.y);
-        nz = make_double2(th * nz.x + th1 * pz.x, th * nz.y + th1 * pz.y);
-      }
-      // rt2pm of N
-      const double2 cp = make_double2(nr.x - nt.y, nr.y + nt.x);
-      const double2 cm = make_double2(nr.x + nt.y, nr.y - nt.x);
-      double2 rp = cp, rm = cm, rz = nz;
-      const double r2i = me[mo.rinv2 + n];
-      const int j0 = n - KL < 0 ? 0 : n - KL, j1 = n + KL > N - 1 ? N - 1 : n + KL;
-      for (int j = j0; j <= j1; ++j) {
-        const double lap = me[mo.lapb + (KL + n - j) * N + j];
-        double ap = c2 * lap, am = c2 * lap, az = c2 * lap;
-        if (j == n) {
-          ap = (ap + c2 * diag_d(r2i, G.Mp, m, prm.alpha, k, 1)) + c1;
-          am = (am + c2 * diag_d(r2i, G.Mp, m, prm.alpha, k, -1)) + c1;
-          az = (az + c2 * diag_d(r2i, G.Mp, m, prm.alpha, k, 0)) + c1;
-        }
-        const size_t oj = (size_t)j * hs + h;
-        const double2 br = ub[oj], bt = ub[cs + oj], bz = ub[2 * cs + oj];
-        const double2 up = make_double2(br.x - bt.y, br.y + bt.x);
-        const double2 um = make_double2(br.x + bt.y, br.y - bt.x);
-        rp.x = rp.x + ap * up.x; rp.y = rp.y + ap * up.y;
-        rm.x = rm.x + am * um.x; rm.y = rm.y + am * um.y;
-        rz.x = rz.x + az * bz.x; rz.y = rz.y + az * bz.y;
-      }
-      R[o] = rp;
-      R[cs + o] = rm;
-      R[2 * cs + o] = rz;
-    }
-
-    // -- stage 2: pressure RHS = div(pm2rt(R+, R-), Rz), zero BC rows --------
-    for (int n = 0; n < N; ++n) {
-      const size_t o = (size_t)n * hs + h;
-      double2 dv = make_double2(0.0, 0.0);
-      if (n != 0 && n != N - 1) {
-        const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
-        double2 d = make_double2(0.0, 0.0);
-        const int j0 = n - KL < 0 ? 0 : n - KL, j1 = n + KL > N - 1 ? N - 1 : n + KL;
-        for (int j = j0; j <= j1; ++j) {
-          const size_t oj = (size_t)j * hs + h;
-          const double2 a = R[oj], bm = R[cs + oj];
-          const double2 rr = make_double2(0.5 * (a.x + bm.x), 0.5 * (a.y + bm.y));
-          const double wgt = me[mo.dr1b + (KL + n - j) * N + j];
-          d.x = d.x + wgt * rr.x;
-          d.y = d.y + wgt * rr.y;
-        }
-        const double2 a = R[o], bm = R[cs + o], z = R[2 * cs + o];
-        const double2 rr = make_double2(0.5 * (a.x + bm.x), 0.5 * (a.y + bm.y));
-        const double2 tt = make_double2(0.5 * (a.y - bm.y), -0.5 * (a.x - bm.x));
-        const double ri = me[mo.rinv + n];
-        dv.x = ((rr.x * ri + d.x) - tt.y * mr) - z.y * ak;
-        dv.y = ((rr.y * ri + d.y) + tt.x * mr) + z.x * ak;
-      }
-      S[o] = dv;
-    }
-    // pressure solve (LNp, Neumann); harmonic (0,0) has p = 0
-    const size_t lu_mat = (size_t)N * NLU * P;
-    const double* LUs = B.LU + (size_t)slot * 4 * lu_mat + p;
-    const unsigned char* PVs = B.PV + (size_t)slot * 4 * N * P + p;
-    if (h00) {
-      for (int n = 0; n < N; ++n) S[(size_t)n * hs + h] = make_double2(0.0, 0.0);
-    } else {
-      lu_solve<double2>(LUs + MAT_LNP * lu_mat, PVs + (size_t)MAT_LNP * N * P, P, S + h, hs, N);
-    }
-
-    // -- stage 3: subtract grad p (rt2pm), zero BC rows ---------------------
-    for (int n = 0; n < N; ++n) {
-      const size_t o = (size_t)n * hs + h;
-      if (n == 0 || n == N - 1) {
-        R[o] = R[cs + o] = R[2 * cs + o] = make_double2(0.0, 0.0);
-        continue;
-      }
-      const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
-      const double2 gr = band_row(me + mo.dr1b, N, n, S + h, hs, make_double2(0.0, 0.0));
-      const double2 pv = S[o];
-      const double2 gt = make_double2(-(pv.y * mr), pv.x * mr);
-      const double2 gz = make_double2(-(pv.y * ak), pv.x * ak);
-      const double2 gp = make_double2(gr.x - gt.y, gr.y + gt.x);
-      const double2 gm = make_double2(gr.x + gt.y, gr.y - gt.x);
-      R[o] = csub(R[o], gp);
-      R[cs + o] = csub(R[cs + o], gm);
-      R[2 * cs + o] = csub(R[2 * cs + o], gz);
-    }
-
-    // -- stage 4: Helmholtz solves LDp, LDm, LDz ----------------------------
-    for (int c = 0; c < 3; ++c)
-      lu_solve<double2>(LUs + c * lu_mat, PVs + (size_t)c * N * P, P, R + c * cs + h, hs, N);
-
-    // -- stage 5: influence-matrix correction, pm2rt, store -----------------
-    double aR[4] = {0, 0, 0, 0}, aI[4] = {0, 0, 0, 0};
-    if (!h00) {
-      double2 wp[KL + 1], wm[KL + 1], wz[KL + 1];
-      for (int i = 0; i <= KL; ++i) {
-        const size_t o = (size_t)(N - 1 - KL + i) * hs + h;
-        wp[i] = R[o];
-        wm[i] = R[cs + o];
-        wz[i] = R[2 * cs + o];
-      }
-      double BRe[4], BIm[4];
-      eval_bc(wp, wm, wz, dr1c, me[mo.rinv + N - 1], (double)(m * G.Mp), ak, BRe, BIm);
-      const double* Ai = B.Ainv + (size_t)slot * 16 * G.H + h;
-      for (int i = 0; i < 4; ++i) {
-        double sR = 0.0, sI = 0.0;
-        for (int j = 0; j < 4; ++j) {
-          const double aij = Ai[(size_t)(i * 4 + j) * G.H];
-          sR = sR + aij * BRe[j];
-          sI = sI + aij * BIm[j];
-        }
-        aR[i] = -sR;
-        aI[i] = -sI;
-      }
-    }
-    const double* U = B.Uinf + (size_t)slot * 6 * N * P + p;
-    for (int n = 0; n < N; ++n) {
-      const size_t o = (size_t)n * hs + h;
-      double2 up = R[o], um = R[cs + o], uz = R[2 * cs + o];
-      if (!h00) {
-        const size_t q = (size_t)n * P;
-        const double U1 = U[q], U2 = U[(size_t)N * P + q], U3 = U[(size_t)2 * N * P + q];
-        const double U4 = U[(size_t)3 * N * P + q], U5 = U[(size_t)4 * N * P + q];
-        const double U6 = U[(size_t)5 * N * P + q] * ak;
-        up.x = up.x + aR[0] * U1; up.y = up.y + aI[0] * U1;
-        um.x = um.x + aR[1] * U2; um.y = um.y + aI[1] * U2;
-        uz.x = uz.x - aI[2] * U3; uz.y = uz.y + aR[2] * U3;
-        up.x = up.x + aR[3] * U4; up.y = up.y + aI[3] * U4;
-        um.x = um.x + aR[3] * U5; um.y = um.y + aI[3] * U5;
-        uz.x = uz.x - aI[3] * U6; uz.y = uz.y + aR[3] * U6;
-      }
-      const double2 ur = make_double2(0.5 * (up.x + um.x), 0.5 * (up.y + um.y));
-      const double2 ut = make_double2(0.5 * (up.y - um.y), -0.5 * (up.x - um.x));
-      un[o] = ur;
-      un[cs + o] = ut;
-      un[2 * cs + o] = uz;
-      if (corr && !h00) {
-        const double2 d0 = csub(uc[o], ur), d1 = csub(uc[cs + o], ut), d2 = csub(uc[2 * cs + o], uz);
-        dmax = fmax(dmax, fmax(d0.x * d0.x + d0.y * d0.y, fmax(d1.x * d1.x + d1.y * d1.y, d2.x * d2.x + d2.y * d2.y)));
-      }
-    }
-    if (h00) {
-      // vel_adjustFlux(1) (velocity.f90:170-175) and the zeroing of (0,0)
-      double Pr0 = pr_c;
-      if (prm.const_flux) {
-        double s = 0.0;
-        for (int n = 0; n < N; ++n) s = s + un[2 * cs + (size_t)n * hs].x * me[mo.intrdr + n];
-        const double d2 = 2.0 * s;
-        const double d3 = -d2 / ctl->flux_d1;
-        const double* Ui = B.Ui + (size_t)slot * N;
-        for (int n = 0; n < N; ++n) {
-          double2& z = un[2 * cs + (size_t)n * hs];
-          z.x = z.x + d3 * Ui[n];
-        }
-        Pr0 = Pr0 + (d3 * prm.Re) / 4.0;
-      }
-      for (int n = 0; n < N; ++n) {
-        const size_t o = (size_t)n * hs;
-        un[o] = make_double2(0.0, 0.0);
-        un[cs + o].y = 0.0;
-        un[2 * cs + o].y = 0.0;
-        if (corr) {
-          const double2 d0 = csub(uc[o], un[o]), d1 = csub(uc[cs + o], un[cs + o]),
-                        d2 = csub(uc[2 * cs + o], un[2 * cs + o]);
-          dmax = fmax(dmax, fmax(d0.x * d0.x + d0.y * d0.y, fmax(d1.x * d1.x + d1.y * d1.y, d2.x * d2.x + d2.y * d2.y)));
-        }
-      }
-      ctl->Pr0 = Pr0;
-      ctl->Pr0_hpf = pr_c;
-    }
-  }
-  if (corr) {
-    // All 64 lanes take part (lanes past H carry 0).  Max is order
-    // independent, and the bit pattern of a non-negative double orders like
-    // its value, so an integer atomic max gives the exact global maximum.
-    for (int off = 32; off >= 1; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&ctl->dterr_bits, (unsigned long long)__double_as_longlong(dmax));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// curl of the new iterate (U[unext]) -> om, for the next evaluation
-// ---------------------------------------------------------------------------
-__global__ void k_curl_next(Geo G, Bufs B, int slot0) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = slot0 + blockIdx.z;
-  const SlotCtl* c = &B.ctl[slot];
-  if (!c->active || h >= G.H) return;
-  int k, m;
-  harmonic_km(h, G.K, k, m);
-  const double2* u = B.U[c->unext] + (size_t)slot * 3 * G.coll;
-  curl_lane(G, h, k, m, B.prm[slot].alpha, u, u + G.coll, u + 2 * G.coll,
-            B.om + (size_t)slot * 3 * G.coll);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,9 +602,7 @@ __global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, double2* x) 
     for (int n = 0; n < N; ++n) x[(size_t)n * G.Hp + h] = make_double2(0.0, 0.0);
     return;
   }
-  const double* F = B.LU + ((size_t)(slot * 4 + which) * N) * NLU * P + p;
-  const unsigned char* pv = B.PV + ((size_t)(slot * 4 + which) * N) * P + p;
-  lu_solve<double2>(F, pv, P, x + h, G.Hp, N);
+  lu_solve<double2>(lu_block(B, slot, which, p, P, N), pv_block(B, slot, which, p, P, N), x + h, G.Hp, N);
 }
 
 }  // namespace opf

@@ -11,7 +11,8 @@ namespace opf {
 constexpr int KL = 4;          // parameters.f90:45 (half band width)
 constexpr int LDA = 2 * KL + 1;  // 'mesh' band rows (meshs.f90:37-39)
 constexpr int LDAB = 3 * KL + 1; // 'lumesh' rows (meshs.f90:42-45)
-constexpr int NLU = 13;        // stored rows per factor column: 9 of U + 4 multipliers
+constexpr int NLU = 13;        // rows per factor column: 9 of U + 4 multipliers
+constexpr int LUS = 16;        // stored doubles per factor column (128-byte aligned)
 constexpr int MAXN = 128;      // largest radial resolution supported
 constexpr double D_PI = 3.1415926535897931;  // parameters.f90:54
 
