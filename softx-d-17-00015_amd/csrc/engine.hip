@@ -130,6 +130,8 @@ size_t bytes_needed(int N, int K, int M, int cap, int diag_cap) {
   b += (size_t)cap * s.coll * sizeof(double2);               // S
   b += (size_t)cap * 4 * N * LUS * s.P * sizeof(double);     // LU
   b += (size_t)cap * 4 * N * s.P;                            // PV
+  b += (size_t)cap * 4 * N * s.P * (FW_Q + BW_Q) * sizeof(double2);  // FW, BW
+  b += (size_t)cap * 4 * s.P * 2;                            // META
   b += (size_t)cap * 6 * N * s.P * sizeof(double);           // Uinf
   b += (size_t)cap * 16 * s.H * sizeof(double);              // Ainv
   b += (size_t)cap * 3 * s.Hp * sizeof(double);              // diagb
@@ -153,33 +155,29 @@ FftTiles choose_tiles(int Z, int T, int M) {
   return t;
 }
 
-size_t step_smem(int N, int gp, int cs) {
+size_t step_smem(int N, int gp, int bufd2) {
   const int GH = 2 * gp;
-  const size_t FCP = (size_t)cs * LUS + 2;
-  size_t b = (size_t)7 * N * GH * sizeof(double2);          // V[4] + UB[3]
-  b += std::max((size_t)2 * 3 * gp * FCP * sizeof(double), (size_t)3 * N * GH * sizeof(double));
-  b += (size_t)GH * 8 * sizeof(double) + 8 * sizeof(double);
-  b += (size_t)4 * gp * N;
+  size_t b = (size_t)4 * N * GH * sizeof(double2);           // V[4]
+  b += std::max((size_t)2 * bufd2 * sizeof(double2), (size_t)3 * N * GH * sizeof(double));
+  b += (size_t)GH * 8 * sizeof(double) + 8 * sizeof(double);  // AC, RED
+  b += 4 * sizeof(int);                                       // META
   return round_up(b, 16);
 }
 
-StepTiles choose_step_tiles(int N, int P, int gp_req, int cs_req) {
+// Pairs per workgroup of the fused step kernel and its factor ring.  A
+// chunk block must hold one window of the widest U sweep (9 columns of 5
+// double2) plus its pad; 48 double2 per Helmholtz block gives 15-column
+// chunks for the common band widths at 26 KiB of LDS (N = 64, two pairs),
+// six workgroups per CU.
+StepTiles choose_step_tiles(int N, int P, int gp_req) {
   StepTiles t{};
-  t.cs = cs_req > 0 ? cs_req : 16;
   int gp = gp_req > 0 ? gp_req : 2;
-  gp = std::min(gp, 8);
-  auto fits = [&](int g) {
-    // LDS within the default 64 KiB and every loader thread within STEP_LOADMAX
-    const int nsolv = 2 * 2 * g * 3;
-    const int first = nsolv <= 64 ? 64 : ((nsolv + 63) / 64) * 64;
-    const int total = 3 * g * t.cs * (LUS / 2);
-    return step_smem(N, g, t.cs) <= 64 * 1024 && total <= STEP_LOADMAX * (STEP_THREADS - first);
-  };
-  while (gp > 1 && !fits(gp)) gp /= 2;
-  if (!fits(gp)) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit");
+  if (gp != 1 && gp != 2) throw OpfError(OPF_ERR_ARG, "step pairs per workgroup must be 1 or 2");
   t.gp = gp;
+  t.bufd2 = 3 * gp * 49;
   t.ngroups = (P + gp - 1) / gp;
-  t.smem = step_smem(N, gp, t.cs);
+  t.smem = step_smem(N, gp, t.bufd2);
+  if (t.smem > 64 * 1024) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit in LDS");
   return t;
 }
 
@@ -231,7 +229,8 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
 
 void launch_step(opf_ctx* c, int s0, int ns) {
   const int per = (c->st.ngroups + 7) / 8;   // XCD-aware order inside the kernel
-  launch(c, "step", k_step, dim3(8 * per, 1, ns), dim3(STEP_THREADS), c->st.smem, c->G, c->B, c->st, s0);
+  auto k = c->st.gp == 1 ? k_step_g1 : k_step_g2;
+  launch(c, "step", k, dim3(8 * per, 1, ns), dim3(STEP_THREADS), c->st.smem, c->G, c->B, c->st, s0);
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }
 
@@ -291,8 +290,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->tiles = choose_tiles(c->Z, c->T, M);
   {
     const char* e1 = std::getenv("OPF_STEP_GP");
-    const char* e2 = std::getenv("OPF_STEP_CS");
-    c->st = choose_step_tiles(N, c->P, e1 ? std::atoi(e1) : 0, e2 ? std::atoi(e2) : 0);
+    c->st = choose_step_tiles(N, c->P, e1 ? std::atoi(e1) : 0);
   }
   c->smem_z = (size_t)c->tiles.tm * c->Z * sizeof(double2);
   c->smem_t = (size_t)3 * c->tiles.tz * c->T * sizeof(double2);
@@ -325,6 +323,9 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   B.S = c->alloc<double2>((size_t)cap * s.coll);
   B.LU = c->alloc<double>((size_t)cap * 4 * N * LUS * s.P);
   B.PV = c->alloc<unsigned char>((size_t)cap * 4 * N * s.P);
+  B.FW = c->alloc<double2>((size_t)cap * 4 * N * s.P * FW_Q);
+  B.BW = c->alloc<double2>((size_t)cap * 4 * N * s.P * BW_Q);
+  B.META = c->alloc<unsigned char>((size_t)cap * 4 * s.P * 2);
   B.Uinf = c->alloc<double>((size_t)cap * 6 * N * s.P);
   B.Ainv = c->alloc<double>((size_t)cap * 16 * s.H);
   B.Ui = c->alloc<double>((size_t)cap * N);
@@ -374,6 +375,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   write_ctl(c, ctl);
   const dim3 gp((c->P + 63) / 64, 4, 1);
   launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot);
+  launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, slot);
   launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
   launch(c, "ainv", k_ainv, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
   ctl = read_ctl(c);

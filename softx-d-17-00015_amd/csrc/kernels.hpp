@@ -15,6 +15,9 @@ struct Bufs {
   double2* S;           // step scratch: pressure [slot][N][Hp]
   double* LU;           // factors [slot][4 mats][P pairs][N columns][LUS]
   unsigned char* PV;    // pivot offsets ipiv(j)-j [slot][4][P][N]
+  double2* FW;          // forward-sweep copy [slot][4][P][N][FW_Q]: (offset, l1) (l2, l3) (l4, 0)
+  double2* BW;          // backward-sweep copy [slot][4][P][BW_Q][N]: (u0, 1/u0) (u1, u2) .. (u7, u8)
+  unsigned char* META;  // [slot][4][P][2]: widest used superdiagonal, largest pivot offset
   double* Uinf;         // influence vectors [slot][6][N][P]
   double* Ainv;         // inverted influence matrices [slot][16][H]
   double* Ui;           // flux correction vector [slot][N]
@@ -44,25 +47,44 @@ __host__ __device__ inline unsigned char* pv_block(const Bufs& B, int slot, int 
   return B.PV + ((size_t)(slot * 4 + mat) * P + p) * N;
 }
 
-// Shape of the fused step kernel: pairs per workgroup and factor chunk.
+// Solve-ready copy of the factors used by the fused step kernel.  Column j of
+// the forward block holds the pivot offset (as a double) and the multipliers
+// l1..l4 = AB(10..13, j); plane q of the backward block holds, per column j,
+// q = 0: (u0, 1/u0) with u0 = AB(9, j), q >= 1: (u_{2q-1}, u_{2q}) with
+// u_d = AB(9-d, j) = U(j-d, j).
+constexpr int FW_Q = 3;
+constexpr int BW_Q = 5;
+__host__ __device__ inline double2* fw_block(const Bufs& B, int slot, int mat, int p, int P, int N) {
+  return B.FW + (((size_t)(slot * 4 + mat) * P + p) * N) * FW_Q;
+}
+__host__ __device__ inline double2* bw_block(const Bufs& B, int slot, int mat, int p, int P, int N) {
+  return B.BW + (((size_t)(slot * 4 + mat) * P + p) * N) * BW_Q;
+}
+__host__ __device__ inline unsigned char* meta_block(const Bufs& B, int slot, int mat, int p, int P) {
+  return B.META + ((size_t)(slot * 4 + mat) * P + p) * 2;
+}
+
+// Shape of the fused step kernel.
 struct StepTiles {
-  int gp;        // (|k|, m) pairs per workgroup (<= 8)
-  int cs;        // factor columns per LDS chunk
+  int gp;        // (|k|, m) pairs per workgroup: 1 or 2
+  int bufd2;     // double2 per factor chunk buffer (two buffers)
   int ngroups;   // workgroups per slot
   size_t smem;   // dynamic LDS bytes
 };
-constexpr int STEP_THREADS = 256;
-constexpr int STEP_LOADMAX = 4;  // factor double2 loaded per loader thread per chunk
+constexpr int STEP_THREADS = 128;  // wave 0 solves, wave 1 streams factor chunks
+constexpr int STEP_BLKCAP = 64;    // double2 per (system, pair) block of a chunk (one per loader lane)
 
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_MAXB = 4;      // butterflies held per thread per pass
 
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0);
+__global__ void k_lu_pack(Geo G, Bufs B, int slot0);
 __global__ void k_influence(Geo G, Bufs B, int slot0);
 __global__ void k_ainv(Geo G, Bufs B, int slot0);
 __global__ void k_curl(Geo G, Bufs B, int slot0);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
-__global__ void k_step(Geo G, Bufs B, StepTiles ST, int slot0);
+__global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
+__global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
 // transforms

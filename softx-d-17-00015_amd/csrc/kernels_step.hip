@@ -315,6 +315,43 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
 }
 
 // ---------------------------------------------------------------------------
+// Solve-ready copy of the factors (see fw_block / bw_block) and, per
+// (pair, matrix), the widest superdiagonal of U holding a non-zero and the
+// largest pivot offset.  The fused step kernel skips the superdiagonals that
+// are exactly zero (x - t*0 = x) and replaces the division by the diagonal
+// with q = b*r, e = fma(-u0, q, b), x = fma(e, r, q), r = 1/u0, which is the
+// correctly rounded quotient b/u0 (Markstein's theorem; r correctly rounded).
+// grid: x = pairs/64, y = 4 matrices, z = slot.
+// ---------------------------------------------------------------------------
+__global__ void k_lu_pack(Geo G, Bufs B, int slot0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mat = blockIdx.y;
+  const int slot = slot0 + blockIdx.z;
+  if (p >= G.P) return;
+  const int N = G.N, P = G.P;
+  const double* F = lu_block(B, slot, mat, p, P, N);
+  const unsigned char* pv = pv_block(B, slot, mat, p, P, N);
+  double2* fw = fw_block(B, slot, mat, p, P, N);
+  double2* bw = bw_block(B, slot, mat, p, P, N);
+  int kue = 1, mo = 0;
+  for (int j = 0; j < N; ++j) {
+    const double* c = F + (size_t)j * LUS;
+    fw[j * FW_Q + 0] = make_double2((double)pv[j], c[9]);
+    fw[j * FW_Q + 1] = make_double2(c[10], c[11]);
+    fw[j * FW_Q + 2] = make_double2(c[12], 0.0);
+    const double u0 = c[2 * KL];
+    bw[j] = make_double2(u0, u0 != 0.0 ? 1.0 / u0 : 0.0);
+    for (int q = 1; q < BW_Q; ++q) bw[(size_t)q * N + j] = make_double2(c[2 * KL - (2 * q - 1)], c[2 * KL - 2 * q]);
+    for (int d = 1; d <= 2 * KL; ++d)
+      if (j - d >= 0 && c[2 * KL - d] != 0.0) kue = d > kue ? d : kue;
+    mo = pv[j] > mo ? pv[j] : mo;
+  }
+  unsigned char* mt = meta_block(B, slot, mat, p, P);
+  mt[0] = (unsigned char)kue;
+  mt[1] = (unsigned char)mo;
+}
+
+// ---------------------------------------------------------------------------
 // vel_adjPPE(0) influence vectors (velocity.f90:191-218) per pair, and
 // vel_adjustFlux(0) (velocity.f90:160-168) on pair 0.
 // Uinf slots: 0 U1 (LDp), 1 U2 (LDm), 2 U3 (LDz), 3 U4, 4 U5, 5 p (U6 = p*alpha*k).
