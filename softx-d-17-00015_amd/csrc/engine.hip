@@ -81,6 +81,7 @@ struct opf_ctx {
   FftTiles tiles{};
   StepTiles st{};
   size_t smem_z = 0, smem_t = 0;
+  FftKerns fk{};
   std::vector<void*> allocs;
   size_t bytes = 0;
   std::vector<int> occupied;   // slot holds a state
@@ -144,14 +145,11 @@ FftTiles choose_tiles(int Z, int T, int M) {
   FftTiles t{};
   t.pz = make_plan(Z);
   t.pt = make_plan(T);
-  // LDS budget 32 KiB per workgroup: tm*Z and 3*tz*T complex values, so that
-  // every radix-2 pass holds at most FFT_MAXB butterflies per thread
-  t.tm = std::max(1, std::min(M, 2048 / Z));
-  t.tz = std::max(1, std::min(Z, 2048 / (3 * T)));
-  // every Stockham pass must fit FFT_MAXB butterflies per thread (radix 2 worst)
-  const int cap = FFT_MAXB * FFT_THREADS;
-  if (t.tm * Z / 2 > cap || 3 * t.tz * T / 2 > cap || 2 * T / 2 > cap)
-    throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
+  // a workgroup transforms at most FFT_TILE complex values: tm columns of
+  // length Z, or 3*tz packed lines of length T
+  if (Z > FFT_TILE || 3 * T > FFT_TILE) throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
+  t.tm = std::max(1, std::min(M, FFT_TILE / Z));
+  t.tz = std::max(1, std::min(Z, FFT_TILE / (3 * T)));
   return t;
 }
 
@@ -219,11 +217,11 @@ dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); 
 // terms) for slots [s0, s0+ns).
 void launch_eval(opf_ctx* c, int s0, int ns) {
   const FftTiles& t = c->tiles;
-  launch(c, "zfwd", k_zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, ns), dim3(FFT_THREADS), c->smem_z,
+  launch(c, "zfwd", c->fk.zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, ns), dim3(FFT_THREADS), c->smem_z,
          c->G, c->B, t, s0);
-  launch(c, "theta", k_theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
+  launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
          c->G, c->B, t, s0);
-  launch(c, "zbwd", k_zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, ns), dim3(FFT_THREADS), c->smem_z,
+  launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, ns), dim3(FFT_THREADS), c->smem_z,
          c->G, c->B, t, s0);
 }
 
@@ -288,12 +286,13 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->H = s.H; c->Hp = s.Hp; c->P = s.P; c->Z = s.Z; c->T = s.T;
   build_mesh(N, r, c->mesh);
   c->tiles = choose_tiles(c->Z, c->T, M);
+  c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
   {
     const char* e1 = std::getenv("OPF_STEP_GP");
     c->st = choose_step_tiles(N, c->P, e1 ? std::atoi(e1) : 0);
   }
-  c->smem_z = (size_t)c->tiles.tm * c->Z * sizeof(double2);
-  c->smem_t = (size_t)3 * c->tiles.tz * c->T * sizeof(double2);
+  c->smem_z = ((size_t)c->tiles.tm * padded_len(c->Z) + c->Z) * sizeof(double2);       // data + twiddles
+  c->smem_t = ((size_t)3 * c->tiles.tz * padded_len(c->T) + c->T) * sizeof(double2);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   // device mesh block
   const MeshOff mo = MeshOff::make(N);
@@ -664,7 +663,7 @@ int opf_vel_transform(opf_ctx* c, int slot) {
   return guard([&] {
     with_active(c, slot, -1, [&] {
       const FftTiles& t = c->tiles;
-      launch(c, "zfwd", k_zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+      launch(c, "zfwd", c->fk.zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
     });
   });
@@ -674,9 +673,9 @@ int opf_vel_nonlinear(opf_ctx* c, int slot) {
   return guard([&] {
     with_active(c, slot, -1, [&] {
       const FftTiles& t = c->tiles;
-      launch(c, "theta", k_theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, 1), dim3(FFT_THREADS), c->smem_t, c->G,
+      launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, 1), dim3(FFT_THREADS), c->smem_t, c->G,
              c->B, t, slot);
-      launch(c, "zbwd", k_zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+      launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
     });
   });
@@ -752,7 +751,7 @@ int opf_tra_coll2phys(opf_ctx* c, const double* cre, const double* cim, double* 
     const FftTiles& t = c->tiles;
     launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
            c->B, t, (const double2*)c->tC, c->tY);
-    launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), c->T * sizeof(double2), c->G,
+    launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), (padded_len(c->T) + c->T) * sizeof(double2), c->G,
            c->B, t, (const double2*)c->tY, c->tP);
     HIPCHK(hipMemcpyAsync(phys, c->tP, sizeof(double) * c->N * c->Z * c->T, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -766,7 +765,7 @@ int opf_tra_phys2coll(opf_ctx* c, const double* phys, double* cre, double* cim) 
     HIPCHK(hipMemcpyAsync(c->tP, phys, sizeof(double) * c->N * c->Z * c->T, hipMemcpyHostToDevice, c->stream));
     const FftTiles& t = c->tiles;
     HIPCHK(hipMemsetAsync(c->tC, 0, sizeof(double2) * c->G.coll, c->stream));
-    launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), c->T * sizeof(double2), c->G,
+    launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), (padded_len(c->T) + c->T) * sizeof(double2), c->G,
            c->B, t, (const double*)c->tP, c->tY);
     launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
            c->B, t, (const double2*)c->tY, c->tC);

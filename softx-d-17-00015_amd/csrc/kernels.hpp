@@ -75,7 +75,11 @@ constexpr int STEP_THREADS = 128;  // wave 0 solves, wave 1 streams factor chunk
 constexpr int STEP_BLKCAP = 64;    // double2 per (system, pair) block of a chunk (one per loader lane)
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_MAXB = 4;      // butterflies held per thread per pass
+constexpr int FFT_TILE = 2048;   // complex values per transform workgroup (all its lines)
+
+// LDS stride of one transform of length L in the transform kernels: one pad
+// slot per 16 elements, rounded up to an odd count.
+__host__ __device__ inline int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
 
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0);
 __global__ void k_lu_pack(Geo G, Bufs B, int slot0);
@@ -87,10 +91,13 @@ __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
-// transforms
-__global__ void k_zfwd(Geo G, Bufs B, FftTiles T, int slot0);
-__global__ void k_theta(Geo G, Bufs B, FftTiles T, int slot0);
-__global__ void k_zbwd(Geo G, Bufs B, FftTiles T, int slot0);
+// transforms: kernels specialised to the radix plans of (Z, T), chosen once
+// per context by fft_kernels (generic kernels for other lengths)
+using FftKern = void (*)(Geo, Bufs, FftTiles, int);
+struct FftKerns {
+  FftKern zfwd, theta, zbwd;
+};
+FftKerns fft_kernels(const FftPlan& pz, const FftPlan& pt);
 // materialise physical fields / transform arbitrary coll fields (granular API)
 __global__ void k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y);
 __global__ void k_theta_c2r(Geo G, Bufs B, FftTiles T, const double2* Y, double* phys);

@@ -4,8 +4,8 @@
 // 65-147): a complex length-3K transform along z for every azimuthal mode
 // (exponent +1 to physical space, -1 back), and a real length-3M transform
 // along theta for every z line, with the 3/2 zero padding and truncation done
-// by copies.  Here the same DFTs run as LDS-staged Stockham FFTs (radix 4, 2,
-// 3) in three kernels with intermediates laid out for coalesced access:
+// by copies.  Here the same DFTs run as LDS-resident Stockham FFTs in three
+// kernels with intermediates laid out for coalesced access:
 //
 //   k_zfwd : 6 coll fields (u, curl u) -> Yf[slot][n][z][6][M]   (z transform)
 //   k_theta: Yf -> theta transforms, products, scaling, inverse theta
@@ -17,11 +17,19 @@
 // theta transforms share one complex FFT (a + ib), so the three c2r
 // transforms per pair of fields and the r2c transforms run as 3 + 2 complex
 // FFTs per line.
+//
+// Every pass is one radix-R butterfly per thread (R up to 16, built from
+// radix 2/3/4 DFTs in registers), so the lengths 3K, 3M of the usual grids
+// take two or three passes.  The twiddle table of the length is copied into
+// LDS with the data; global loads are issued in batches so that a kernel's
+// input costs one memory latency.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
 
 namespace opf {
+
+namespace {
 
 __device__ __forceinline__ double2 cmul_tw(double2 a, double2 w, double sgn) {
   // a * (w.x + i sgn w.y)
@@ -30,6 +38,8 @@ __device__ __forceinline__ double2 cmul_tw(double2 a, double2 w, double sgn) {
 }
 
 template <int R> __device__ __forceinline__ void dft(double2* v, double sgn);
+
+template <> __device__ __forceinline__ void dft<1>(double2*, double) {}
 
 template <> __device__ __forceinline__ void dft<2>(double2* v, double) {
   const double2 a = v[0], b = v[1];
@@ -63,58 +73,163 @@ template <> __device__ __forceinline__ void dft<4>(double2* v, double sgn) {
   v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
 }
 
-// One Stockham pass of radix R over B transforms of length L stored at
-// buf[b*L + i].  Ns = product of the radices of the previous passes.
+// Radix R = R1 * R2 (R1, R2 <= 4) as R2 DFTs of length R1, the internal
+// twiddles W_R^(n2 k1) and R1 DFTs of length R2 (Cooley-Tukey): input n =
+// R2 n1 + n2, output k = k1 + R1 k2.  W_R^x = tw[x L / R], tw[q] =
+// exp(2 pi i q / L).  Computed in place: output k is left in
+// v[perm<R>(k)], perm(k1 + R1 k2) = R2 k1 + k2.
+template <int R> struct Split;
+template <> struct Split<2> { static constexpr int R1 = 2, R2 = 1; };
+template <> struct Split<3> { static constexpr int R1 = 3, R2 = 1; };
+template <> struct Split<4> { static constexpr int R1 = 4, R2 = 1; };
+template <> struct Split<6> { static constexpr int R1 = 3, R2 = 2; };
+template <> struct Split<8> { static constexpr int R1 = 4, R2 = 2; };
+template <> struct Split<9> { static constexpr int R1 = 3, R2 = 3; };
+template <> struct Split<12> { static constexpr int R1 = 4, R2 = 3; };
+template <> struct Split<16> { static constexpr int R1 = 4, R2 = 4; };
+
 template <int R>
-__device__ void fft_pass(double2* buf, int L, int B, int Ns, double sgn, const double2* tw) {
+__host__ __device__ constexpr int perm(int k) {
+  return Split<R>::R2 * (k % Split<R>::R1) + k / Split<R>::R1;
+}
+
+template <int R>
+__device__ __forceinline__ void bfly(double2 (&v)[R], double sgn, const double2* twg, int L) {
+  constexpr int R1 = Split<R>::R1, R2 = Split<R>::R2;
+  if constexpr (R2 == 1) {
+    dft<R1>(v, sgn);
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      double2 a[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+      dft<R1>(a, sgn);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[R2 * k1 + n2] = a[k1];
+    }
+    const int step = L / R;
+#pragma unroll
+    for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1)
+        v[R2 * k1 + n2] = cmul_tw(v[R2 * k1 + n2], twg[((n2 * k1) % R) * step], sgn);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      double2 a[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) a[n2] = v[R2 * k1 + n2];
+      dft<R2>(a, sgn);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[R2 * k1 + k2] = a[k2];
+    }
+  }
+}
+
+// LDS layout of a batch of transforms of length L: element i of transform b
+// at buf[b*Lp + pidx(i)], pidx(i) = i + i/16, Lp = padded_len(L).  The pad
+// spreads the stride-R writes of the first Stockham pass and the strided
+// column fills over the LDS banks.
+__host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+
+// One Stockham pass of radix R over B transforms of length L stored at
+// buf[b*Lp + pidx(i)] (B*L <= FFT_TILE).  Ns = product of the radices of the
+// previous passes (the twiddles of the first pass are all 1 and skipped);
+// tw: the length-L twiddle table in LDS, twg: the same table in global
+// memory, read with uniform indices for the butterflies' internal twiddles.
+// Outer twiddles are fetched four at a time to bound register use.
+template <int R>
+__device__ __forceinline__ void fft_pass(double2* buf, int L, int Lp, int B, int Ns, double sgn, const double2* tw,
+                                         const double2* twg) {
+  constexpr int MB = (FFT_TILE + R * FFT_THREADS - 1) / (R * FFT_THREADS);
   const int nb = L / R;
   const int total = nb * B;
   const int tid = threadIdx.x;
   const int stride_tw = L / (Ns * R);
-  double2 v[FFT_MAXB][R];
+  double2 v[MB][R];
+  int dst[MB];
 #pragma unroll
-  for (int i = 0; i < FFT_MAXB; ++i) {
+  for (int i = 0; i < MB; ++i) {
     const int q = tid + i * FFT_THREADS;
+    dst[i] = -1;
     if (q < total) {
       const int b = q / nb, j = q - b * nb;
-      const double2* x = buf + b * L;
+      const double2* x = buf + b * Lp;
       const int kk = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = x[j + r * nb];
+      for (int r = 0; r < R; ++r) v[i][r] = x[pidx(j + r * nb)];
+      if (Ns > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[i][r] = cmul_tw(v[i][r], tw[kk * r * stride_tw], sgn);
-      dft<R>(v[i], sgn);
+        for (int g = 1; g < R; g += 4) {
+          __builtin_amdgcn_sched_barrier(0);
+          double2 w[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (g + u < R) w[u] = tw[kk * (g + u) * stride_tw];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (g + u < R) v[i][g + u] = cmul_tw(v[i][g + u], w[u], sgn);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      bfly<R>(v[i], sgn, twg, L);
+      dst[i] = b * Lp + (j / Ns) * Ns * R + kk;
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < FFT_MAXB; ++i) {
-    const int q = tid + i * FFT_THREADS;
-    if (q < total) {
-      const int b = q / nb, j = q - b * nb;
-      double2* x = buf + b * L;
-      const int kk = j % Ns;
-      const int base = (j / Ns) * Ns * R + kk;
+  for (int i = 0; i < MB; ++i) {
+    if (dst[i] >= 0) {
+      const int q = tid + i * FFT_THREADS;
+      const int b = q / nb;
+      const int e = dst[i] - b * Lp;          // unpadded index of output 0
+      double2* x = buf + b * Lp;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[base + r * Ns] = v[i][r];
+      for (int r = 0; r < R; ++r) x[pidx(e + r * Ns)] = v[i][perm<R>(r)];
     }
   }
   __syncthreads();
 }
 
 // Full transform: sgn = +1 (to physical space) or -1, unnormalised (FFTW).
-__device__ void fft_lds(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw) {
+__device__ __forceinline__ void fft_lds(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
+                                        const double2* twg) {
+  const int Lp = padded_len(pl.L);
   int Ns = 1;
   for (int p = 0; p < pl.nrad; ++p) {
-    const int R = pl.rad[p];
-    if (R == 4) fft_pass<4>(buf, pl.L, B, Ns, sgn, tw);
-    else if (R == 2) fft_pass<2>(buf, pl.L, B, Ns, sgn, tw);
-    else fft_pass<3>(buf, pl.L, B, Ns, sgn, tw);
-    Ns *= R;
+    switch (pl.rad[p]) {
+      case 16: fft_pass<16>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 12: fft_pass<12>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 9: fft_pass<9>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 8: fft_pass<8>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 6: fft_pass<6>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 4: fft_pass<4>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 3: fft_pass<3>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      default: fft_pass<2>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+    }
+    Ns *= pl.rad[p];
   }
 }
 
-extern __shared__ double2 smem[];
+// Transform with the radix sequence fixed at compile time (R0 = 0: the
+// plan's own sequence).  Each pass then keeps only its own registers live.
+template <int R0, int R1, int R2>
+__device__ __forceinline__ void run_fft(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
+                                        const double2* twg) {
+  if constexpr (R0 == 0) {
+    fft_lds(buf, B, pl, sgn, tw, twg);
+  } else {
+    const int Lp = padded_len(pl.L);
+    fft_pass<R0>(buf, pl.L, Lp, B, 1, sgn, tw, twg);
+    if constexpr (R1 > 1) fft_pass<R1>(buf, pl.L, Lp, B, R0, sgn, tw, twg);
+    if constexpr (R2 > 1) fft_pass<R2>(buf, pl.L, Lp, B, R0 * R1, sgn, tw, twg);
+  }
+}
+
+// twiddle table of length L into LDS
+__device__ __forceinline__ void load_tw(double2* twl, const double2* tw, int L) {
+  for (int i = threadIdx.x; i < L; i += FFT_THREADS) twl[i] = tw[i];
+}
 
 // spectral coefficient (k, m) at radial row `row` of a coll field, k in
 // (-K, K); for m = 0 the k < 0 half is the conjugate of k > 0
@@ -128,20 +243,76 @@ __device__ __forceinline__ double2 coef(const double2* row, int K, int k, int m)
   return row[K + (m - 1) * (2 * K - 1) + k + K - 1];
 }
 
-// Load the padded z column of (k, m) into buf[0..Z): X(j) = a(k=j) for j < K,
-// X(Z+k) = a(k) for k < 0, zero in between (transform.fftw3.f90:72-84).
-__device__ __forceinline__ void load_zcol(double2* buf, const double2* row, int K, int Z, int m, int j) {
-  double2 v = make_double2(0.0, 0.0);
-  if (j < K) v = coef(row, K, j, m);
-  else if (j > 2 * K) v = coef(row, K, j - Z, m);
-  buf[j] = v;
+// Padded z columns of modes m0 .. m0+tm-1 into the tile: X(j) = a(k=j)
+// for j < K, X(Z+k) = a(k) for k < 0, zero in between
+// (transform.fftw3.f90:72-84).  A tile holds at most FFT_TILE values, so
+// every thread issues its (at most ZLD) loads in one batch.
+constexpr int ZLD = FFT_TILE / FFT_THREADS;
+__device__ __forceinline__ void load_zcols(double2* buf, const double2* row, int K, int Z, int Zp, int m0, int tm) {
+  const int nk = 2 * K - 1;
+  const int total = tm * nk;
+  double2 v[ZLD];
+  int d[ZLD];
+#pragma unroll
+  for (int u = 0; u < ZLD; ++u) {
+    const int i = threadIdx.x + u * FFT_THREADS;
+    const bool ok = i < total;
+    const int ic = ok ? i : 0;
+    const int b = ic / nk, e = ic - b * nk;
+    const int k = e - (K - 1);
+    v[u] = coef(row, K, k, m0 + b);
+    d[u] = ok ? b * Zp + pidx(k >= 0 ? k : k + Z) : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < ZLD; ++u)
+    if (d[u] >= 0) buf[d[u]] = v[u];
+  const int nzero = K + 1;   // j = K .. 2K
+  for (int i = threadIdx.x; i < tm * nzero; i += FFT_THREADS) {
+    const int b = i / nzero, j = K + (i - b * nzero);
+    buf[b * Zp + pidx(j)] = make_double2(0.0, 0.0);
+  }
 }
+
+// column b of the tile (padded layout) = Y[z*ystride + m0 + b], one batch.
+__device__ __forceinline__ void load_ycols(double2* buf, const double2* Y, size_t ystride, int Z, int Zp, int m0, int tm) {
+  const int total = tm * Z;
+  double2 v[ZLD];
+  int d[ZLD];
+#pragma unroll
+  for (int u = 0; u < ZLD; ++u) {
+    const int i = threadIdx.x + u * FFT_THREADS;
+    const bool ok = i < total;
+    const int ic = ok ? i : 0;
+    const int z = ic / tm, b = ic - z * tm;
+    v[u] = Y[(size_t)z * ystride + m0 + b];
+    d[u] = ok ? b * Zp + pidx(z) : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < ZLD; ++u)
+    if (d[u] >= 0) buf[d[u]] = v[u];
+}
+
+// truncation of a z column to the 2K-1 retained k into a coll row
+__device__ __forceinline__ void store_zcol(double2* row, const double2* buf, int K, int Z, int m, int tid, int nth) {
+  if (m == 0) {
+    for (int k = tid; k < K; k += nth) row[k] = buf[pidx(k)];
+  } else {
+    double2* blk = row + K + (m - 1) * (2 * K - 1) + (K - 1);
+    for (int k = tid - (K - 1); k < K; k += nth) blk[k] = buf[pidx(k >= 0 ? k : k + Z)];
+  }
+}
+
+}  // namespace
+
+extern __shared__ double2 smem[];
 
 // ---------------------------------------------------------------------------
 // z transform of u and curl u for all active slots.
 // grid: x = m tiles, y = N * 6 (plane, field), z = slot.
+// LDS: tm columns at stride Zp, then Z twiddles.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(FFT_THREADS) k_zfwd(Geo G, Bufs B, FftTiles T, int slot0) {
+template <int R0, int R1, int R2>
+__global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles T, int slot0) {
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
@@ -152,26 +323,29 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd(Geo G, Bufs B, FftTiles T,
   const double2* src = (f < 3 ? B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll
                               : B.om + (size_t)slot * 3 * G.coll + (size_t)(f - 3) * G.coll) +
                        (size_t)n * G.Hp;
+  const int Zp = padded_len(Z);
   double2* buf = smem;
-  for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
-    const int b = i / Z, j = i - b * Z;
-    load_zcol(buf + b * Z, src, K, Z, m0 + b, j);
-  }
+  double2* twl = smem + (size_t)T.tm * Zp;
+  load_tw(twl, B.twZ, Z);
+  load_zcols(buf, src, K, Z, Zp, m0, tm);
   __syncthreads();
-  fft_lds(buf, tm, T.pz, 1.0, B.twZ);
+  run_fft<R0, R1, R2>(buf, tm, T.pz, 1.0, twl, B.twZ);
   double2* Y = B.Yf + (((size_t)slot * G.N + n) * Z) * 6 * M;
   for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
     const int z = i / tm, b = i - z * tm;
-    Y[((size_t)z * 6 + f) * M + m0 + b] = buf[b * Z + z];
+    Y[((size_t)z * 6 + f) * M + m0 + b] = buf[b * Zp + pidx(z)];
   }
 }
 
 // ---------------------------------------------------------------------------
 // theta transforms and products.  grid: x = z tiles, y = N, z = slot.
-// LDS: buf[(pair*tz + line)*T + theta] for the three packed pairs, so the two
-// forward transforms of every line are the first 2*tz contiguous transforms.
+// LDS: transform (pair*tz + line) of the three packed pairs at stride Tp, so
+// the two forward transforms of every line are the first 2*tz transforms;
+// then T twiddles.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(FFT_THREADS) k_theta(Geo G, Bufs B, FftTiles T, int slot0) {
+constexpr int TLD = (FFT_TILE / 3 + FFT_THREADS - 1) / FFT_THREADS;
+template <int R0, int R1, int R2>
+__global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles T, int slot0) {
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
@@ -180,44 +354,65 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta(Geo G, Bufs B, FftTiles T
   const int tz = min(T.tz, G.Z - z0);
   const int M = G.M, TT = G.T;
   const double2* Yf = B.Yf + (((size_t)slot * G.N + n) * G.Z + z0) * 6 * M;
+  const int Tp = padded_len(TT);
   double2* buf = smem;
-  // pack (u_r,u_t), (u_z,w_r), (w_t,w_z) as a + ib into full Hermitian spectra;
-  // c2r reads only the real part of the m = 0 bin (transform.fftw3.f90:97-98)
-  for (int i = threadIdx.x; i < tz * 3 * TT; i += FFT_THREADS) {
-    const int mm = i % TT;
-    const int pl = i / TT;
-    const int pr = pl / tz, line = pl - pr * tz;
-    const double2* src = Yf + (size_t)line * 6 * M;
-    double2 v = make_double2(0.0, 0.0);
+  double2* twl = smem + (size_t)3 * T.tz * Tp;
+  load_tw(twl, B.twT, TT);
+  // pack (u_r,u_t), (u_z,w_r), (w_t,w_z) as a + ib into full Hermitian
+  // spectra: item (line, pair, m) fills bins m and T-m; c2r reads only the
+  // real part of the m = 0 bin (transform.fftw3.f90:97-98)
+  const int total = tz * 3 * M;   // <= FFT_TILE / 3 <= TLD * FFT_THREADS
+  double2 A[TLD], Bv[TLD];
+  int it[TLD];
+#pragma unroll
+  for (int u = 0; u < TLD; ++u) {
+    const int i = threadIdx.x + u * FFT_THREADS;
+    const bool ok = i < total;
+    const int ic = ok ? i : 0;
+    const int line = ic / (3 * M), rem = ic - line * 3 * M;
+    const int pr = rem / M, mm = rem - pr * M;
+    const double2* src = Yf + (size_t)line * 6 * M + (size_t)(2 * pr) * M + mm;
+    A[u] = src[0];
+    Bv[u] = src[M];
+    it[u] = ok ? i : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < TLD; ++u) {
+    if (it[u] < 0) continue;
+    const int i = it[u];
+    const int line = i / (3 * M), rem = i - line * 3 * M;
+    const int pr = rem / M, mm = rem - pr * M;
+    double2* L = buf + ((size_t)pr * tz + line) * Tp;
     if (mm == 0) {
-      v = make_double2(src[(2 * pr) * M].x, src[(2 * pr + 1) * M].x);
-    } else if (mm < M) {
-      const double2 A = src[(2 * pr) * M + mm], Bv = src[(2 * pr + 1) * M + mm];
-      v = make_double2(A.x - Bv.y, A.y + Bv.x);
-    } else if (mm > TT - M) {
-      const int q = TT - mm;
-      const double2 A = src[(2 * pr) * M + q], Bv = src[(2 * pr + 1) * M + q];
-      v = make_double2(A.x + Bv.y, -A.y + Bv.x);
+      L[0] = make_double2(A[u].x, Bv[u].x);
+    } else {
+      L[pidx(mm)] = make_double2(A[u].x - Bv[u].y, A[u].y + Bv[u].x);
+      L[pidx(TT - mm)] = make_double2(A[u].x + Bv[u].y, -A[u].y + Bv[u].x);
     }
-    buf[i] = v;
+  }
+  const int nzero = TT - 2 * M + 1;   // bins M .. T-M
+  for (int i = threadIdx.x; i < 3 * tz * nzero; i += FFT_THREADS) {
+    const int pl = i / nzero, mm = M + (i - pl * nzero);
+    const int pr = pl / tz, line = pl - pr * tz;
+    buf[((size_t)pr * tz + line) * Tp + pidx(mm)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds(buf, 3 * tz, T.pt, 1.0, B.twT);
+  run_fft<R0, R1, R2>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
   // products (velocity.f90:325-329), scaled as tra_phys2spec (:114-117)
   const double scale = 1.0 / (double)(G.Z * TT);
   for (int i = threadIdx.x; i < tz * TT; i += FFT_THREADS) {
     const int line = i / TT, th = i - line * TT;
-    double2* L0 = buf + (size_t)line * TT + th;
-    const double2 w0 = L0[0], w1 = L0[(size_t)tz * TT], w2 = L0[(size_t)2 * tz * TT];
+    double2* L0 = buf + (size_t)line * Tp + pidx(th);
+    const double2 w0 = L0[0], w1 = L0[(size_t)tz * Tp], w2 = L0[(size_t)2 * tz * Tp];
     const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
     const double Nr = ut * cz - uz * ct;
     const double Nt = uz * cr - ur * cz;
     const double Nz = ur * ct - ut * cr;
     L0[0] = make_double2(scale * Nr, scale * Nt);
-    L0[(size_t)tz * TT] = make_double2(scale * Nz, 0.0);
+    L0[(size_t)tz * Tp] = make_double2(scale * Nz, 0.0);
   }
   __syncthreads();
-  fft_lds(buf, 2 * tz, T.pt, -1.0, B.twT);
+  run_fft<R0, R1, R2>(buf, 2 * tz, T.pt, -1.0, twl, B.twT);
   // unpack the real transforms: A(m) = (X(m) + conj X(-m))/2, B(m) = (X(m) - conj X(-m))/2i
   double2* Yb = B.Yb + (((size_t)slot * G.N + n) * G.Z + z0) * 3 * M;
   for (int i = threadIdx.x; i < tz * 3 * M; i += FFT_THREADS) {
@@ -225,10 +420,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta(Geo G, Bufs B, FftTiles T
     const int f = rem / M, mm = rem - f * M;
     double2 v;
     if (f == 2) {
-      v = buf[((size_t)tz + line) * TT + mm];
+      v = buf[((size_t)tz + line) * Tp + pidx(mm)];
     } else {
-      const double2* L = buf + (size_t)line * TT;
-      const double2 X = L[mm], Xc = L[(TT - mm) % TT];
+      const double2* L = buf + (size_t)line * Tp;
+      const double2 X = L[pidx(mm)], Xc = L[pidx((TT - mm) % TT)];
       if (f == 0) v = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
       else v = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
     }
@@ -238,18 +433,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta(Geo G, Bufs B, FftTiles T
 
 // ---------------------------------------------------------------------------
 // inverse z transform and truncation into the raw nonlinear terms.
-// grid: x = m tiles, y = N * 3, z = slot.
+// grid: x = m tiles, y = N * 3, z = slot.  LDS: tm columns at stride Zp, then Z twiddles.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void store_zcol(double2* row, const double2* buf, int K, int Z, int m, int tid, int nth) {
-  if (m == 0) {
-    for (int k = tid; k < K; k += nth) row[k] = buf[k];
-  } else {
-    double2* blk = row + K + (m - 1) * (2 * K - 1) + (K - 1);
-    for (int k = tid - (K - 1); k < K; k += nth) blk[k] = buf[k >= 0 ? k : k + Z];
-  }
-}
-
-__global__ void __launch_bounds__(FFT_THREADS) k_zbwd(Geo G, Bufs B, FftTiles T, int slot0) {
+template <int R0, int R1, int R2>
+__global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles T, int slot0) {
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
@@ -257,16 +444,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd(Geo G, Bufs B, FftTiles T,
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
   const int Z = G.Z, K = G.K, M = G.M;
-  const double2* Y = B.Yb + (((size_t)slot * G.N + n) * Z) * 3 * M;
+  const double2* Y = B.Yb + (((size_t)slot * G.N + n) * Z) * 3 * M + (size_t)f * M;
+  const int Zp = padded_len(Z);
   double2* buf = smem;
-  for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
-    const int z = i / tm, b = i - z * tm;
-    buf[b * Z + z] = Y[((size_t)z * 3 + f) * M + m0 + b];
-  }
+  double2* twl = smem + (size_t)T.tm * Zp;
+  load_tw(twl, B.twZ, Z);
+  load_ycols(buf, Y, (size_t)3 * M, Z, Zp, m0, tm);
   __syncthreads();
-  fft_lds(buf, tm, T.pz, -1.0, B.twZ);
+  run_fft<R0, R1, R2>(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = B.Nraw[c->ncur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll + (size_t)n * G.Hp;
-  for (int b = 0; b < tm; ++b) store_zcol(row, buf + b * Z, K, Z, m0 + b, threadIdx.x, FFT_THREADS);
+  for (int b = 0; b < tm; ++b) store_zcol(row, buf + b * Zp, K, Z, m0 + b, threadIdx.x, FFT_THREADS);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,26 +466,29 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_one(Geo G, Bufs B, FftTile
   const int tm = min(T.tm, G.M - m0);
   const int Z = G.Z, K = G.K, M = G.M;
   const double2* src = a + (size_t)n * G.Hp;
+  const int Zp = padded_len(Z);
   double2* buf = smem;
-  for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
-    const int b = i / Z, j = i - b * Z;
-    load_zcol(buf + b * Z, src, K, Z, m0 + b, j);
-  }
+  double2* twl = smem + (size_t)T.tm * Zp;
+  load_tw(twl, B.twZ, Z);
+  load_zcols(buf, src, K, Z, Zp, m0, tm);
   __syncthreads();
-  fft_lds(buf, tm, T.pz, 1.0, B.twZ);
+  fft_lds(buf, tm, T.pz, 1.0, twl, B.twZ);
   for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
     const int z = i / tm, b = i - z * tm;
-    Y[((size_t)n * Z + z) * M + m0 + b] = buf[b * Z + z];
+    Y[((size_t)n * Z + z) * M + m0 + b] = buf[b * Zp + pidx(z)];
   }
 }
 
 // c2r along theta of one field, lines paired as a + ib.  grid: x = z pairs, y = N.
+// LDS: one padded line, then T twiddles.
 __global__ void __launch_bounds__(FFT_THREADS) k_theta_c2r(Geo G, Bufs B, FftTiles T, const double2* Y, double* phys) {
   const int n = blockIdx.y;
   const int z0 = blockIdx.x * 2;
   const int M = G.M, TT = G.T;
   const bool two = z0 + 1 < G.Z;
   double2* buf = smem;
+  double2* twl = smem + padded_len(TT);
+  load_tw(twl, B.twT, TT);
   const double2* A = Y + ((size_t)n * G.Z + z0) * M;
   const double2* Bp = A + M;
   for (int mm = threadIdx.x; mm < TT; mm += FFT_THREADS) {
@@ -312,15 +502,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_c2r(Geo G, Bufs B, FftTil
       const double2 a = A[q], b = two ? Bp[q] : make_double2(0.0, 0.0);
       v = make_double2(a.x + b.y, -a.y + b.x);
     }
-    buf[mm] = v;
+    buf[pidx(mm)] = v;
   }
   __syncthreads();
-  fft_lds(buf, 1, T.pt, 1.0, B.twT);
+  fft_lds(buf, 1, T.pt, 1.0, twl, B.twT);
   // phys layout of the reference: Re(0:3K-1, 0:3M-1, N), z fastest
   double* out = phys + (size_t)n * TT * G.Z + z0;
   for (int th = threadIdx.x; th < TT; th += FFT_THREADS) {
-    out[(size_t)th * G.Z] = buf[th].x;
-    if (two) out[(size_t)th * G.Z + 1] = buf[th].y;
+    out[(size_t)th * G.Z] = buf[pidx(th)].x;
+    if (two) out[(size_t)th * G.Z + 1] = buf[pidx(th)].y;
   }
 }
 
@@ -332,14 +522,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_r2c(Geo G, Bufs B, FftTil
   const bool two = z0 + 1 < G.Z;
   const double scale = 1.0 / (double)(G.Z * TT);
   double2* buf = smem;
+  double2* twl = smem + padded_len(TT);
+  load_tw(twl, B.twT, TT);
   const double* in = phys + (size_t)n * TT * G.Z + z0;
   for (int th = threadIdx.x; th < TT; th += FFT_THREADS)
-    buf[th] = make_double2(scale * in[(size_t)th * G.Z], two ? scale * in[(size_t)th * G.Z + 1] : 0.0);
+    buf[pidx(th)] = make_double2(scale * in[(size_t)th * G.Z], two ? scale * in[(size_t)th * G.Z + 1] : 0.0);
   __syncthreads();
-  fft_lds(buf, 1, T.pt, -1.0, B.twT);
+  fft_lds(buf, 1, T.pt, -1.0, twl, B.twT);
   double2* A = Y + ((size_t)n * G.Z + z0) * M;
   for (int mm = threadIdx.x; mm < M; mm += FFT_THREADS) {
-    const double2 X = buf[mm], Xc = buf[(TT - mm) % TT];
+    const double2 X = buf[pidx(mm)], Xc = buf[pidx((TT - mm) % TT)];
     A[mm] = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
     if (two) A[M + mm] = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
   }
@@ -350,15 +542,54 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_one(Geo G, Bufs B, FftTile
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
   const int Z = G.Z, K = G.K, M = G.M;
+  const int Zp = padded_len(Z);
   double2* buf = smem;
-  for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
-    const int z = i / tm, b = i - z * tm;
-    buf[b * Z + z] = Y[((size_t)n * Z + z) * M + m0 + b];
-  }
+  double2* twl = smem + (size_t)T.tm * Zp;
+  load_tw(twl, B.twZ, Z);
+  load_ycols(buf, Y + (size_t)n * Z * M, (size_t)M, Z, Zp, m0, tm);
   __syncthreads();
-  fft_lds(buf, tm, T.pz, -1.0, B.twZ);
+  fft_lds(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = a + (size_t)n * G.Hp;
-  for (int b = 0; b < tm; ++b) store_zcol(row, buf + b * Z, K, Z, m0 + b, threadIdx.x, FFT_THREADS);
+  for (int b = 0; b < tm; ++b) store_zcol(row, buf + b * Zp, K, Z, m0 + b, threadIdx.x, FFT_THREADS);
+}
+
+// Radix sequences with specialised transform kernels (the transform lengths
+// 3K, 3M of the usual grids); other lengths use the generic kernels.
+namespace {
+template <int R0, int R1, int R2>
+FftKerns kerns_of() {
+  return FftKerns{k_zfwd_t<R0, R1, R2>, k_theta_t<R0, R1, R2>, k_zbwd_t<R0, R1, R2>};
+}
+struct PlanKerns {
+  int r[3];
+  FftKerns k;
+};
+}  // namespace
+
+FftKerns fft_kernels(const FftPlan& pz, const FftPlan& pt) {
+  static const PlanKerns table[] = {
+      {{9, 2, 1}, kerns_of<9, 2, 1>()},     {{12, 1, 1}, kerns_of<12, 1, 1>()},
+      {{12, 2, 1}, kerns_of<12, 2, 1>()},   {{12, 3, 1}, kerns_of<12, 3, 1>()},
+      {{12, 6, 1}, kerns_of<12, 6, 1>()},   {{16, 3, 1}, kerns_of<16, 3, 1>()},
+      {{16, 9, 1}, kerns_of<16, 9, 1>()},   {{16, 12, 1}, kerns_of<16, 12, 1>()},
+      {{16, 12, 2}, kerns_of<16, 12, 2>()}, {{16, 16, 3}, kerns_of<16, 16, 3>()},
+  };
+  auto find = [&](const FftPlan& p) -> const FftKerns* {
+    if (p.nrad < 1 || p.nrad > 3) return nullptr;
+    int r[3] = {1, 1, 1};
+    for (int i = 0; i < p.nrad; ++i) r[i] = p.rad[i];
+    for (const auto& e : table)
+      if (e.r[0] == r[0] && e.r[1] == r[1] && e.r[2] == r[2]) return &e.k;
+    return nullptr;
+  };
+  const FftKerns generic = kerns_of<0, 0, 0>();
+  FftKerns k = generic;
+  if (const FftKerns* z = find(pz)) {
+    k.zfwd = z->zfwd;
+    k.zbwd = z->zbwd;
+  }
+  if (const FftKerns* t = find(pt)) k.theta = t->theta;
+  return k;
 }
 
 }  // namespace opf

@@ -230,10 +230,17 @@ FftPlan make_plan(int L) {
   FftPlan p;
   p.L = L;
   int n = L;
-  while (n % 4 == 0) { p.rad[p.nrad++] = 4; n /= 4; }
-  while (n % 2 == 0) { p.rad[p.nrad++] = 2; n /= 2; }
-  while (n % 3 == 0) { p.rad[p.nrad++] = 3; n /= 3; }
-  if (n != 1) throw std::runtime_error("FFT length must be 2^a 3^b");
+  // largest supported radix first: lengths 3K and 3M of the usual grids need
+  // two or three passes (144 = 16*9, 192 = 16*12, 768 = 16*16*3)
+  static const int radices[] = {16, 12, 9, 8, 6, 4, 3, 2};
+  while (n > 1) {
+    int r = 0;
+    for (int c : radices)
+      if (n % c == 0) { r = c; break; }
+    if (r == 0 || p.nrad == 16) throw std::runtime_error("FFT length must be 2^a 3^b");
+    p.rad[p.nrad++] = r;
+    n /= r;
+  }
   return p;
 }
 
