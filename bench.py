@@ -45,7 +45,8 @@ def alg_bytes(N, K, M):
     Z = 3 * K
     C = 16 * N * H
     Y = 16 * N * Z * M                       # one field of the z-transformed intermediates
-    F = 4 * P * N * (13 * 8 + 1) + 6 * P * N * 8 + 16 * H * 8 + 8 * N
+    # LU factors and pivots once per (|k|, m) pair, influence vectors and 4x4 inverses
+    F = 4 * P * (8 * 13 * N + 4 * N) + H * (48 * N + 128)
     return dict(C=C, F=F,
                 zfwd=6 * C + 6 * Y, theta=9 * Y, zbwd=3 * Y + 3 * C,
                 step_pred=9 * C + F, step_corr=15 * C + F, curl=6 * C)
@@ -126,7 +127,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=40)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes five times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of the traffic figures) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        a = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design, not a code path)"}))
