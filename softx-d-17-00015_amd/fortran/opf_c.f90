@@ -1,0 +1,256 @@
+!*************************************************************************
+! opf_c.f90 -- Fortran interfaces (iso_c_binding) of the C ABI in
+! include/opf.h (libopf.so).  Pointer arguments that may be NULL are
+! type(c_ptr), value; fields are passed as the reference's coll planes
+! Re(1:N,0:H-1) / Im(1:N,0:H-1) (variables.f90:37-40), radial index fastest.
+!*************************************************************************
+module opf_c
+   use iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: OPF_OK = 0, OPF_ERR_ARG = 1, OPF_ERR_HIP = 2, &
+                                OPF_ERR_SINGULAR = 3, OPF_ERR_CORRECTOR = 4, &
+                                OPF_ERR_STATE = 5, OPF_ERR_NOMEM = 6
+
+   interface
+      function opf_last_error() bind(C, name="opf_last_error") result(msg)
+         import :: c_ptr
+         type(c_ptr) :: msg
+      end function
+
+      function opf_version() bind(C, name="opf_version") result(v)
+         import :: c_int
+         integer(c_int) :: v
+      end function
+
+      ! mes_precompute + var_precompute + tra_precompute (meshs.f90:66-103,
+      ! variables.f90:60-88, transform.fftw3.f90:35-59); r = NULL: default mesh
+      function opf_ctx_create(ctx, N, K, M, Mp, capacity, r) bind(C, name="opf_ctx_create") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), intent(out) :: ctx
+         integer(c_int), value :: N, K, M, Mp, capacity
+         type(c_ptr), value :: r
+         integer(c_int) :: rc
+      end function
+
+      function opf_ctx_destroy(ctx) bind(C, name="opf_ctx_destroy") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int) :: rc
+      end function
+
+      function opf_ctx_dims(ctx, N, K, M, Mp, H, capacity, bytes) bind(C, name="opf_ctx_dims") result(rc)
+         import :: c_ptr, c_int, c_size_t
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: N, K, M, Mp, H, capacity
+         integer(c_size_t), intent(out) :: bytes
+         integer(c_int) :: rc
+      end function
+
+      function opf_ctx_bytes_needed(N, K, M, capacity) bind(C, name="opf_ctx_bytes_needed") result(b)
+         import :: c_int, c_size_t
+         integer(c_int), value :: N, K, M, capacity
+         integer(c_size_t) :: b
+      end function
+
+      ! mes_D (meshs.f90:48-58); NULL pointers are skipped
+      function opf_mesh_get(ctx, r, intrdr, dr0, dr1, dr1band, radlap) bind(C, name="opf_mesh_get") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, r, intrdr, dr0, dr1, dr1band, radlap
+         integer(c_int) :: rc
+      end function
+
+      function opf_mesh_build(N, r_in, r, intrdr, dr0, dr1, dr1band, radlap) bind(C, name="opf_mesh_build") result(rc)
+         import :: c_ptr, c_int
+         integer(c_int), value :: N
+         type(c_ptr), value :: r_in, r, intrdr, dr0, dr1, dr1band, radlap
+         integer(c_int) :: rc
+      end function
+
+      ! vel_matrices (velocity.f90:120-146) for one state slot
+      function opf_slot_setup(ctx, slot, Re, alpha, dt, implicit, const_flux) bind(C, name="opf_slot_setup") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, const_flux
+         real(c_double), value :: Re, alpha, dt, implicit
+         integer(c_int) :: rc
+      end function
+
+      function opf_put_state(ctx, slot, ur_re, ur_im, ut_re, ut_im, uz_re, uz_im, t) &
+            bind(C, name="opf_put_state") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(in) :: ur_re(*), ur_im(*), ut_re(*), ut_im(*), uz_re(*), uz_im(*)
+         real(c_double), value :: t
+         integer(c_int) :: rc
+      end function
+
+      function opf_get_state(ctx, slot, ur_re, ur_im, ut_re, ut_im, uz_re, uz_im, t) &
+            bind(C, name="opf_get_state") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: ur_re(*), ur_im(*), ut_re(*), ut_im(*), uz_re(*), uz_im(*)
+         real(c_double), intent(out) :: t
+         integer(c_int) :: rc
+      end function
+
+      ! main.f90:18-46 on the device
+      function opf_run(ctx, nsteps, corr_fixed, diag_every) bind(C, name="opf_run") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nsteps, corr_fixed, diag_every
+         integer(c_int) :: rc
+      end function
+
+      function opf_run_slots(ctx, slots, nslots, nsteps, corr_fixed, diag_every) &
+            bind(C, name="opf_run_slots") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(in) :: slots(*)
+         integer(c_int), value :: nslots, nsteps, corr_fixed, diag_every
+         integer(c_int) :: rc
+      end function
+
+      function opf_slot_status(ctx, slot, step, t, status, last_ncorr, last_dterr, Pr0) &
+            bind(C, name="opf_slot_status") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         integer(c_int), intent(out) :: step, status, last_ncorr
+         real(c_double), intent(out) :: t, last_dterr, Pr0
+         integer(c_int) :: rc
+      end function
+
+      function opf_diag_get(ctx, slot, recs, maxrec, nrec) bind(C, name="opf_diag_get") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, maxrec
+         real(c_double), intent(out) :: recs(*)
+         integer(c_int), intent(out) :: nrec
+         integer(c_int) :: rc
+      end function
+
+      ! reference procedures, one call each
+      function opf_vel_imposesym(ctx, slot) bind(C, name="opf_vel_imposesym") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         integer(c_int) :: rc
+      end function
+
+      function opf_vel_transform(ctx, slot) bind(C, name="opf_vel_transform") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         integer(c_int) :: rc
+      end function
+
+      function opf_vel_nonlinear(ctx, slot) bind(C, name="opf_vel_nonlinear") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         integer(c_int) :: rc
+      end function
+
+      function opf_vel_predictor(ctx, slot) bind(C, name="opf_vel_predictor") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         integer(c_int) :: rc
+      end function
+
+      function opf_vel_corrector(ctx, slot, dterr) bind(C, name="opf_vel_corrector") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: dterr
+         integer(c_int) :: rc
+      end function
+
+      function opf_get_nonlinear(ctx, slot, nr_re, nr_im, nt_re, nt_im, nz_re, nz_im) &
+            bind(C, name="opf_get_nonlinear") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: nr_re(*), nr_im(*), nt_re(*), nt_im(*), nz_re(*), nz_im(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_tra_coll2phys(ctx, c_re, c_im, phys) bind(C, name="opf_tra_coll2phys") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: c_re(*), c_im(*)
+         real(c_double), intent(out) :: phys(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_tra_phys2coll(ctx, phys, c_re, c_im) bind(C, name="opf_tra_phys2coll") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: phys(*)
+         real(c_double), intent(out) :: c_re(*), c_im(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_tim_lumesh_get(ctx, slot, which, ab, ipiv) bind(C, name="opf_tim_lumesh_get") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, which
+         type(c_ptr), value :: ab, ipiv
+         integer(c_int) :: rc
+      end function
+
+      function opf_tim_lumesh_invert(ctx, slot, which, b_re, b_im) bind(C, name="opf_tim_lumesh_invert") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, which
+         real(c_double), intent(inout) :: b_re(*), b_im(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_kernel_timing(ctx, enable) bind(C, name="opf_kernel_timing") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: enable
+         integer(c_int) :: rc
+      end function
+
+      function opf_kernel_time(ctx, name, ms, launches) bind(C, name="opf_kernel_time") result(rc)
+         import :: c_ptr, c_int, c_double, c_char
+         type(c_ptr), value :: ctx
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(out) :: ms
+         integer(c_int), intent(out) :: launches
+         integer(c_int) :: rc
+      end function
+   end interface
+
+contains
+
+   ! message of the last failing call, as a Fortran string
+   function opf_error_message() result(s)
+      character(len=:), allocatable :: s
+      character(kind=c_char), pointer :: p(:)
+      type(c_ptr) :: cp
+      integer :: n
+      interface
+         function c_strlen(str) bind(C, name="strlen") result(len)
+            import :: c_ptr, c_size_t
+            type(c_ptr), value :: str
+            integer(c_size_t) :: len
+         end function
+      end interface
+      cp = opf_last_error()
+      if (.not. c_associated(cp)) then
+         s = ''
+         return
+      end if
+      n = int(c_strlen(cp))
+      call c_f_pointer(cp, p, [n])
+      allocate(character(len=n) :: s)
+      s = transfer(p(1:n), s)
+   end function opf_error_message
+
+end module opf_c

@@ -1,0 +1,78 @@
+"""The Fortran host (softx-d-17-00015_amd/fortran): the reference's main loop
+(main.f90:18-46) written against procedures with the reference's names
+(vel_predictor, vel_corrector, tim_check_cgce, ...) that call the C ABI.
+
+CPU: the host builds and links against libopf.so.  GPU: run from the
+reference's initial states, its states after 1, 10 and 100 steps match the
+reference's own to the step tolerances, and its corrector counts match the
+reference's series."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import PKG, load_golden, rel_err
+
+FDIR = os.path.join(PKG, "fortran")
+EXE = os.path.join(FDIR, "opf_main")
+
+
+def _write_state(path, u):
+    parts = []
+    for f in u:                       # Re(1:N,0:H-1), Im(1:N,0:H-1), N fastest
+        parts.append(np.asarray(f.real, dtype="<f8").T.ravel())
+        parts.append(np.asarray(f.imag, dtype="<f8").T.ravel())
+    np.concatenate(parts).tofile(path)
+
+
+def _read_state(path, N, H):
+    a = np.fromfile(path, dtype="<f8").reshape(6, H, N)
+    return [(a[2 * c] + 1j * a[2 * c + 1]).T for c in range(3)]
+
+
+@pytest.mark.skipif(shutil.which("amdflang") is None, reason="amdflang not available")
+def test_fortran_host_builds():
+    subprocess.run(["make", "-s", "-C", FDIR], check=True)
+    assert os.access(EXE, os.X_OK)
+    out = subprocess.run(["nm", "-u", EXE], check=True, capture_output=True, text=True).stdout
+    for sym in ("opf_ctx_create", "opf_slot_setup", "opf_vel_predictor", "opf_vel_corrector",
+                "opf_vel_transform", "opf_vel_nonlinear", "opf_put_state", "opf_get_state"):
+        assert sym in out, sym
+
+
+def test_fortran_host_rejects_missing_arguments():
+    if not os.access(EXE, os.X_OK):
+        pytest.skip("Fortran host not built")
+    r = subprocess.run([EXE], capture_output=True, text=True)
+    assert r.returncode != 0 or "usage" in (r.stdout + r.stderr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,nsteps,key", [("mini", 10, "state_10"), ("c0", 1, "state_1"),
+                                             ("c0", 100, "state_100")])
+def test_fortran_host_matches_reference(name, nsteps, key):
+    if not os.access(EXE, os.X_OK):
+        pytest.skip("Fortran host not built")
+    g = load_golden(name)
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    H = (2 * K - 1) * M - (K - 1)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, fser = (os.path.join(td, x) for x in ("in.bin", "out.bin", "series.txt"))
+        _write_state(fin, g["ic"])
+        args = [EXE, str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])),
+                repr(float(g["dt"])), str(nsteps), fin, fout, fser]
+        print("fortran host:", name, nsteps, flush=True)
+        subprocess.run(args, check=True, timeout=120)
+        u = _read_state(fout, N, H)
+        ser = np.loadtxt(fser, ndmin=2)
+    tol = 1e-12 if nsteps == 1 else 1e-10
+    for c in range(3):
+        assert rel_err(u[c], g[key][c]) <= tol, c
+    # correctors per step equal the reference's (series column 8: ncorr)
+    ref_ncorr = g["series"][:nsteps, 8].astype(int)
+    assert np.array_equal(ser[:, 1].astype(int), ref_ncorr)
