@@ -110,6 +110,83 @@ def cpu_baseline(name, wl, ic_fields, target_s):
                 sample=f"{n} steps of the same workload, numpy oracle, 1 process, {secs:.1f} s")
 
 
+# batched mode: many (32,24,12) states with their own Re, alpha and initial
+# energy, advanced together (every launch covers all slots)
+BATCH = dict(N=32, K=24, M=12, dt=0.01, seed=2024)
+
+
+def batched_leg(nstates, steps):
+    """Batched throughput, the same states advanced one at a time (plain
+    per-state loop), and the parity of slot 0, which holds the reference's
+    (32,24,12) case, against the reference's own energy series."""
+    import opf
+    from opf import ic
+    N, K, M, dt = BATCH["N"], BATCH["K"], BATCH["M"], BATCH["dt"]
+    rng = np.random.default_rng(BATCH["seed"])
+    params = []
+    for i in range(nstates):
+        Re = float(rng.uniform(1800.0, 3000.0))
+        alpha = float(rng.choice([1.0, 1.25, 1.7]))
+        E0 = float(10.0 ** rng.uniform(-5.0, -2.0))
+        params.append((Re, alpha, E0, i + 1))
+    params[0] = (2400.0, 1.25, 1e-3, 1)           # the reference's (32,24,12) case
+    ctx = opf.Context(N, K, M, capacity=nstates)
+    gpath = os.path.join(REPO, "tests", "golden", "golden_c0.npz")
+    golden = None
+    if os.path.exists(gpath):
+        with np.load(gpath) as g:
+            golden = dict(ic=g["ic"], series=g["series"])
+    t0 = time.perf_counter()
+    me = None
+    for s, (Re, alpha, E0, seed) in enumerate(params):
+        ctx.setup(s, Re, alpha, dt)
+        if me is None:
+            me = ctx.mesh()
+        if s == 0 and golden is not None:
+            f = golden["ic"]
+        else:
+            f = ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
+        ctx.put_state(s, f[0], f[1], f[2])
+    setup_s = time.perf_counter() - t0
+    ctx.run(2, diag_every=10)                       # warm-up (untimed)
+    t0 = time.perf_counter()
+    ctx.run(steps, diag_every=10)
+    secs = time.perf_counter() - t0
+    value = nstates * steps / secs
+    parity = {"reference": "golden_c0.npz (slot 0)", "checked_records": 0}
+    if golden is not None:
+        d = ctx.diag(0)
+        rows = [r for r in d if int(r[0]) < golden["series"].shape[0]]
+        if rows:
+            d = np.array(rows)
+            ref = golden["series"][d[:, 0].astype(int)]
+            dE = np.abs(d[:, 2] - ref[:, 2]) / ref[:, 2]
+            parity.update(checked_records=len(rows), max_rel_dE=float(dE.max()),
+                          ok=bool(dE.max() <= 1e-10))
+    ncorr = [ctx.status(s)["last_ncorr"] for s in range(nstates)]
+    ctx.close()
+    # plain per-state loop: the same kind of state advanced alone
+    one = opf.Context(N, K, M)
+    Re, alpha, E0, seed = params[1]
+    one.setup(0, Re, alpha, dt)
+    f = ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
+    one.put_state(0, *f)
+    one.run(5)
+    n1 = max(50, steps)
+    t0 = time.perf_counter()
+    one.run(n1, diag_every=10)
+    alone = n1 / (time.perf_counter() - t0)
+    one.close()
+    return {"metric": "batched state-steps/s", "value": value, "unit": "state-steps/s",
+            "config": {"workload": f"{nstates} states at (N,K,M)=({N},{K},{M}), Re~U[1800,3000], "
+                                   f"alpha in {{1.0,1.25,1.7}}, E0 log-uniform [1e-5,1e-2], dt={dt}, "
+                                   f"{steps} steps, corrector to convergence",
+                       "states": nstates, "steps": steps},
+            "ms_per_step": 1000.0 * secs / steps, "setup_s": setup_s,
+            "plain_per_state_loop": alone, "speedup_vs_plain_loop": value / alone,
+            "mean_last_ncorr": float(np.mean(ncorr)), "parity": parity}
+
+
 def load_traffic(kernel, workload):
     path = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
     if os.path.exists(path):
@@ -126,6 +203,9 @@ def main():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-states", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes five times (calibrates the "
@@ -201,6 +281,11 @@ def main():
     B_alg = survey_bytes_per_step(N, K, M, ncorr_mean)
     step_gbs = B_alg / (secs / args.steps) / 1e9
 
+    batched = None
+    if not args.no_batched:
+        ctx.close()
+        batched = batched_leg(args.batch_states, args.batch_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(name, wl, fields, args.cpu_seconds)
@@ -231,6 +316,7 @@ def main():
         "step_roofline": {"B_alg_bytes": B_alg, "achieved_GBs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "kernels_us": {k: round(v["us_avg"], 3) for k, v in kt.items()},
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
