@@ -79,6 +79,30 @@ def _div_r(c, times=1):
     return c[times:] if len(c) > times else np.zeros(1)
 
 
+def _radial_profiles(r, mm):
+    """Values at r of the potentials' derivative combinations for azimuthal
+    index mm: psi/r, psi', phi/r, phi', and the Laplacian term of u_z."""
+    N = r.size
+    psi = _poly_rm(mm, 2)
+    phi = _poly_rm(mm, 3)
+    dpsi = P.polyder(psi)
+    dphi = P.polyder(phi)
+    ddphi = P.polyder(dphi)
+    # (r phi' - m^2 phi) / r^2
+    q = P.polysub(P.polymulx(dphi), mm * mm * phi)
+    lap_rest = _div_r(q, 2)
+    v_dpsi = P.polyval(r, dpsi)
+    v_dphi = P.polyval(r, dphi)
+    v_lap = P.polyval(r, ddphi) + P.polyval(r, lap_rest)
+    if mm > 0:
+        v_psi_r = P.polyval(r, _div_r(psi, 1))
+        v_phi_r = P.polyval(r, _div_r(phi, 1))
+    else:
+        v_psi_r = np.zeros(N)
+        v_phi_r = np.zeros(N)
+    return v_psi_r, v_dpsi, v_phi_r, v_dphi, v_lap
+
+
 def seeded_perturbation(r, intrdr, K, M, alpha, seed, target_energy, Mp=1):
     """Return (ur, ut, uz), complex arrays of shape (N, H), with the given
     energy.  Deterministic in (r, K, M, Mp, alpha, seed, target_energy)."""
@@ -87,41 +111,24 @@ def seeded_perturbation(r, intrdr, K, M, alpha, seed, target_energy, Mp=1):
     ks, ms = harmonic_km(K, M)
     H = ks.size
     rng = np.random.default_rng(seed)
+    g = rng.standard_normal((H, 4))      # the per-harmonic draws of 4, in harmonic order
+    env = np.exp(-(ks * ks + ms * ms) / 16.0)
+    cpsi = (g[:, 0] + 1j * g[:, 1]) * env
+    cphi = (g[:, 2] + 1j * g[:, 3]) * env
+    cpsi[0] = cpsi[0].real
+    cphi[0] = cphi[0].real
     ur = np.zeros((N, H), np.complex128)
     ut = np.zeros((N, H), np.complex128)
     uz = np.zeros((N, H), np.complex128)
-    for h in range(H):
-        k, m = int(ks[h]), int(ms[h])
-        env = np.exp(-(k * k + m * m) / 16.0)
-        g = rng.standard_normal(4)
-        cpsi = (g[0] + 1j * g[1]) * env
-        cphi = (g[2] + 1j * g[3]) * env
-        if k == 0 and m == 0:
-            cpsi, cphi = cpsi.real, cphi.real
+    for m in range(M):
+        sel = np.nonzero(ms == m)[0]
         mm = m * Mp
-        ak = alpha * k
-        psi = _poly_rm(mm, 2)
-        phi = _poly_rm(mm, 3)
-        dpsi = P.polyder(psi)
-        dphi = P.polyder(phi)
-        ddphi = P.polyder(dphi)
-        # (r phi' - m^2 phi) / r^2
-        q = P.polysub(P.polymulx(dphi), mm * mm * phi)
-        lap_rest = _div_r(q, 2)
-        v_psi = P.polyval(r, psi)
-        v_dpsi = P.polyval(r, dpsi)
-        v_dphi = P.polyval(r, dphi)
-        v_ddphi = P.polyval(r, ddphi)
-        v_lap = v_ddphi + P.polyval(r, lap_rest)
-        if mm > 0:
-            v_psi_r = P.polyval(r, _div_r(psi, 1))
-            v_phi_r = P.polyval(r, _div_r(phi, 1))
-        else:
-            v_psi_r = np.zeros(N)
-            v_phi_r = np.zeros(N)
-        ur[:, h] = 1j * mm * v_psi_r * cpsi + 1j * ak * v_dphi * cphi
-        ut[:, h] = -v_dpsi * cpsi - (mm * ak) * v_phi_r * cphi
-        uz[:, h] = -v_lap * cphi
+        v_psi_r, v_dpsi, v_phi_r, v_dphi, v_lap = _radial_profiles(r, mm)
+        ak = alpha * ks[sel]
+        cp, cf = cpsi[sel], cphi[sel]
+        ur[:, sel] = (1j * mm * v_psi_r)[:, None] * cp[None, :] + (1j * v_dphi[:, None] * ak[None, :]) * cf[None, :]
+        ut[:, sel] = (-v_dpsi)[:, None] * cp[None, :] - (v_phi_r[:, None] * (mm * ak)[None, :]) * cf[None, :]
+        uz[:, sel] = (-v_lap)[:, None] * cf[None, :]
     ur[:, 0] = 0.0
     ut[:, 0] = ut[:, 0].real
     uz[:, 0] = uz[:, 0].real
