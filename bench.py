@@ -1,17 +1,19 @@
 """Benchmark of the Openpipeflow time step on one MI355X.
 
-Workload (default): the plain single-state run at configs[1] size,
-(N, K, M) = (64, 64, 48), Mp = 1, alpha = 0.5, Re = 5300, dt = 0.005,
-implicitness 0.51, constant flux, seeded perturbation of energy 1e-2, with
-the corrector iterated to convergence as main.f90:35-41 does.  The timed
-region is `--steps` consecutive steps of the device-resident loop after
-`--warmup` steps; the state is in HBM before timing starts.
+Headline workload: one state at (N, K, M) = (64, 64, 48), Mp = 1,
+alpha = 0.5, Re = 5300, dt = 0.005, implicitness 0.51, constant flux,
+seeded perturbation of energy 1e-2, with the corrector iterated to
+convergence as main.f90:35-41 does.  The timed region is `--steps`
+consecutive steps of the device-resident loop after `--warmup` steps; the
+state is in HBM before timing starts.
 
 Printed: one JSON line with the metric (state-steps/s), the parity check of
 the energy / friction-velocity series against the reference's own run
 (tests/golden/golden_c1.npz, |dE|/E <= 1e-10), the HBM roofline of the
-dominant kernel (HIP events on the library's stream), and the reference CPU
-code timed on this host's cores on a bounded sample of the same workload.
+dominant kernel (HIP events on the library's stream), the reference CPU
+code timed on this host's cores on a bounded sample of the same workload,
+and two more legs: batched mode (512 mixed states at (32,24,12)) and the
+request server (Poisson arrivals of three sizes, p50/p99 latency).
 """
 from __future__ import annotations
 
@@ -187,6 +189,91 @@ def batched_leg(nstates, steps):
             "mean_last_ncorr": float(np.mean(ncorr)), "parity": parity}
 
 
+# serving: Poisson arrivals of integration requests of three sizes (5:3:2),
+# one corrector per step (steporbit), T ~ U[5, 50] at dt = 0.01
+SERVE = dict(classes=[(32, 24, 12), (48, 48, 24), (64, 64, 48)], ratio=(5, 3, 2), dt=0.01, seed=77)
+
+
+def serving_leg(nreq, rate, budget_gb, chunk):
+    """p50/p99 request latency and throughput of the request server; parity
+    of a served request against the reference, and of a served request
+    against the same integration run alone."""
+    import opf
+    from opf import ic
+    classes, ratio, dt = SERVE["classes"], np.array(SERVE["ratio"], float), SERVE["dt"]
+    rng = np.random.default_rng(SERVE["seed"])
+    per = [opf.bytes_needed(N, K, M, 1) for (N, K, M) in classes]
+    w = ratio * np.array(per, float)
+    caps = [max(1, int(budget_gb * 1e9 * wi / w.sum() / pi)) for wi, pi in zip(w, per)]
+    meshes = [opf.host_mesh(N) for (N, _, _) in classes]
+    reqs = []
+    t = 0.0
+    for i in range(nreq):
+        cls = int(rng.choice(len(classes), p=ratio / ratio.sum()))
+        Re = float(rng.uniform(2000.0, 5300.0))
+        alpha = float(rng.choice([0.5, 1.25]))
+        nsteps = int(round(float(rng.uniform(5.0, 50.0)) / dt))
+        E0 = float(10.0 ** rng.uniform(-5.0, -3.0))
+        t += float(rng.exponential(1.0 / rate))
+        N, K, M = classes[cls]
+        st = ic.seeded_perturbation(meshes[cls]["r"], meshes[cls]["intrdr"], K, M, alpha, 1000 + i, E0)
+        reqs.append(dict(cls=cls, Re=Re, alpha=alpha, nsteps=nsteps, state=st, arrival=t))
+    gpath = os.path.join(REPO, "tests", "golden", "golden_c0.npz")
+    golden = None
+    if os.path.exists(gpath):
+        with np.load(gpath) as g:
+            golden = dict(ic=g["ic"], state_100=g["state_100"])
+        reqs[0].update(cls=0, Re=2400.0, alpha=1.25, nsteps=100, state=tuple(golden["ic"]))
+    srv = opf.Server(classes, caps, chunk=chunk)
+    t0 = time.perf_counter()
+    ids = []
+    for r in reqs:
+        wait = r["arrival"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(r["cls"], r["Re"], r["alpha"], dt, r["nsteps"], r["state"], corr_fixed=1))
+    while not all(srv.poll(i)["done"] for i in ids):
+        time.sleep(0.005)
+    polls = [srv.poll(i) for i in ids]
+    lat = np.array([p["t_done"] - p["t_submit"] for p in polls])
+    span = max(p["t_done"] for p in polls) - min(p["t_submit"] for p in polls)
+    steps = sum(r["nsteps"] for r in reqs)
+    failed = sum(1 for p in polls if p["status"] != 0)
+    parity = {}
+    if golden is not None:
+        u = srv.result(ids[0], 0)
+        parity["reference_c0_100_steps_rel_err"] = max(
+            float(np.abs(u[k] - golden["state_100"][k]).max() / np.abs(golden["state_100"][k]).max())
+            for k in range(3))
+    # a request of the mix against the same integration alone
+    j = next(k for k in range(1, len(reqs)) if reqs[k]["cls"] == 0 and polls[k]["status"] == 0)
+    u = srv.result(ids[j], 0)
+    srv.close()
+    r = reqs[j]
+    N, K, M = classes[0]
+    one = opf.Context(N, K, M)
+    one.setup(0, r["Re"], r["alpha"], dt)
+    one.put_state(0, *r["state"][:3])
+    one.run(r["nsteps"], corr_fixed=1)
+    v = one.get_state(0)
+    parity["alone_vs_in_mix_bitwise"] = bool(all(np.array_equal(u[k], v[k]) for k in range(3)))
+    one.close()
+    by_cls = {}
+    for c, (N, K, M) in enumerate(classes):
+        lc = lat[[k for k, rq in enumerate(reqs) if rq["cls"] == c]]
+        if lc.size:
+            by_cls[f"({N},{K},{M})"] = dict(requests=int(lc.size), p50_s=float(np.percentile(lc, 50)),
+                                            p99_s=float(np.percentile(lc, 99)))
+    return {"metric": "request latency p50/p99 (s) and served state-steps/s",
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "throughput": steps / span, "unit": "state-steps/s",
+            "config": {"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, sizes "
+                                   f"{[list(c) for c in classes]} in ratio 5:3:2, Re~U[2000,5300], "
+                                   f"alpha in {{0.5,1.25}}, T~U[5,50] at dt={dt}, one corrector per step",
+                       "slots": caps, "chunk_steps": chunk, "hbm_budget_gb": budget_gb},
+            "failed": failed, "by_class": by_cls, "parity": parity}
+
+
 def load_traffic(kernel, workload):
     path = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
     if os.path.exists(path):
@@ -206,6 +293,11 @@ def main():
     ap.add_argument("--batch-states", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=48.0)
+    ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes five times (calibrates the "
@@ -282,9 +374,13 @@ def main():
     step_gbs = B_alg / (secs / args.steps) / 1e9
 
     batched = None
+    ctx.close()
     if not args.no_batched:
-        ctx.close()
         batched = batched_leg(args.batch_states, args.batch_steps)
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_leg(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_chunk)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -303,7 +399,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded divergence-free perturbation of Hagen-Poiseuille flow)",
-        "config": {"workload": f"plain single-state time step, configs[1] size "
+        "config": {"workload": f"plain single-state time step at "
                                f"(N,K,M)=({N},{K},{M}), Re={Re}, alpha={alpha}, dt={dt}, E0={E0}",
                    "N": N, "K": K, "M": M, "Re": Re, "alpha": alpha, "dt": dt,
                    "corrector": "iterated to convergence (main.f90:35-41)",
@@ -317,6 +413,7 @@ def main():
                           "frac": step_gbs / HBM_PEAK_GBS},
         "kernels_us": {k: round(v["us_avg"], 3) for k, v in kt.items()},
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -144,6 +144,32 @@ int opf_tim_lumesh_get(opf_ctx* ctx, int slot, int which, double* ab, int* ipiv)
  * BC as for that matrix, on a host coll field (in place). */
 int opf_tim_lumesh_invert(opf_ctx* ctx, int slot, int which, double* b_re, double* b_im);
 
+/* ---- request server (continuous batching of integrations) ----------------
+ * A request integrates one state for `nsteps` steps with its own Re, alpha
+ * and dt (the time integration steporbit hands to the stepper,
+ * newton.f90:743-787, with corr_fixed = 1 correctors per step there;
+ * 0 = to convergence).  The server holds one context per resolution class
+ * (N, K, M of nkm[3i..3i+2]) with capacity[i] slots, sized by the caller
+ * with opf_ctx_bytes_needed; a scheduler thread admits queued requests into
+ * free slots, advances all occupied slots of all classes together in chunks
+ * of at most `chunk` steps, and retires finished ones.  Results do not depend
+ * on what else is being served.  Thread-safe; no counterpart in the
+ * reference (its states are integrated one per process). */
+typedef struct opf_server opf_server;
+const char* opf_server_last_error(void);
+int opf_server_create(opf_server** out, int nclass, const int* nkm, const int* capacity, int chunk);
+int opf_server_destroy(opf_server* srv);
+int opf_server_submit(opf_server* srv, int cls, double Re, double alpha, double dt, int nsteps, int corr_fixed,
+                      const double* ur_re, const double* ur_im, const double* ut_re, const double* ut_im,
+                      const double* uz_re, const double* uz_im, long long* id);
+/* done = 1 once finished; status 0 = ok; times in seconds since creation. */
+int opf_server_poll(opf_server* srv, long long id, int* done, int* status, double* t_submit, double* t_admit,
+                    double* t_done);
+/* Final state of a finished request (coll planes), then forgets the request. */
+int opf_server_result(opf_server* srv, long long id, double* ur_re, double* ur_im, double* ut_re,
+                      double* ut_im, double* uz_re, double* uz_im, double* t);
+int opf_server_stats(opf_server* srv, int* queued, int* running, long long* state_steps);
+
 /* ---- timing hooks --------------------------------------------------------- */
 
 /* Record HIP events around every launch of the named kernel on the

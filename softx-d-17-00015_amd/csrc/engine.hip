@@ -21,6 +21,7 @@
 
 #include "../../include/opf.h"
 #include "kernels.hpp"
+#include "engine_internal.hpp"
 
 using namespace opf;
 
@@ -414,45 +415,78 @@ void put_state(opf_ctx* c, int slot, const double* const* f, double t) {
   c->occupied[slot] = 1;
 }
 
-void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
-  if (nsteps < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
+}  // namespace
+
+// ---- the device-resident loop in phases (also used by the server) --------
+namespace opf {
+
+RunPlan run_begin(opf_ctx* c, const std::vector<int>& slots, const std::vector<int>& nsteps,
+                  const std::vector<int>& corr_fixed, int diag_every, bool strict) {
+  if (slots.size() != nsteps.size() || slots.size() != corr_fixed.size())
+    throw OpfError(OPF_ERR_ARG, "run plan: size mismatch");
   auto ctl = read_ctl(c);
+  RunPlan p;
   int lo = c->cap, hi = -1;
-  for (int s : slots) {
+  for (size_t i = 0; i < slots.size(); ++i) {
+    const int s = slots[i];
     check_slot(c, s);
     if (!c->occupied[s]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    if (nsteps[i] < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
     SlotCtl& k = ctl[s];
-    if (k.status != 0) throw OpfError(OPF_ERR_CORRECTOR, "slot failed earlier");
-    if (nsteps == 0) continue;
+    if (k.status != 0) {
+      if (strict) throw OpfError(OPF_ERR_CORRECTOR, "slot failed earlier");
+      continue;
+    }
+    if (nsteps[i] == 0) continue;
     k.active = 1;
     k.mode = MODE_PRED;
-    k.target_steps = k.step + nsteps;
-    k.corr_fixed = corr_fixed;
+    k.target_steps = k.step + nsteps[i];
+    k.corr_fixed = corr_fixed[i];
     k.diag_every = diag_every;
     k.host_driven = 0;
+    p.slots.push_back(s);
+    p.max_steps = std::max(p.max_steps, nsteps[i]);
     lo = std::min(lo, s);
     hi = std::max(hi, s);
   }
-  if (hi < 0) return;
+  if (hi < 0) return p;
   write_ctl(c, ctl);
-  const int ns = hi - lo + 1;
+  p.lo = lo;
+  p.ns = hi - lo + 1;
+  return p;
+}
+
+void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
+  for (long i = 0; i < iterations && p.ns > 0; ++i) launch_iteration(c, p.lo, p.ns);
+}
+
+int run_remaining(opf_ctx* c, const RunPlan& p) {
+  if (p.ns == 0) return 0;
+  auto ctl = read_ctl(c);
+  int remaining = 0;
+  for (int s : p.slots)
+    if (ctl[s].active) remaining = std::max(remaining, std::max(1, ctl[s].target_steps - ctl[s].step));
+  return remaining;
+}
+
+}  // namespace opf
+
+namespace {
+
+void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
+  if (nsteps < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
+  const RunPlan p = run_begin(c, slots, std::vector<int>(slots.size(), nsteps),
+                              std::vector<int>(slots.size(), corr_fixed), diag_every, true);
+  if (p.ns == 0) return;
   // Most steps take one corrector; enqueue 2 iterations per step, then top
   // up until every slot has reached its target.
   long chunk = 2L * nsteps;
-  for (int round = 0; round < 64; ++round) {
-    for (long i = 0; i < chunk; ++i) launch_iteration(c, lo, ns);
-    ctl = read_ctl(c);
-    int remaining = 0;
-    bool any = false;
-    for (int s : slots) {
-      if (ctl[s].active) {
-        any = true;
-        remaining = std::max(remaining, ctl[s].target_steps - ctl[s].step);
-      }
-    }
-    if (!any) break;
-    chunk = std::max(2L, 2L * remaining);
+  for (int round = 0; round < 64 && chunk > 0; ++round) {
+    run_enqueue(c, p, chunk);
+    const int remaining = run_remaining(c, p);
+    chunk = remaining > 0 ? std::max(2L, 2L * remaining) : 0;
   }
+  auto ctl = read_ctl(c);
   for (int s : slots)
     if (ctl[s].status != 0) {
       char msg[128];

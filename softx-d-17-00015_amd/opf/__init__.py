@@ -2,3 +2,4 @@
 interface over the HIP library libopf.so (include/opf.h)."""
 from .engine import Context, bytes_needed, host_mesh, LDP, LDM, LDZ, LNP, DIAG_FIELDS  # noqa: F401
 from . import ic  # noqa: F401
+from .serve import Server  # noqa: F401

@@ -48,6 +48,14 @@ PROTOTYPES = {
     "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
     "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
     "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
+    "opf_server_last_error": (C.c_char_p, []),
+    "opf_server_create": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, C.c_int]),
+    "opf_server_destroy": (C.c_int, [vp]),
+    "opf_server_submit": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    dp, dp, dp, dp, dp, dp, C.POINTER(C.c_longlong)]),
+    "opf_server_poll": (C.c_int, [vp, C.c_longlong, ip, ip, dp, dp, dp]),
+    "opf_server_result": (C.c_int, [vp, C.c_longlong, dp, dp, dp, dp, dp, dp, dp]),
+    "opf_server_stats": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong)]),
 }
 
 

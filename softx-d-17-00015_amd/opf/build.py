@@ -19,7 +19,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libopf.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["mesh.cpp", "kernels_step.hip", "kernels_harmonic.hip", "kernels_fft.hip", "engine.hip"]
+SOURCES = ["mesh.cpp", "kernels_step.hip", "kernels_harmonic.hip", "kernels_fft.hip", "engine.hip", "serve.cpp"]
 ARCH = os.environ.get("OPF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -47,7 +47,7 @@ def build(verbose: bool = False) -> str:
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -217,7 +217,7 @@ def test_c1_series(opf):
 
 
 def test_large_size_properties(opf):
-    """configs[2] size (N, K, M) = (96, 256, 128): no oracle run at this size;
+    """The largest size, (N, K, M) = (96, 256, 128): no oracle run at this size;
     checks the properties the method fixes: the influence correction makes
     u = 0 at the wall to machine precision, the (0,0) mode keeps zero net
     flux, harmonic 0 keeps u_r = 0 and real u_t, u_z, and the transforms
