@@ -53,6 +53,8 @@ class Server:
 
     def submit(self, cls, Re, alpha, dt, nsteps, state, corr_fixed=1):
         """Queue an integration of `state` (ur, ut, uz) for nsteps steps."""
+        if not 0 <= int(cls) < len(self.classes):
+            raise ServerError(f"OPF_ERR_ARG: no resolution class {cls}")
         N, _, _ = self.classes[cls]
         parts = []
         for a in state[:3]:
