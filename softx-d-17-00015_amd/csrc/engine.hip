@@ -228,7 +228,7 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
 
 void launch_step(opf_ctx* c, int s0, int ns) {
   const int per = (c->st.ngroups + 7) / 8;   // XCD-aware order inside the kernel
-  auto k = c->st.gp == 1 ? k_step_g1 : k_step_g2;
+  auto k = c->st.gp == 1 ? k_step_g1 : (c->N * 4 <= 2 * STEP_THREADS ? k_step_g2 : k_step_g2x);
   launch(c, "step", k, dim3(8 * per, 1, ns), dim3(STEP_THREADS), c->st.smem, c->G, c->B, c->st, s0);
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }

@@ -88,7 +88,8 @@ __global__ void k_ainv(Geo G, Bufs B, int slot0);
 __global__ void k_curl(Geo G, Bufs B, int slot0);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
-__global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);
+__global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
+__global__ void k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 128
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
 // transforms: kernels specialised to the radix plans of (Z, T), chosen once

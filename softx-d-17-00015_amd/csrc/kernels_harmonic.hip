@@ -163,61 +163,35 @@ __device__ __forceinline__ Seg seg_of(const Stream& st, int g, int& c) {
   return S;
 }
 
-// Loader lane `lane` (0..63) fetches element `lane` of every block of chunk
-// g into registers r[block]; ld_store writes them into ring buffer g & 1.
+// Loader wave: chunk g of the stream is copied from global memory straight
+// into ring buffer g & 1 (global_load_lds_dwordx4, no registers): lane
+// `lane` moves element `lane` of every (system, pair) block.  The copy is
+// issued while the solver works on chunk g-1 and is complete at the
+// workgroup barrier that ends that chunk.
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
 template <int GP>
-__device__ __forceinline__ void ld_issue(const Bufs& B, const Stream& st, int g, int slot, int p0, int np, int P,
-                                         int N, int lane, double2 (&r)[3 * GP]) {
+__device__ __forceinline__ void ld_dma(const Lds& L, const Bufs& B, const Stream& st, int g, int slot, int p0,
+                                       int np, int P, int N, int lane) {
   if (g >= st.total) return;
   int c;
   const Seg S = seg_of(st, g, c);
   const int per = seg_ncol(S, c, N) * S.qn;
   if (lane >= per) return;
   const int col = lane / S.qn, q = lane - col * S.qn;
+  const int BS = S.cs * S.qn + 1;
+  double2* buf = L.FC + (size_t)(g & 1) * L.bufd2;
 #pragma unroll
   for (int sys = 0; sys < 3; ++sys)
 #pragma unroll
     for (int pl = 0; pl < GP; ++pl) {
       if (sys < S.nsys && pl < np) {
         const int mat = S.m0 + sys;
-        r[sys * GP + pl] = S.fwd ? fw_block(B, slot, mat, p0 + pl, P, N)[(size_t)c * S.cs * FW_Q + lane]
-                                 : bw_block(B, slot, mat, p0 + pl, P, N)[(size_t)q * N + (N - 1 - c * S.cs - col)];
+        const double2* src = S.fwd ? fw_block(B, slot, mat, p0 + pl, P, N) + (size_t)c * S.cs * FW_Q + lane
+                                   : bw_block(B, slot, mat, p0 + pl, P, N) + (size_t)q * N + (N - 1 - c * S.cs - col);
+        __builtin_amdgcn_global_load_lds((const void*)src, (lds_ptr_t)(buf + (sys * GP + pl) * BS), 16, 0, 0);
       }
     }
-}
-
-template <int GP>
-__device__ __forceinline__ void ld_store(const Lds& L, const Stream& st, int g, int np, int N, int lane,
-                                         const double2 (&r)[3 * GP]) {
-  if (g >= st.total) return;
-  int c;
-  const Seg S = seg_of(st, g, c);
-  const int per = seg_ncol(S, c, N) * S.qn;
-  if (lane >= per) return;
-  const int BS = S.cs * S.qn + 1;
-  double2* buf = L.FC + (size_t)(g & 1) * L.bufd2;
-#pragma unroll
-  for (int sys = 0; sys < 3; ++sys)
-#pragma unroll
-    for (int pl = 0; pl < GP; ++pl)
-      if (sys < S.nsys && pl < np) buf[(sys * GP + pl) * BS + lane] = r[sys * GP + pl];
-}
-
-// One step of the ring while the solver works on chunk g: r1 holds chunk
-// g+1 (fetched two chunk times ago), r2 chunk g+2.  Store r1, fetch chunk
-// g+3 into it and swap, so that r1, r2 again hold the next two chunks.
-template <int GP>
-__device__ __forceinline__ void ld_step(const Lds& L, const Bufs& B, const Stream& st, int g, int slot, int p0,
-                                        int np, int P, int N, int lane, double2 (&r1)[3 * GP],
-                                        double2 (&r2)[3 * GP]) {
-  ld_store<GP>(L, st, g + 1, np, N, lane, r1);
-  ld_issue<GP>(B, st, g + 3, slot, p0, np, P, N, lane, r1);
-#pragma unroll
-  for (int i = 0; i < 3 * GP; ++i) {
-    const double2 t = r1[i];
-    r1[i] = r2[i];
-    r2[i] = t;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -362,8 +336,7 @@ __device__ __forceinline__ void bwd_chunk(const double2* F, double* x, int xs, i
 template <int GP>
 __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const Stream& st, const Seg& SF,
                                             const Seg& SB, int kue, int mo, bool pressure, int gbase, int slot,
-                                            int p0, int np, int P, int K, int N, double2 (&ra)[3 * GP],
-                                            double2 (&rb)[3 * GP]) {
+                                            int p0, int np, int P, int K, int N) {
   const int tid = threadIdx.x;
   const int nsolv = 2 * L.GH * SF.nsys;
   const bool solver = tid < nsolv;
@@ -389,6 +362,7 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
   const int BSF = SF.cs * SF.qn + 1;
   for (int c = 0; c < SF.nch; ++c) {
     const int g = gbase + c;
+    if (loader) ld_dma<GP>(L, B, st, g + 1, slot, p0, np, P, N, lane);
     if (act) {
       const double2* F = L.FC + (size_t)(g & 1) * L.bufd2 + (size_t)blk * BSF;
       const int j0 = c * SF.cs, ncol = seg_ncol(SF, c, N);
@@ -400,7 +374,6 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
         default: fwd_chunk<4>(F, x, xs, j0, ncol, N, w); break;
       }
     }
-    if (loader) ld_step<GP>(L, B, st, g, slot, p0, np, P, N, lane, ra, rb);
     __syncthreads();
   }
   double v[9];
@@ -418,6 +391,7 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
   const int BSB = SB.cs * SB.qn + 1;
   for (int c = 0; c < SB.nch; ++c) {
     const int g = gbase + SF.nch + c;
+    if (loader) ld_dma<GP>(L, B, st, g + 1, slot, p0, np, P, N, lane);
     if (act) {
       const double2* F = L.FC + (size_t)(g & 1) * L.bufd2 + (size_t)blk * BSB;
       const int jtop = N - 1 - c * SB.cs, ncol = seg_ncol(SB, c, N);
@@ -432,7 +406,6 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
         default: bwd_chunk<8>(F, x, xs, jtop, ncol, v); break;
       }
     }
-    if (loader) ld_step<GP>(L, B, st, g, slot, p0, np, P, N, lane, ra, rb);
     __syncthreads();
   }
 }
@@ -445,7 +418,9 @@ __device__ __forceinline__ double pr0_of(const double2* uz0, size_t hs, int N, c
   return s / (-2.0);
 }
 
-template <int GP>
+// IT: (radial point, harmonic) items per thread in the parallel stages,
+// N * 2 * GP <= IT * STEP_THREADS.
+template <int GP, int IT>
 __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const StepTiles& ST, int slot0) {
   const int slot = slot0 + blockIdx.z;
   SlotCtl* ctl = &B.ctl[slot];
@@ -514,19 +489,40 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     L.META[2 * tid] = kue;
     L.META[2 * tid + 1] = mxo;
   }
-  for (int it = tid; it < NI; it += HT) {
+  // all global inputs of this thread's items in one batch: u_ (staged into
+  // V[0..2] for the stencils), the current iterate (a separate buffer only
+  // for the corrector), N and, for the corrector, N_
+  // (unconditional loads: in the predictor the current iterate is u_ and
+  // N_ is not used, and the repeated addresses hit the cache)
+  double2 vb[IT][3], vc[IT][3], vn[IT][3], vp[IT][3];
+  const double2* ucl = corr ? uc : ub;
+  const double2* npl = corr ? Np : Nc;
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = tid + q * HT;
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
-    double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0;
-    if (S.valid) {
-      const size_t o = (size_t)n * hs + S.h;
-      a0 = ub[o];
-      a1 = ub[csz + o];
-      a2 = ub[2 * csz + o];
+    const bool ok = it < NI && S.valid;
+    const size_t o = ok ? (size_t)n * hs + S.h : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      vb[q][c] = ub[c * csz + o];
+      vn[q][c] = Nc[c * csz + o];
+      vc[q][c] = ucl[c * csz + o];
+      vp[q][c] = npl[c * csz + o];
     }
-    Vat(L, 0, n, lh, N) = a0;
-    Vat(L, 1, n, lh, N) = a1;
-    Vat(L, 2, n, lh, N) = a2;
+    if (!ok) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vb[q][c] = make_double2(0.0, 0.0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int it = tid + q * HT;
+    if (it >= NI) continue;
+    const int n = it / L.GH, lh = it - n * L.GH;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Vat(L, c, n, lh, N) = vb[q][c];
   }
   __syncthreads();
 
@@ -542,10 +538,9 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   // ---- stage 1: effective N and RHS = Lt u_ + N (tim_meshmult) -----------
   // Results are held in registers until every thread has read its stencil
   // from u_ (V[0..2]), then written over it.
-  constexpr int MAXIT = (MAXN * 2 * GP + HT - 1) / HT;
-  double2 res[MAXIT][3];
+  double2 res[IT][3];
 #pragma unroll
-  for (int q = 0; q < MAXIT; ++q) {
+  for (int q = 0; q < IT; ++q) {
     const int it = tid + q * HT;
     res[q][0] = res[q][1] = res[q][2] = make_double2(0.0, 0.0);
     if (it >= NI) continue;
@@ -560,8 +555,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     const double ak = prm.alpha * (double)S.k;
     const double a = ak * me[mo.Ubase + n];
     const double b = me[mo.twor + n];
-    const double2 ucr = uc[o], uct = uc[csz + o], ucz = uc[2 * csz + o];
-    double2 nr = Nc[o], nt = Nc[csz + o], nz = Nc[2 * csz + o];
+    const double2 ucr = vc[q][0], uct = vc[q][1], ucz = vc[q][2];
+    double2 nr = vn[q][0], nt = vn[q][1], nz = vn[q][2];
     nr = make_double2(nr.x + a * ucr.y, nr.y - a * ucr.x);
     nt = make_double2(nt.x + a * uct.y, nt.y - a * uct.x);
     nz = make_double2((nz.x + a * ucz.y) + b * ucr.x, (nz.y - a * ucz.x) + b * ucr.y);
@@ -574,7 +569,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
       const double2 bur = Vat(L, 0, n, lh, N);
       const double2 but = Vat(L, 1, n, lh, N);
       const double2 buz = Vat(L, 2, n, lh, N);
-      double2 pr = Np[o], pt = Np[csz + o], pz = Np[2 * csz + o];
+      double2 pr = vp[q][0], pt = vp[q][1], pz = vp[q][2];
       pr = make_double2(pr.x + a * bur.y, pr.y - a * bur.x);
       pt = make_double2(pt.x + a * but.y, pt.y - a * but.x);
       pz = make_double2((pz.x + a * buz.y) + b * bur.x, (pz.y - a * buz.x) + b * bur.y);
@@ -630,17 +625,9 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     }
     if (S.h00 && n == 0) ctl->Pr0_hpf = pr_c;
   }
-  // the first two factor chunks: they land during the stages up to the
-  // pressure solve (fetched after the right-hand sides so that their
-  // registers are not live together)
-  double2 ra[3 * GP], rb[3 * GP];
-  if (loader) {
-    ld_issue<GP>(B, st, 0, slot, p0, np, P, N, lane, ra);
-    ld_issue<GP>(B, st, 1, slot, p0, np, P, N, lane, rb);
-  }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < MAXIT; ++q) {
+  for (int q = 0; q < IT; ++q) {
     const int it = tid + q * HT;
     if (it >= NI) continue;
     const int n = it / L.GH, lh = it - n * L.GH;
@@ -662,6 +649,9 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
 
   // ---- stage 2: pressure RHS = div(pm2rt(R+, R-), Rz) (vel_adjPPE(1)) ----
+  // the first factor chunk streams in meanwhile (the energy terms that
+  // shared its memory have been summed)
+  if (loader) ld_dma<GP>(L, B, st, 0, slot, p0, np, P, N, lane);
   for (int it = tid; it < NI; it += HT) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
@@ -688,22 +678,10 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     }
     Vat(L, 3, n, lh, N) = dv;
   }
-  // the first factor chunk enters the ring (the energy terms that shared
-  // its memory have been summed)
-  if (loader) {
-    ld_store<GP>(L, st, 0, np, N, lane, ra);
-    ld_issue<GP>(B, st, 2, slot, p0, np, P, N, lane, ra);
-#pragma unroll
-    for (int i = 0; i < 3 * GP; ++i) {   // ra: chunk 1, rb: chunk 2
-      const double2 t = ra[i];
-      ra[i] = rb[i];
-      rb[i] = t;
-    }
-  }
   __syncthreads();
 
   // ---- stage 3: pressure solve with LNp (tim_lumesh_invert(1,...)) -------
-  solve_phase<GP>(L, B, st, st.s[0], st.s[1], kueP, moP, true, 0, slot, p0, np, P, K, N, ra, rb);
+  solve_phase<GP>(L, B, st, st.s[0], st.s[1], kueP, moP, true, 0, slot, p0, np, P, K, N);
 
   // ---- stage 4: subtract grad p (rt2pm), zero the BC rows ----------------
   for (int it = tid; it < NI; it += HT) {
@@ -740,7 +718,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 
   // ---- stage 5: Helmholtz solves LDp, LDm, LDz ---------------------------
   solve_phase<GP>(L, B, st, st.s[2], st.s[3], kueH, moH, false, st.s[0].nch + st.s[1].nch, slot, p0, np, P, K,
-                  N, ra, rb);
+                  N);
 
   // ---- stage 6: influence-matrix correction (vel_adjPPE(2)) --------------
   if (tid < L.GH) {
@@ -881,10 +859,15 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 
 __global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0) {
-  step_body<1>(G, B, ST, slot0);
+  step_body<1, 2>(G, B, ST, slot0);
 }
 __global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0) {
-  step_body<2>(G, B, ST, slot0);
+  step_body<2, 2>(G, B, ST, slot0);
 }
+__global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0) {
+  step_body<2, 4>(G, B, ST, slot0);
+}
+
 }  // namespace opf
