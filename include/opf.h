@@ -128,7 +128,22 @@ int opf_vel_corrector(opf_ctx* ctx, int slot, double* dterr);
 int opf_get_nonlinear(opf_ctx* ctx, int slot, double* nr_re, double* nr_im, double* nt_re,
                       double* nt_im, double* nz_re, double* nz_im);
 
-/* tra_coll2phys = var_coll2spec + tra_spec2phys (variables.f90:184-197,
+/* var_coll_norm (variables.f90:718-751) of component comp (0 u_r, 1 u_th,
+ * 2 u_z) of the slot's current state: E, Ek(0:K-1), Em(0:M-1) (NULL
+ * skips).  io_write_energy sums E over the three components. */
+int opf_var_norm(opf_ctx* ctx, int slot, int comp, double* E, double* Ek, double* Em);
+/* io_write_friction (io.f90:627-639): Ub, Uc, u_tau of the current state. */
+int opf_friction(opf_ctx* ctx, int slot, double* Ub, double* Uc, double* Ufr);
+/* vel_transform's physical fields (velocity.f90:304-317) of the current
+ * state for var_null(2) hooks: which = 0..2 vel_r, vel_t, vel_z; 3..5
+ * vel_curlr, vel_curlt, vel_curlz; phys layout. */
+int opf_get_phys(opf_ctx* ctx, int slot, int which, double* phys);
+/* vel_N* += dN for the current evaluation (forcing of var_null(2) hooks,
+ * e.g. Coriolis.f90:54-71); coll layout. */
+int opf_add_nonlinear(opf_ctx* ctx, int slot, const double* nr_re, const double* nr_im, const double* nt_re,
+                      const double* nt_im, const double* nz_re, const double* nz_im);
+
+/* tra_coll2phys = var_coll2spec + tra_spec2phys/* tra_coll2phys = var_coll2spec + tra_spec2phys (variables.f90:184-197,
  * transform.fftw3.f90:65-102) and tra_phys2coll = tra_phys2spec +
  * var_spec2coll (transform.fftw3.f90:108-147, variables.f90:281-294), on
  * host arrays of one field. */

@@ -777,6 +777,127 @@ int opf_get_nonlinear(opf_ctx* c, int slot, double* nrr, double* nri, double* nt
   });
 }
 
+// the slot's current iterate, one component, on the host (coll planes)
+static void slot_field_to_host(opf_ctx* c, int slot, int comp, std::vector<double>& re, std::vector<double>& im) {
+  auto ctl = read_ctl(c);
+  re.assign((size_t)c->N * c->H, 0.0);
+  im.assign((size_t)c->N * c->H, 0.0);
+  const double2* u = c->B.U[ctl[slot].ucur] + (size_t)slot * 3 * c->G.coll + (size_t)comp * c->G.coll;
+  dev_to_coll(c, u, re.data(), im.data());
+}
+
+int opf_var_norm(opf_ctx* c, int slot, int comp, double* E, double* Ek, double* Em) {
+  // var_coll_norm (variables.f90:718-751): per harmonic b = (w f) . intrdr,
+  // halved for (0,0); Em(m) and Ek(|k|) accumulate in harmonic order;
+  // E = sum(Em)
+  return guard([&] {
+    check_slot(c, slot);
+    if (comp < 0 || comp > 2) throw OpfError(OPF_ERR_ARG, "component must be 0, 1 or 2");
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    SlotParams prm;
+    HIPCHK(hipMemcpy(&prm, c->B.prm + slot, sizeof prm, hipMemcpyDeviceToHost));
+    std::vector<double> re, im;
+    slot_field_to_host(c, slot, comp, re, im);
+    const int N = c->N;
+    std::vector<double> ek(c->K, 0.0), em(c->M, 0.0);
+    const double w = 4.0 * D_PI * D_PI / prm.alpha;
+    for (int hh = 0; hh < c->H; ++hh) {
+      int k, m;
+      if (hh < c->K) { m = 0; k = hh; }
+      else { const int q = hh - c->K; m = 1 + q / (2 * c->K - 1); k = q % (2 * c->K - 1) - (c->K - 1); }
+      double b = 0.0;
+      for (int n = 0; n < N; ++n) {
+        const size_t o = (size_t)hh * N + n;
+        const double f = w * (re[o] * re[o] + im[o] * im[o]);
+        b = b + f * c->mesh.intrdr[n];
+      }
+      if (k == 0 && m == 0) b = 0.5 * b;
+      em[m] = em[m] + b;
+      ek[k < 0 ? -k : k] = ek[k < 0 ? -k : k] + b;
+    }
+    double e = 0.0;
+    for (double v : em) e = e + v;
+    if (E) *E = e;
+    if (Ek) std::copy(ek.begin(), ek.end(), Ek);
+    if (Em) std::copy(em.begin(), em.end(), Em);
+  });
+}
+
+int opf_friction(opf_ctx* c, int slot, double* Ub, double* Uc, double* Ufr) {
+  // io_write_friction (io.f90:627-639) on the (0,0) mode of u_z; with
+  // constant flux Ub is vel_Pr0 of the current state (velocity.f90:356-357)
+  return guard([&] {
+    check_slot(c, slot);
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    SlotParams prm;
+    HIPCHK(hipMemcpy(&prm, c->B.prm + slot, sizeof prm, hipMemcpyDeviceToHost));
+    std::vector<double> re, im;
+    slot_field_to_host(c, slot, 2, re, im);
+    const int N = c->N;
+    const HostMesh& me = c->mesh;
+    const double* uz = re.data();   // harmonic 0 occupies the first N values
+    double ub = 0.0;
+    if (prm.const_flux) {
+      double s = 0.0;
+      for (int i = 0; i <= KL; ++i) s = s + uz[N - 1 - KL + i] * me.dr1[i + (KL + 1) * 1];
+      ub = s / (-2.0);
+    } else {
+      double s = 0.0;
+      for (int n = 0; n < N; ++n) s = s + uz[n] * me.intrdr[n];
+      ub = 0.5 + 2.0 * s;
+    }
+    double uc = 0.0;
+    for (int i = 0; i <= KL; ++i) uc = uc + uz[i] * me.dr0[i];
+    double f = 0.0;
+    for (int i = 0; i <= KL; ++i) f = f + uz[N - 1 - KL + i] * me.dr1[i + (KL + 1) * 1];
+    if (Ub) *Ub = ub;
+    if (Uc) *Uc = 1.0 + uc;
+    if (Ufr) *Ufr = std::sqrt(std::fabs((f - 2.0) / prm.Re));
+  });
+}
+
+int opf_get_phys(opf_ctx* c, int slot, int which, double* phys) {
+  // vel_r/t/z (which 0..2) and vel_curlr/t/z (3..5) of vel_transform
+  // (velocity.f90:304-317) for the slot's current iterate, phys layout
+  return guard([&] {
+    check_slot(c, slot);
+    if (which < 0 || which > 5 || !phys) throw OpfError(OPF_ERR_ARG, "which must be 0..5");
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    ensure_temps(c);
+    auto ctl = read_ctl(c);
+    const double2* src = which < 3 ? c->B.U[ctl[slot].ucur] + (size_t)slot * 3 * c->G.coll + (size_t)which * c->G.coll
+                                   : c->B.om + (size_t)slot * 3 * c->G.coll + (size_t)(which - 3) * c->G.coll;
+    const FftTiles& t = c->tiles;
+    launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+           c->B, t, src, c->tY);
+    launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
+           (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, (const double2*)c->tY, c->tP);
+    HIPCHK(hipMemcpyAsync(phys, c->tP, sizeof(double) * c->N * c->Z * c->T, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int opf_add_nonlinear(opf_ctx* c, int slot, const double* nrr, const double* nri, const double* ntr,
+                      const double* nti, const double* nzr, const double* nzi) {
+  // var_null(2) hooks add forcing to vel_N* after vel_nonlinear
+  // (Coriolis.f90:54-71); added here to the slot's current evaluation
+  return guard([&] {
+    check_slot(c, slot);
+    if (!nrr || !ntr || !nzr) throw OpfError(OPF_ERR_ARG, "null argument");
+    ensure_temps(c);
+    auto ctl = read_ctl(c);
+    double2* nraw = c->B.Nraw[ctl[slot].ncur] + (size_t)slot * 3 * c->G.coll;
+    const double* re[3] = {nrr, ntr, nzr};
+    const double* im[3] = {nri, nti, nzi};
+    for (int comp = 0; comp < 3; ++comp) {
+      coll_to_dev(c, re[comp], im[comp], c->tC);
+      launch(c, "add_coll", k_add_coll, dim3((c->H + 63) / 64, c->N, 1), dim3(64), 0, c->G,
+             nraw + (size_t)comp * c->G.coll, (const double2*)c->tC);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
 int opf_tra_coll2phys(opf_ctx* c, const double* cre, const double* cim, double* phys) {
   return guard([&] {
     if (!c || !cre || !phys) throw OpfError(OPF_ERR_ARG, "null argument");

@@ -87,6 +87,7 @@ __global__ void k_influence(Geo G, Bufs B, int slot0);
 __global__ void k_ainv(Geo G, Bufs B, int slot0);
 __global__ void k_curl(Geo G, Bufs B, int slot0);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
+__global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
 __global__ void k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 128

@@ -618,7 +618,13 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
         DiagRec* rec = B.diag + (size_t)slot * ctl->diag_cap + (ctl->ndiag % ctl->diag_cap);
         rec->step = ctl->step;
         rec->t = ctl->t;
-        rec->Ub = prm.const_flux ? pr0_of(uc + 2 * csz, hs, N, dr1c) : 0.0;
+        if (prm.const_flux) {
+          rec->Ub = pr0_of(uc + 2 * csz, hs, N, dr1c);
+        } else {   // io.f90:630
+          double q = 0.0;
+          for (int i = 0; i < N; ++i) q = q + uc[2 * csz + (size_t)i * hs].x * me[mo.intrdr + i];
+          rec->Ub = 0.5 + 2.0 * q;
+        }
         rec->Uc = 1.0 + s;
         rec->Ufr = sqrt(fabs((f - 2.0) / prm.Re));
       }

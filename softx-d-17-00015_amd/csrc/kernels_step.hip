@@ -1,15 +1,15 @@
-// Per-harmonic kernels of the time step: banded LU factorisation and solves,
-// the pressure-Poisson projection with the influence-matrix correction, the
-// predictor/corrector step, the curl, diagnostics and the on-device corrector
-// control.
+// Per-state set-up kernels and the small kernels around the fused step:
+// banded LU factorisation of the four matrices per (|k|, m) pair and its
+// solve-ready copy, the influence vectors and 4x4 inverses of the
+// influence-matrix method, the curl, imposesym, on-device corrector control
+// and diagnostics sums, the granular tim_lumesh_invert, and forcing added
+// to the nonlinear terms.
 //
 // Layout: a spectral (coll) field is stored harmonic-fastest, one complex
 // double2 per (n, h): element (slot, comp, n, h) at ((slot*3+comp)*N + n)*Hp + h.
-// One lane owns one harmonic, so every radial sweep reads and writes rows of
-// 64 consecutive harmonics: fully coalesced, and the transposes of the
-// reference's MPI code (variables.f90:184-370) disappear.  The LU factors are
-// stored once per (|k|, m) pair (the matrices depend on k only through
-// (alpha k)^2, timestep.f90:76-77) as [slot][mat][n][13 rows][pair].
+// The LU factors are stored once per (|k|, m) pair (the matrices depend on k
+// only through (alpha k)^2, timestep.f90:76-77) in LAPACK band form,
+// [slot][mat][pair][n][16] (see lu_block).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -349,6 +349,19 @@ __global__ void k_lu_pack(Geo G, Bufs B, int slot0) {
   unsigned char* mt = meta_block(B, slot, mat, p, P);
   mt[0] = (unsigned char)kue;
   mt[1] = (unsigned char)mo;
+}
+
+// ---------------------------------------------------------------------------
+// dst += src on one coll component (harmonics 0..H-1 of every radial row):
+// forcing terms added to vel_N* by var_null(2) hooks.  grid: x = H/64, y = N.
+// ---------------------------------------------------------------------------
+__global__ void k_add_coll(Geo G, double2* dst, const double2* src) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
+  if (h >= G.H || n >= G.N) return;
+  const size_t o = (size_t)n * G.Hp + h;
+  const double2 a = dst[o], b = src[o];
+  dst[o] = make_double2(a.x + b.x, a.y + b.y);
 }
 
 // ---------------------------------------------------------------------------

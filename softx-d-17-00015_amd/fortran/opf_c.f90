@@ -178,6 +178,43 @@ module opf_c
          integer(c_int) :: rc
       end function
 
+      ! var_coll_norm (variables.f90:718-751) of one component of the state
+      function opf_var_norm(ctx, slot, comp, E, Ek, Em) bind(C, name="opf_var_norm") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, comp
+         real(c_double), intent(out) :: E, Ek(*), Em(*)
+         integer(c_int) :: rc
+      end function
+
+      ! io_write_friction (io.f90:627-639)
+      function opf_friction(ctx, slot, Ub, Uc, Ufr) bind(C, name="opf_friction") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: Ub, Uc, Ufr
+         integer(c_int) :: rc
+      end function
+
+      ! vel_r/t/z, vel_curlr/t/z for var_null(2) hooks
+      function opf_get_phys(ctx, slot, which, phys) bind(C, name="opf_get_phys") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, which
+         real(c_double), intent(out) :: phys(*)
+         integer(c_int) :: rc
+      end function
+
+      ! vel_N* += dN (forcing of var_null(2) hooks)
+      function opf_add_nonlinear(ctx, slot, nr_re, nr_im, nt_re, nt_im, nz_re, nz_im) &
+            bind(C, name="opf_add_nonlinear") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(in) :: nr_re(*), nr_im(*), nt_re(*), nt_im(*), nz_re(*), nz_im(*)
+         integer(c_int) :: rc
+      end function
+
       function opf_tra_coll2phys(ctx, c_re, c_im, phys) bind(C, name="opf_tra_coll2phys") result(rc)
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx

@@ -180,6 +180,19 @@ contains
       call opf_check(opf_tim_lumesh_invert(h, slot, int(which, c_int), b%Re, b%Im), 'tim_lumesh_invert')
    end subroutine tim_lumesh_invert
 
+   ! var_coll_norm (variables.f90:718-751) of component comp (0..2)
+   subroutine var_coll_norm(comp, E, Ek, Em)
+      integer, intent(in) :: comp
+      double precision, intent(out) :: E, Ek(0:i_K-1), Em(0:i_M-1)
+      call opf_check(opf_var_norm(h, slot, int(comp, c_int), E, Ek, Em), 'var_coll_norm')
+   end subroutine var_coll_norm
+
+   ! io_write_friction quantities (io.f90:627-639)
+   subroutine vel_friction(Ub, Uc, Ufr)
+      double precision, intent(out) :: Ub, Uc, Ufr
+      call opf_check(opf_friction(h, slot, Ub, Uc, Ufr), 'io_write_friction')
+   end subroutine vel_friction
+
    ! vel_Pr0 and the step counters as the device holds them
    subroutine vel_status(step, ncorr, dterr)
       integer, intent(out) :: step, ncorr

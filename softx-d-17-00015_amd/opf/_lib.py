@@ -48,6 +48,10 @@ PROTOTYPES = {
     "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
     "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
     "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
+    "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
+    "opf_friction": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+    "opf_get_phys": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+    "opf_add_nonlinear": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
     "opf_server_last_error": (C.c_char_p, []),
     "opf_server_create": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, C.c_int]),
     "opf_server_destroy": (C.c_int, [vp]),

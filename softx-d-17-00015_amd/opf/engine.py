@@ -154,6 +154,36 @@ class Context:
         check(self.lib.opf_get_nonlinear(self.h, slot, *[_dptr(p) for p in parts]))
         return (_join(parts[0], parts[1]), _join(parts[2], parts[3]), _join(parts[4], parts[5]))
 
+    def var_norm(self, slot=0, comp=0):
+        """var_coll_norm (variables.f90:718-751) of one component of the
+        current state: (E, Ek[0:K], Em[0:M])."""
+        E = C.c_double()
+        Ek = np.zeros(self.K)
+        Em = np.zeros(self.M)
+        check(self.lib.opf_var_norm(self.h, slot, comp, C.byref(E), _dptr(Ek), _dptr(Em)))
+        return E.value, Ek, Em
+
+    def friction(self, slot=0):
+        """io_write_friction (io.f90:627-639): (Ub, Uc, u_tau)."""
+        v = [C.c_double() for _ in range(3)]
+        check(self.lib.opf_friction(self.h, slot, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def get_phys(self, slot=0, which=0):
+        """vel_r/t/z (0..2), vel_curlr/t/z (3..5) of the current state ->
+        phys[n, th, z]."""
+        Z, T = 3 * self.K, 3 * self.M
+        p = np.zeros((self.N, T, Z))
+        check(self.lib.opf_get_phys(self.h, slot, which, _dptr(p)))
+        return p
+
+    def add_nonlinear(self, slot, nr, nt, nz):
+        """vel_N* += (nr, nt, nz): forcing of var_null(2) hooks."""
+        parts = []
+        for a in (nr, nt, nz):
+            parts.extend(_split(a, self.N, self.H))
+        check(self.lib.opf_add_nonlinear(self.h, slot, *[_dptr(p) for p in parts]))
+
     def coll2phys(self, a):
         """tra_coll2phys of one field -> phys[n, th, z]."""
         re, im = _split(a, self.N, self.H)

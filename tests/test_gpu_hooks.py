@@ -1,0 +1,83 @@
+"""Entry points a Fortran host uses around the step: var_coll_norm and the
+friction quantities (io_write_energy / io_write_friction), the physical
+fields of vel_transform and forcing added to vel_N* for var_null(2) hooks."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(g, const_flux=True):
+    import opf
+    ctx = opf.Context(int(g["N"]), int(g["K"]), int(g["M"]))
+    ctx.setup(0, float(g["Re"]), float(g["alpha"]), float(g["dt"]), const_flux=const_flux)
+    ic = g["ic"]
+    ctx.put_state(0, ic[0], ic[1], ic[2])
+    return ctx
+
+
+@pytest.mark.parametrize("name", ["mini", "c0"])
+def test_norm_and_friction_match_reference(name):
+    g = load_golden(name)
+    ctx = _ctx(g)
+    E, Ek0, Em0 = 0.0, 0.0, 0.0
+    for comp in range(3):
+        e, ek, em = ctx.var_norm(0, comp)
+        assert ek.shape == (int(g["K"]),) and em.shape == (int(g["M"]),)
+        assert abs(e - em.sum()) <= 1e-14 * abs(e)
+        E, Ek0, Em0 = E + e, Ek0 + ek[0], Em0 + em[0]
+    ref = g["series"][0]          # step 0: the reference's io quantities of the initial state
+    assert abs(E - ref[2]) <= 1e-12 * ref[2]
+    assert abs(Ek0 - ref[3]) <= 1e-12 * abs(ref[3])
+    assert abs(Em0 - ref[4]) <= 1e-12 * abs(ref[4])
+    Ub, Uc, Ufr = ctx.friction(0)
+    assert abs(Ub - ref[5]) <= 1e-12 * abs(ref[5])
+    assert abs(Uc - ref[6]) <= 1e-13 * abs(ref[6])
+    assert abs(Ufr - ref[7]) <= 1e-12 * abs(ref[7])
+
+
+def test_friction_without_constant_flux_and_diag_record():
+    g = load_golden("mini")
+    ctx = _ctx(g, const_flux=False)
+    Ub, _, _ = ctx.friction(0)
+    me = ctx.mesh()
+    uz0 = g["ic"][2][:, 0].real
+    ub_ref = 0.5 + 2.0 * float(np.dot(uz0, me["intrdr"]))      # io.f90:630
+    assert abs(Ub - ub_ref) <= 1e-13
+    ctx.run(1, diag_every=1)
+    rec = ctx.diag(0)[0]
+    assert abs(rec[5] - ub_ref) <= 1e-13
+
+
+def test_phys_fields_and_curl():
+    import opf_oracle as O
+    g = load_golden("mini")
+    ctx = _ctx(g)
+    u = ctx.get_state(0)
+    for c in range(3):
+        assert np.array_equal(ctx.get_phys(0, c), ctx.coll2phys(u[c]))
+    oc = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), Re=float(g["Re"]), alpha=float(g["alpha"]),
+               dt=float(g["dt"]))
+    oc.set_state(u[0], u[1], u[2])
+    curl = oc.curl(oc.ur, oc.ut, oc.uz)
+    for c in range(3):
+        p = ctx.get_phys(0, 3 + c)
+        assert rel_err(p, oc.spec2phys(curl[c]).transpose(0, 2, 1)) <= 1e-12, c   # oracle p[n, z, th]
+
+
+def test_add_nonlinear_is_added_to_the_evaluation():
+    g = load_golden("mini")
+    ctx = _ctx(g)
+    ctx.transform(0)
+    ctx.nonlinear(0)
+    n1 = ctx.get_nonlinear(0)
+    rng = np.random.default_rng(3)
+    d = [1e-3 * (rng.standard_normal(n1[0].shape) + 1j * rng.standard_normal(n1[0].shape)) for _ in range(3)]
+    ctx.add_nonlinear(0, *d)
+    n2 = ctx.get_nonlinear(0)
+    for c in range(3):
+        assert np.max(np.abs(n2[c] - (n1[c] + d[c]))) <= 1e-15 * max(1.0, np.abs(n1[c]).max())

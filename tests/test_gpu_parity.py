@@ -112,7 +112,7 @@ def test_transforms_match_oracle(opf, shape):
 
 
 def test_unsupported_transform_length_rejected(opf):
-    """3K and 3M must be 2^a 3^b (radix-4/2/3 FFTs); others fail loudly."""
+    """3K and 3M must be 2^a 3^b; other lengths fail loudly."""
     from opf._lib import OpfError
     with pytest.raises(OpfError):
         opf.Context(16, 6, 5)
