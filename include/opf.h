@@ -194,6 +194,12 @@ int opf_server_stats(opf_server* srv, int* queued, int* running, long long* stat
 int opf_kernel_timing(opf_ctx* ctx, int enable);
 int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
 
+/* Launch shape of the fused step kernel chosen at opf_create: (|k|, m)
+ * pairs per workgroup (1, 2 or 4; environment OPF_STEP_GP overrides the
+ * default), threads per workgroup and dynamic LDS bytes.  No reference
+ * counterpart (introspection for tests and benchmarks). */
+int opf_step_config(const opf_ctx* ctx, int* pairs_per_group, int* threads, long long* lds_bytes);
+
 #ifdef __cplusplus
 }
 #endif

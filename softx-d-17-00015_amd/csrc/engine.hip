@@ -157,7 +157,7 @@ FftTiles choose_tiles(int Z, int T, int M) {
 size_t step_smem(int N, int gp, int bufd2) {
   const int GH = 2 * gp;
   size_t b = (size_t)4 * N * GH * sizeof(double2);           // V[4]
-  b += std::max((size_t)2 * bufd2 * sizeof(double2), (size_t)3 * N * GH * sizeof(double));
+  b += std::max(((size_t)2 * bufd2 + FC_TAIL) * sizeof(double2), (size_t)3 * N * GH * sizeof(double));
   b += (size_t)GH * 8 * sizeof(double) + 8 * sizeof(double);  // AC, RED
   b += 4 * sizeof(int);                                       // META
   return round_up(b, 16);
@@ -166,14 +166,19 @@ size_t step_smem(int N, int gp, int bufd2) {
 // Pairs per workgroup of the fused step kernel and its factor ring.  A
 // chunk block must hold one window of the widest U sweep (9 columns of 5
 // double2) plus its pad; 48 double2 per Helmholtz block gives 15-column
-// chunks for the common band widths at 26 KiB of LDS (N = 64, two pairs),
-// six workgroups per CU.
-StepTiles choose_step_tiles(int N, int P, int gp_req) {
+// chunks for the common band widths.  Four pairs per workgroup (N <= 64)
+// put 48 of the solver wave's 64 lanes to work in the Helmholtz solves
+// instead of 24 and halve the solve instructions per harmonic; they are the
+// default when the slots give at least three such workgroups per CU (51 KiB
+// of LDS each at N = 64), otherwise two pairs (26 KiB, six per CU).
+StepTiles choose_step_tiles(int N, int P, int cap, int ncu, int gp_req) {
   StepTiles t{};
-  int gp = gp_req > 0 ? gp_req : 2;
-  if (gp != 1 && gp != 2) throw OpfError(OPF_ERR_ARG, "step pairs per workgroup must be 1 or 2");
+  int gp = gp_req;
+  if (gp <= 0) gp = (N <= 64 && (long long)((P + 3) / 4) * cap >= 3LL * ncu) ? 4 : 2;
+  if (gp != 1 && gp != 2 && gp != 4) throw OpfError(OPF_ERR_ARG, "step pairs per workgroup must be 1, 2 or 4");
+  if (gp == 4 && N > 64) throw OpfError(OPF_ERR_ARG, "four pairs per workgroup need N <= 64");
   t.gp = gp;
-  t.bufd2 = 3 * gp * 49;
+  t.bufd2 = step_bufd2(gp);
   t.ngroups = (P + gp - 1) / gp;
   t.smem = step_smem(N, gp, t.bufd2);
   if (t.smem > 64 * 1024) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit in LDS");
@@ -228,8 +233,8 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
 
 void launch_step(opf_ctx* c, int s0, int ns) {
   const int per = (c->st.ngroups + 7) / 8;   // XCD-aware order inside the kernel
-  auto k = c->st.gp == 1 ? k_step_g1 : (c->N * 4 <= 2 * STEP_THREADS ? k_step_g2 : k_step_g2x);
-  launch(c, "step", k, dim3(8 * per, 1, ns), dim3(STEP_THREADS), c->st.smem, c->G, c->B, c->st, s0);
+  auto k = c->st.gp == 4 ? k_step_g4 : c->st.gp == 1 ? k_step_g1 : (c->N <= 64 ? k_step_g2 : k_step_g2x);
+  launch(c, "step", k, dim3(8 * per, 1, ns), dim3(step_threads(c->st.gp)), c->st.smem, c->G, c->B, c->st, s0);
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }
 
@@ -290,7 +295,10 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
   {
     const char* e1 = std::getenv("OPF_STEP_GP");
-    c->st = choose_step_tiles(N, c->P, e1 ? std::atoi(e1) : 0);
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    c->st = choose_step_tiles(N, c->P, cap, ncu, e1 ? std::atoi(e1) : 0);
   }
   c->smem_z = ((size_t)c->tiles.tm * padded_len(c->Z) + c->Z) * sizeof(double2);       // data + twiddles
   c->smem_t = ((size_t)3 * c->tiles.tz * padded_len(c->T) + c->T) * sizeof(double2);
@@ -981,6 +989,15 @@ extern "C" int opf_kernel_timing(opf_ctx* c, int enable) {
     for (auto& r : c->tm.recs) { c->tm.pool.push_back(r.a); c->tm.pool.push_back(r.b); }
     c->tm.recs.clear();
     c->tm.on = enable != 0;
+  });
+}
+
+extern "C" int opf_step_config(const opf_ctx* c, int* gp, int* threads, long long* lds) {
+  return guard([&] {
+    if (!c || !gp || !threads || !lds) throw OpfError(OPF_ERR_ARG, "null argument");
+    *gp = c->st.gp;
+    *threads = step_threads(c->st.gp);
+    *lds = (long long)c->st.smem;
   });
 }
 

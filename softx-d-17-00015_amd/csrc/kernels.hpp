@@ -66,13 +66,17 @@ __host__ __device__ inline unsigned char* meta_block(const Bufs& B, int slot, in
 
 // Shape of the fused step kernel.
 struct StepTiles {
-  int gp;        // (|k|, m) pairs per workgroup: 1 or 2
+  int gp;        // (|k|, m) pairs per workgroup: 1, 2 or 4
   int bufd2;     // double2 per factor chunk buffer (two buffers)
   int ngroups;   // workgroups per slot
   size_t smem;   // dynamic LDS bytes
 };
-constexpr int STEP_THREADS = 128;  // wave 0 solves, wave 1 streams factor chunks
+// threads of a step workgroup: wave 0 solves, the other waves stream factor chunks
+inline int step_threads(int gp) { return gp == 4 ? 256 : 128; }
+constexpr int step_bufd2(int gp) { return 3 * gp * 49; }   // double2 per factor chunk buffer
 constexpr int STEP_BLKCAP = 64;    // double2 per (system, pair) block of a chunk (one per loader lane)
+constexpr int FC_TAIL = 16;        // double2 after the factor ring: the solvers' prefetch reads up to two
+                                   // columns (at most 10 double2) past a chunk
 
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_TILE = 2048;   // complex values per transform workgroup (all its lines)
@@ -91,6 +95,7 @@ __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
 __global__ void k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 128
+__global__ void k_step_g4(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64, 256 threads
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
 // transforms: kernels specialised to the radix plans of (Z, T), chosen once

@@ -4,18 +4,19 @@
 // (timestep.f90:190-207) and the curl of the new velocity for the next
 // nonlinear evaluation (var_coll_curl, variables.f90:408-436).
 //
-// One workgroup of two waves owns a group of GP consecutive (|k|, m) pairs,
-// i.e. the harmonics +k and -k of each pair (2*GP "harmonic slots").
+// One workgroup owns a group of GP consecutive (|k|, m) pairs, i.e. the
+// harmonics +k and -k of each pair (2*GP "harmonic slots"): two waves for
+// GP = 1, 2 and four for GP = 4.
 // Everything a harmonic needs between its input and output stays in LDS:
 //   V[4][N][2GP]  the three right-hand sides (u+, u-, uz) and the pressure.
 // Both waves map threads to (radial point, harmonic) for the stages that are
 // parallel in r (right-hand side, divergence, gradient, influence correction,
 // curl).  The banded LU solves are serial in r: wave 0 runs them, one lane
-// per (system, harmonic, real or imaginary part), while wave 1 streams the
-// factor columns of the group into a two-buffer LDS ring, holding the next
-// two chunks in registers so that a chunk's global loads have two chunk
-// times to arrive.  Factors are stored once per pair and shared by the +k /
-// -k harmonics and their real and imaginary parts.
+// per (system, harmonic, real or imaginary part), while the other waves
+// stream the factor columns of the group into a two-buffer LDS ring with direct
+// global-to-LDS copies, one chunk ahead of the solver.  Factors are stored
+// once per pair and shared by the +k / -k harmonics and their real and
+// imaginary parts.
 //
 // The solve loops are unrolled over the length of their register window (5
 // rows for the L sweep, kue+1 for the U sweep, kue being the widest
@@ -23,7 +24,12 @@
 // window rotates by renaming instead of moving registers.
 //
 // Arithmetic follows the reference operation by operation (see the cited
-// routines); products and sums are rounded separately (no contraction).
+// routines) with products and sums rounded separately (no contraction),
+// except in the banded solves: there each update b - l*t is one fused
+// multiply-add and the division by the pivot is a product with its
+// reciprocal (correctly rounded), which halves the solves' FP64
+// instructions; a solve then differs from the separately rounded one by a
+// few ulps per row (relative 1e-15 on the state).
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -33,8 +39,6 @@
 namespace opf {
 
 namespace {
-
-constexpr int HT = STEP_THREADS;
 
 __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alpha, int k, int PM) {
   // tim_mesh_init / tim_lumesh_init diagonal term (timestep.f90:54-55, 76-77)
@@ -163,19 +167,22 @@ __device__ __forceinline__ Seg seg_of(const Stream& st, int g, int& c) {
   return S;
 }
 
-// Loader wave: chunk g of the stream is copied from global memory straight
-// into ring buffer g & 1 (global_load_lds_dwordx4, no registers): lane
-// `lane` moves element `lane` of every (system, pair) block.  The copy is
-// issued while the solver works on chunk g-1 and is complete at the
-// workgroup barrier that ends that chunk.
+// Loader waves (every wave but the first): chunk g of the stream is copied
+// from global memory straight into ring buffer g & 1
+// (global_load_lds_dwordx4, no registers); loader wave w takes the
+// (system, pair) blocks b with b % NLW == w and its lane l moves element l
+// of each.  The copy is issued while the solver works on chunk g-1 and is
+// complete at the workgroup barrier that ends that chunk.
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-template <int GP>
+template <int GP, int NLW>
 __device__ __forceinline__ void ld_dma(const Lds& L, const Bufs& B, const Stream& st, int g, int slot, int p0,
                                        int np, int P, int N, int lane) {
   if (g >= st.total) return;
   int c;
   const Seg S = seg_of(st, g, c);
+  const int lw = lane >> 6;   // loader wave: blocks lw, lw + NLW, ...
+  lane &= 63;
   const int per = seg_ncol(S, c, N) * S.qn;
   if (lane >= per) return;
   const int col = lane / S.qn, q = lane - col * S.qn;
@@ -185,7 +192,7 @@ __device__ __forceinline__ void ld_dma(const Lds& L, const Bufs& B, const Stream
   for (int sys = 0; sys < 3; ++sys)
 #pragma unroll
     for (int pl = 0; pl < GP; ++pl) {
-      if (sys < S.nsys && pl < np) {
+      if ((sys * GP + pl) % NLW == lw && sys < S.nsys && pl < np) {
         const int mat = S.m0 + sys;
         const double2* src = S.fwd ? fw_block(B, slot, mat, p0 + pl, P, N) + (size_t)c * S.cs * FW_Q + lane
                                    : bw_block(B, slot, mat, p0 + pl, P, N) + (size_t)q * N + (N - 1 - c * S.cs - col);
@@ -198,48 +205,63 @@ __device__ __forceinline__ void ld_dma(const Lds& L, const Bufs& B, const Stream
 // DGBTRS 'N' (KL = KU = 4) for one solver lane, split into chunks.
 // L sweep with the row interchanges: w[i] holds b(j+i), the rows that the
 // multipliers of column j touch.  F: the lane's block of the chunk, column
-// cl at F[3*cl]: (pivot offset, l1) (l2, l3) (l4, 0).  Multiplier slots below
-// the last row hold exact zeros and the window holds zeros there, so the
-// updates need no guard.  MO: largest pivot offset in the block's matrices.
+// cl at F[3*cl]: (pivot offset, l1) (l2, l3) (l4, 0).  MO: largest pivot
+// offset in the block's matrices.
+//
+// The whole-window loop keeps the factors of column cl + 2 and the row that
+// enters the window two columns later in flight while column cl is
+// eliminated (register rings of five named slots, so the prefetch carries
+// across windows without moves).  The last two prefetches of a chunk read
+// up to two columns past it (the next block, or the ring's tail pad of
+// FC_TAIL double2) and are discarded.  Rows at or beyond N enter the window as a
+// copy of row N-1: such a row is never a pivot (pivot rows lie inside the
+// matrix), its multipliers are exact zeros and it is never stored, so its
+// value cannot reach a result.
 // ---------------------------------------------------------------------------
 template <int MO>
-__device__ __forceinline__ void fwd_chunk(const double2* F, double* x, int xs, int j0, int ncol, int N,
+__device__ __forceinline__ void fwd_chunk(const double2* __restrict__ F, double* __restrict__ x, int xs, int j0, int ncol, int N,
                                           double (&w)[5]) {
   int cl = 0;
-  for (; cl + 5 <= ncol; cl += 5) {
-    const int j = j0 + cl;
-    // rows entering the window in this block, then the factors one column
-    // ahead of their use
-    double2 f[2][3];
+  if (ncol >= 5) {
+    double2 f[5][3];
+    double xi[5];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) f[0][q] = F[cl * 3 + q];
+    for (int p = 0; p < 2; ++p) {
 #pragma unroll
-    for (int s = 0; s < 5; ++s) {
-      if (s + 1 < 5) {
+      for (int q = 0; q < 3; ++q) f[p][q] = F[p * 3 + q];
+      const int jn = j0 + p + 5;
+      xi[p] = x[(jn < N ? jn : N - 1) * xs];
+    }
+    for (; cl + 5 <= ncol; cl += 5) {
+      const int j = j0 + cl;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) f[(s + 1) & 1][q] = F[(cl + s + 1) * 3 + q];
+      for (int s = 0; s < 5; ++s) {
+        {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) f[(s + 2) % 5][q] = F[(cl + s + 2) * 3 + q];
+          const int jn = j + s + 7;
+          xi[(s + 2) % 5] = x[(jn < N ? jn : N - 1) * xs];
+        }
+        const double2 f0 = f[s][0], f1 = f[s][1], f2 = f[s][2];
+        if (MO > 0) {
+          const double offd = f0.x;
+          const double a0 = w[s % 5];
+          double sv = a0;
+#pragma unroll
+          for (int i = 1; i <= MO; ++i) sv = offd == (double)i ? w[(s + i) % 5] : sv;
+#pragma unroll
+          for (int i = 1; i <= MO; ++i) w[(s + i) % 5] = offd == (double)i ? a0 : w[(s + i) % 5];
+          w[s % 5] = sv;
+        }
+        const double t = w[s % 5];
+        const double nt = -t;
+        x[(j + s) * xs] = t;
+        w[(s + 1) % 5] = fma(f0.y, nt, w[(s + 1) % 5]);
+        w[(s + 2) % 5] = fma(f1.x, nt, w[(s + 2) % 5]);
+        w[(s + 3) % 5] = fma(f1.y, nt, w[(s + 3) % 5]);
+        w[(s + 4) % 5] = fma(f2.x, nt, w[(s + 4) % 5]);
+        w[s % 5] = xi[s];
       }
-      const double2 f0 = f[s & 1][0], f1 = f[s & 1][1], f2 = f[s & 1][2];
-      const int jn = j + s + 5;
-      const double xv = x[(jn < N ? jn : N - 1) * xs];
-      if (MO > 0) {
-        const double offd = f0.x;
-        const double a0 = w[s % 5];
-        double sv = a0;
-#pragma unroll
-        for (int i = 1; i <= MO; ++i) sv = offd == (double)i ? w[(s + i) % 5] : sv;
-#pragma unroll
-        for (int i = 1; i <= MO; ++i) w[(s + i) % 5] = offd == (double)i ? a0 : w[(s + i) % 5];
-        w[s % 5] = sv;
-      }
-      const double t = w[s % 5];
-      const double nt = -t;
-      x[(j + s) * xs] = t;
-      w[(s + 1) % 5] = w[(s + 1) % 5] + f0.y * nt;
-      w[(s + 2) % 5] = w[(s + 2) % 5] + f1.x * nt;
-      w[(s + 3) % 5] = w[(s + 3) % 5] + f1.y * nt;
-      w[(s + 4) % 5] = w[(s + 4) % 5] + f2.x * nt;
-      w[s % 5] = jn < N ? xv : 0.0;
     }
   }
   for (; cl < ncol; ++cl) {   // remainder of the last chunk: explicit shifts
@@ -258,10 +280,10 @@ __device__ __forceinline__ void fwd_chunk(const double2* F, double* x, int xs, i
     const double t = w[0];
     const double nt = -t;
     x[j * xs] = t;
-    w[1] = w[1] + f0.y * nt;
-    w[2] = w[2] + f1.x * nt;
-    w[3] = w[3] + f1.y * nt;
-    w[4] = w[4] + f2.x * nt;
+    w[1] = fma(f0.y, nt, w[1]);
+    w[2] = fma(f1.x, nt, w[2]);
+    w[3] = fma(f1.y, nt, w[3]);
+    w[4] = fma(f2.x, nt, w[4]);
     w[0] = w[1];
     w[1] = w[2];
     w[2] = w[3];
@@ -273,43 +295,49 @@ __device__ __forceinline__ void fwd_chunk(const double2* F, double* x, int xs, i
 // U sweep (DTBSV upper, no transpose) for rows jtop .. jtop-ncol+1: v[d]
 // holds the pending row jtop-d for d = 0..KUE.  F: column cl (row jtop-cl)
 // at F[QN*cl]: (u0, 1/u0) (u1, u2) ...  Superdiagonals beyond KUE are exact
-// zeros and are skipped; the division by u0 is the correctly rounded
-// reciprocal form (see k_lu_pack).
-__device__ __forceinline__ double div_r(double b, double d, double r) {
-  const double q = b * r;
-  const double e = fma(-d, q, b);
-  return fma(e, r, q);
-}
-
+// zeros and are skipped.
 template <int KUE>
-__device__ __forceinline__ void bwd_chunk(const double2* F, double* x, int xs, int jtop, int ncol,
+__device__ __forceinline__ void bwd_chunk(const double2* __restrict__ F, double* __restrict__ x, int xs, int jtop, int ncol,
                                           double (&v)[9]) {
   constexpr int W = KUE + 1;
   constexpr int QN = 1 + (KUE + 1) / 2;
   // windows wider than 7 rows run the shifting loop only (register budget)
   constexpr bool ROT = KUE <= 6;
   int cl = 0;
-  for (; ROT && cl + W <= ncol; cl += W) {
-    const int j = jtop - cl;
-    double2 u[2][QN];
+  if (ROT && ncol >= W) {
+    // register rings of W named slots: the factors of row jtop-cl-2 and the
+    // row entering the window two rows later are in flight while row
+    // jtop-cl is solved (over-reads past the chunk as in fwd_chunk).  Rows below 0 enter as copies of row 0: they are
+    // only ever updated, never solved or stored.
+    double2 u[W][QN];
+    double xi[W];
 #pragma unroll
-    for (int q = 0; q < QN; ++q) u[0][q] = F[cl * QN + q];
+    for (int p = 0; p < 2; ++p) {
 #pragma unroll
-    for (int s = 0; s < W; ++s) {
-      if (s + 1 < W) {
+      for (int q = 0; q < QN; ++q) u[p][q] = F[p * QN + q];
+      const int jn = jtop - p - W;
+      xi[p] = x[(jn >= 0 ? jn : 0) * xs];
+    }
+    for (; cl + W <= ncol; cl += W) {
+      const int j = jtop - cl;
 #pragma unroll
-        for (int q = 0; q < QN; ++q) u[(s + 1) & 1][q] = F[(cl + s + 1) * QN + q];
+      for (int s = 0; s < W; ++s) {
+        {
+#pragma unroll
+          for (int q = 0; q < QN; ++q) u[(s + 2) % W][q] = F[(cl + s + 2) * QN + q];
+          const int jn = j - s - 2 - W;
+          xi[(s + 2) % W] = x[(jn >= 0 ? jn : 0) * xs];
+        }
+        const double t = v[s % W] * u[s][0].y;
+        x[(j - s) * xs] = t;
+        const double nt = -t;
+#pragma unroll
+        for (int d = 1; d <= KUE; ++d) {
+          const double ud = (d & 1) ? u[s][(d + 1) / 2].x : u[s][d / 2].y;
+          v[(s + d) % W] = fma(nt, ud, v[(s + d) % W]);
+        }
+        v[s % W] = xi[s];
       }
-      const int jn = j - s - W;
-      const double xv = x[(jn >= 0 ? jn : 0) * xs];
-      const double t = div_r(v[s % W], u[s & 1][0].x, u[s & 1][0].y);
-      x[(j - s) * xs] = t;
-#pragma unroll
-      for (int d = 1; d <= KUE; ++d) {
-        const double ud = (d & 1) ? u[s & 1][(d + 1) / 2].x : u[s & 1][d / 2].y;
-        v[(s + d) % W] = v[(s + d) % W] - t * ud;
-      }
-      v[s % W] = jn >= 0 ? xv : 0.0;
     }
   }
   for (; cl < ncol; ++cl) {   // remainder of the last chunk: explicit shifts
@@ -317,12 +345,13 @@ __device__ __forceinline__ void bwd_chunk(const double2* F, double* x, int xs, i
     double2 u[QN];
 #pragma unroll
     for (int q = 0; q < QN; ++q) u[q] = F[cl * QN + q];
-    const double t = div_r(v[0], u[0].x, u[0].y);
+    const double t = v[0] * u[0].y;
     x[j * xs] = t;
+    const double nt = -t;
 #pragma unroll
     for (int d = 1; d <= KUE; ++d) {
       const double ud = (d & 1) ? u[(d + 1) / 2].x : u[d / 2].y;
-      v[d] = v[d] - t * ud;
+      v[d] = fma(nt, ud, v[d]);
     }
 #pragma unroll
     for (int d = 0; d < KUE; ++d) v[d] = v[d + 1];
@@ -330,13 +359,137 @@ __device__ __forceinline__ void bwd_chunk(const double2* F, double* x, int xs, i
   }
 }
 
+// Straight-line forms of the two sweeps for a whole chunk of CS columns
+// (rows), CS a multiple of the window length: with no loop back-edge and no
+// clamped row indices the compiler is free to issue every LDS read as early
+// as the registers allow.  Callers use them only where every row entering
+// the window lies inside the matrix (fwd: j0 + CS + 4 <= N - 1; bwd:
+// jtop - CS + 1 - (KUE + 1) >= 0); the chunk loops above cover the rest.
+template <int MO, int CS>
+__device__ __forceinline__ void fwd_chunk_s(const double2* __restrict__ F, double* __restrict__ x, int xs, int j0,
+                                            double (&w)[5]) {
+  static_assert(CS % 5 == 0, "whole windows only");
+  constexpr int D = 3;   // steps of lookahead for the LDS reads
+  double* xb = x + (size_t)j0 * xs;
+  double2 f[CS][3];
+  double xi[CS];
+#pragma unroll
+  for (int p = 0; p < D && p < CS; ++p) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) f[p][q] = F[p * 3 + q];
+    xi[p] = xb[(p + 5) * xs];
+  }
+#pragma unroll
+  for (int s = 0; s < CS; ++s) {
+    if (s + D < CS) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) f[s + D][q] = F[(s + D) * 3 + q];
+      xi[s + D] = xb[(s + D + 5) * xs];
+    }
+    // keep the reads issued here, ahead of the arithmetic that consumes
+    // older ones
+    __builtin_amdgcn_sched_barrier(0);
+    const double2 f0 = f[s][0], f1 = f[s][1], f2 = f[s][2];
+    if (MO > 0) {
+      const double offd = f0.x;
+      const double a0 = w[s % 5];
+      double sv = a0;
+#pragma unroll
+      for (int i = 1; i <= MO; ++i) sv = offd == (double)i ? w[(s + i) % 5] : sv;
+#pragma unroll
+      for (int i = 1; i <= MO; ++i) w[(s + i) % 5] = offd == (double)i ? a0 : w[(s + i) % 5];
+      w[s % 5] = sv;
+    }
+    const double t = w[s % 5];
+    const double nt = -t;
+    xb[s * xs] = t;
+    w[(s + 1) % 5] = fma(f0.y, nt, w[(s + 1) % 5]);
+    w[(s + 2) % 5] = fma(f1.x, nt, w[(s + 2) % 5]);
+    w[(s + 3) % 5] = fma(f1.y, nt, w[(s + 3) % 5]);
+    w[(s + 4) % 5] = fma(f2.x, nt, w[(s + 4) % 5]);
+    w[s % 5] = xi[s];
+  }
+}
+
+template <int KUE, int CS>
+__device__ __forceinline__ void bwd_chunk_s(const double2* __restrict__ F, double* __restrict__ x, int xs, int jtop,
+                                            double (&v)[9]) {
+  constexpr int W = KUE + 1;
+  constexpr int QN = 1 + (KUE + 1) / 2;
+  static_assert(CS % W == 0, "whole windows only");
+  constexpr int D = 3;
+  double* xb = x + (size_t)jtop * xs;
+  double2 u[CS][QN];
+  double xi[CS];
+#pragma unroll
+  for (int p = 0; p < D && p < CS; ++p) {
+#pragma unroll
+    for (int q = 0; q < QN; ++q) u[p][q] = F[p * QN + q];
+    xi[p] = xb[-(p + W) * xs];
+  }
+#pragma unroll
+  for (int s = 0; s < CS; ++s) {
+    if (s + D < CS) {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) u[s + D][q] = F[(s + D) * QN + q];
+      xi[s + D] = xb[-(s + D + W) * xs];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const double t = v[s % W] * u[s][0].y;
+    xb[-s * xs] = t;
+    const double nt = -t;
+#pragma unroll
+    for (int d = 1; d <= KUE; ++d) {
+      const double ud = (d & 1) ? u[s][(d + 1) / 2].x : u[s][d / 2].y;
+      v[(s + d) % W] = fma(nt, ud, v[(s + d) % W]);
+    }
+    v[s % W] = xi[s];
+  }
+}
+
+// chunk lengths of a segment (make_seg) as compile-time values
+constexpr int seg_cap(int nsys, int gp) {
+  return step_bufd2(gp) / (nsys * gp) - 1 < STEP_BLKCAP ? step_bufd2(gp) / (nsys * gp) - 1 : STEP_BLKCAP;
+}
+constexpr int seg_cs_fwd(int cap) { return (cap / FW_Q) / 5 * 5; }
+constexpr int seg_cs_bwd(int cap, int kue) { return (cap / (1 + (kue + 1) / 2)) / (kue + 1) * (kue + 1); }
+
+// a chunk of either sweep: the straight-line form where it applies
+template <int MO, int CAP>
+__device__ __forceinline__ void fwd_any(const double2* F, double* x, int xs, int j0, int ncol, int cs, int N,
+                                        double (&w)[5]) {
+  constexpr int CS = seg_cs_fwd(CAP);
+  if constexpr (MO <= 2 && CS >= 5) {
+    if (ncol == CS && cs == CS && j0 + CS + 4 <= N - 1) {
+      fwd_chunk_s<MO, CS>(F, x, xs, j0, w);
+      return;
+    }
+  }
+  fwd_chunk<MO>(F, x, xs, j0, ncol, N, w);
+}
+
+template <int KUE, int CAP>
+__device__ __forceinline__ void bwd_any(const double2* F, double* x, int xs, int jtop, int ncol, int cs,
+                                        double (&v)[9]) {
+  constexpr int CS = seg_cs_bwd(CAP, KUE);
+  if constexpr (KUE >= 3 && KUE <= 6 && CS >= KUE + 1) {
+    if (ncol == CS && cs == CS && jtop - CS + 1 - (KUE + 1) >= 0) {
+      bwd_chunk_s<KUE, CS>(F, x, xs, jtop, v);
+      return;
+    }
+  }
+  bwd_chunk<KUE>(F, x, xs, jtop, ncol, v);
+}
+
 // One solve phase (pressure: LNp on component 3; Helmholtz: LDp, LDm, LDz on
 // components 0, 1, 2): L sweep then U sweep, chunk by chunk, with wave 1
 // streaming the factor ring.  gbase: index of the phase's first chunk.
-template <int GP>
+template <int GP, int NLW, bool PRESSURE>
 __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const Stream& st, const Seg& SF,
-                                            const Seg& SB, int kue, int mo, bool pressure, int gbase, int slot,
-                                            int p0, int np, int P, int K, int N) {
+                                            const Seg& SB, int kue, int mo, int gbase, int slot, int p0, int np,
+                                            int P, int K, int N) {
+  constexpr bool pressure = PRESSURE;
+  constexpr int CAP = seg_cap(PRESSURE ? 1 : 3, GP);
   const int tid = threadIdx.x;
   const int nsolv = 2 * L.GH * SF.nsys;
   const bool solver = tid < nsolv;
@@ -362,16 +515,16 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
   const int BSF = SF.cs * SF.qn + 1;
   for (int c = 0; c < SF.nch; ++c) {
     const int g = gbase + c;
-    if (loader) ld_dma<GP>(L, B, st, g + 1, slot, p0, np, P, N, lane);
+    if (loader) ld_dma<GP, NLW>(L, B, st, g + 1, slot, p0, np, P, N, lane);
     if (act) {
       const double2* F = L.FC + (size_t)(g & 1) * L.bufd2 + (size_t)blk * BSF;
       const int j0 = c * SF.cs, ncol = seg_ncol(SF, c, N);
       switch (mo) {
-        case 0: fwd_chunk<0>(F, x, xs, j0, ncol, N, w); break;
-        case 1: fwd_chunk<1>(F, x, xs, j0, ncol, N, w); break;
-        case 2: fwd_chunk<2>(F, x, xs, j0, ncol, N, w); break;
-        case 3: fwd_chunk<3>(F, x, xs, j0, ncol, N, w); break;
-        default: fwd_chunk<4>(F, x, xs, j0, ncol, N, w); break;
+        case 0: fwd_any<0, CAP>(F, x, xs, j0, ncol, SF.cs, N, w); break;
+        case 1: fwd_any<1, CAP>(F, x, xs, j0, ncol, SF.cs, N, w); break;
+        case 2: fwd_any<2, CAP>(F, x, xs, j0, ncol, SF.cs, N, w); break;
+        case 3: fwd_any<3, CAP>(F, x, xs, j0, ncol, SF.cs, N, w); break;
+        default: fwd_any<4, CAP>(F, x, xs, j0, ncol, SF.cs, N, w); break;
       }
     }
     __syncthreads();
@@ -391,19 +544,19 @@ __device__ __forceinline__ void solve_phase(const Lds& L, const Bufs& B, const S
   const int BSB = SB.cs * SB.qn + 1;
   for (int c = 0; c < SB.nch; ++c) {
     const int g = gbase + SF.nch + c;
-    if (loader) ld_dma<GP>(L, B, st, g + 1, slot, p0, np, P, N, lane);
+    if (loader) ld_dma<GP, NLW>(L, B, st, g + 1, slot, p0, np, P, N, lane);
     if (act) {
       const double2* F = L.FC + (size_t)(g & 1) * L.bufd2 + (size_t)blk * BSB;
       const int jtop = N - 1 - c * SB.cs, ncol = seg_ncol(SB, c, N);
       switch (kue) {
-        case 1: bwd_chunk<1>(F, x, xs, jtop, ncol, v); break;
-        case 2: bwd_chunk<2>(F, x, xs, jtop, ncol, v); break;
-        case 3: bwd_chunk<3>(F, x, xs, jtop, ncol, v); break;
-        case 4: bwd_chunk<4>(F, x, xs, jtop, ncol, v); break;
-        case 5: bwd_chunk<5>(F, x, xs, jtop, ncol, v); break;
-        case 6: bwd_chunk<6>(F, x, xs, jtop, ncol, v); break;
-        case 7: bwd_chunk<7>(F, x, xs, jtop, ncol, v); break;
-        default: bwd_chunk<8>(F, x, xs, jtop, ncol, v); break;
+        case 1: bwd_any<1, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 2: bwd_any<2, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 3: bwd_any<3, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 4: bwd_any<4, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 5: bwd_any<5, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 6: bwd_any<6, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        case 7: bwd_any<7, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
+        default: bwd_any<8, CAP>(F, x, xs, jtop, ncol, SB.cs, v); break;
       }
     }
     __syncthreads();
@@ -419,8 +572,9 @@ __device__ __forceinline__ double pr0_of(const double2* uz0, size_t hs, int N, c
 }
 
 // IT: (radial point, harmonic) items per thread in the parallel stages,
-// N * 2 * GP <= IT * STEP_THREADS.
-template <int GP, int IT>
+// N * 2 * GP <= IT * THR; THR: threads of the workgroup (wave 0 solves,
+// the others stream factors during the solves).
+template <int GP, int IT, int THR>
 __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const StepTiles& ST, int slot0) {
   const int slot = slot0 + blockIdx.z;
   SlotCtl* ctl = &B.ctl[slot];
@@ -461,7 +615,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   base += (size_t)4 * N * L.GH * sizeof(double2);
   L.FC = (double2*)base;
   L.E = (double*)base;
-  const size_t fc_bytes = (size_t)2 * L.bufd2 * sizeof(double2);
+  const size_t fc_bytes = ((size_t)2 * L.bufd2 + FC_TAIL) * sizeof(double2);
   const size_t e_bytes = (size_t)3 * N * L.GH * sizeof(double);
   base += (fc_bytes > e_bytes ? fc_bytes : e_bytes);
   L.AC = (double*)base;
@@ -499,7 +653,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   const double2* npl = corr ? Np : Nc;
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int it = tid + q * HT;
+    const int it = tid + q * THR;
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     const bool ok = it < NI && S.valid;
@@ -518,7 +672,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   }
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int it = tid + q * HT;
+    const int it = tid + q * THR;
     if (it >= NI) continue;
     const int n = it / L.GH, lh = it - n * L.GH;
 #pragma unroll
@@ -541,7 +695,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   double2 res[IT][3];
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int it = tid + q * HT;
+    const int it = tid + q * THR;
     res[q][0] = res[q][1] = res[q][2] = make_double2(0.0, 0.0);
     if (it >= NI) continue;
     const int n = it / L.GH, lh = it - n * L.GH;
@@ -634,7 +788,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int it = tid + q * HT;
+    const int it = tid + q * THR;
     if (it >= NI) continue;
     const int n = it / L.GH, lh = it - n * L.GH;
     Vat(L, 0, n, lh, N) = res[q][0];
@@ -657,8 +811,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   // ---- stage 2: pressure RHS = div(pm2rt(R+, R-), Rz) (vel_adjPPE(1)) ----
   // the first factor chunk streams in meanwhile (the energy terms that
   // shared its memory have been summed)
-  if (loader) ld_dma<GP>(L, B, st, 0, slot, p0, np, P, N, lane);
-  for (int it = tid; it < NI; it += HT) {
+  if (loader) ld_dma<GP, THR / 64 - 1>(L, B, st, 0, slot, p0, np, P, N, lane);
+  for (int it = tid; it < NI; it += THR) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     double2 dv = make_double2(0.0, 0.0);
@@ -687,10 +841,10 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
 
   // ---- stage 3: pressure solve with LNp (tim_lumesh_invert(1,...)) -------
-  solve_phase<GP>(L, B, st, st.s[0], st.s[1], kueP, moP, true, 0, slot, p0, np, P, K, N);
+  solve_phase<GP, THR / 64 - 1, true>(L, B, st, st.s[0], st.s[1], kueP, moP, 0, slot, p0, np, P, K, N);
 
   // ---- stage 4: subtract grad p (rt2pm), zero the BC rows ----------------
-  for (int it = tid; it < NI; it += HT) {
+  for (int it = tid; it < NI; it += THR) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     if (!S.valid) continue;
@@ -723,8 +877,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
 
   // ---- stage 5: Helmholtz solves LDp, LDm, LDz ---------------------------
-  solve_phase<GP>(L, B, st, st.s[2], st.s[3], kueH, moH, false, st.s[0].nch + st.s[1].nch, slot, p0, np, P, K,
-                  N);
+  solve_phase<GP, THR / 64 - 1, false>(L, B, st, st.s[2], st.s[3], kueH, moH, st.s[0].nch + st.s[1].nch, slot, p0,
+                                       np, P, K, N);
 
   // ---- stage 6: influence-matrix correction (vel_adjPPE(2)) --------------
   if (tid < L.GH) {
@@ -758,7 +912,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     }
   }
   __syncthreads();
-  for (int it = tid; it < NI; it += HT) {
+  for (int it = tid; it < NI; it += THR) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     if (!S.valid) continue;
@@ -799,7 +953,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   }
   __syncthreads();
   double dmax = 0.0;
-  for (int it = tid; it < NI; it += HT) {
+  for (int it = tid; it < NI; it += THR) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     if (!S.valid) continue;
@@ -828,7 +982,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
 
   // ---- stage 7: curl of the new velocity for the next evaluation ---------
-  for (int it = tid; it < NI; it += HT) {
+  for (int it = tid; it < NI; it += THR) {
     const int n = it / L.GH, lh = it - n * L.GH;
     const Slot S = slot_of(lh, GP, np, p0, K);
     if (!S.valid) continue;
@@ -863,17 +1017,21 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 
 }  // namespace
 
-__global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0) {
-  step_body<1, 2>(G, B, ST, slot0);
+  step_body<1, 2, 128>(G, B, ST, slot0);
 }
-__global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0) {
-  step_body<2, 2>(G, B, ST, slot0);
+  step_body<2, 2, 128>(G, B, ST, slot0);
 }
-__global__ void __launch_bounds__(STEP_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0) {
-  step_body<2, 4>(G, B, ST, slot0);
+  step_body<2, 4, 128>(G, B, ST, slot0);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_g4(Geo G, Bufs B, StepTiles ST, int slot0) {
+  step_body<4, 2, 256>(G, B, ST, slot0);
 }
 
 }  // namespace opf

@@ -318,9 +318,8 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
 // Solve-ready copy of the factors (see fw_block / bw_block) and, per
 // (pair, matrix), the widest superdiagonal of U holding a non-zero and the
 // largest pivot offset.  The fused step kernel skips the superdiagonals that
-// are exactly zero (x - t*0 = x) and replaces the division by the diagonal
-// with q = b*r, e = fma(-u0, q, b), x = fma(e, r, q), r = 1/u0, which is the
-// correctly rounded quotient b/u0 (Markstein's theorem; r correctly rounded).
+// are exactly zero (x - t*0 = x) and multiplies by r = 1/u0 (correctly
+// rounded) instead of dividing by the diagonal.
 // grid: x = pairs/64, y = 4 matrices, z = slot.
 // ---------------------------------------------------------------------------
 __global__ void k_lu_pack(Geo G, Bufs B, int slot0) {

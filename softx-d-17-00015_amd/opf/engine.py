@@ -219,6 +219,15 @@ class Context:
     def kernel_timing(self, enable=True):
         check(self.lib.opf_kernel_timing(self.h, int(bool(enable))))
 
+    def step_config(self):
+        """(pairs per workgroup, threads per workgroup, LDS bytes) of the step kernel."""
+        gp, thr, lds = C.c_int(), C.c_int(), C.c_longlong()
+        check(self.lib.opf_step_config(self.h, C.byref(gp), C.byref(thr), C.byref(lds)))
+        return gp.value, thr.value, lds.value
+
+    def step_pairs(self):
+        return self.step_config()[0]
+
     def kernel_time(self, name):
         ms = C.c_double()
         n = C.c_int()

@@ -244,3 +244,30 @@ def test_large_size_properties(opf):
     st = ctx.status(0)
     assert st["status"] == 0 and st["step"] == 2
     del rng
+
+
+@pytest.mark.parametrize("name", ["mini", "c0"])
+def test_pairs_per_workgroup_agree(opf, name, monkeypatch):
+    """The step kernel's grouping (1, 2 or 4 (|k|, m) pairs per workgroup)
+    changes only which lanes and band widths a harmonic shares, never its
+    arithmetic: the states agree bit for bit, and four pairs per workgroup
+    meets the golden series."""
+    g = load_golden(name)
+    ic = g["ic"]
+    states = {}
+    for gp in (1, 2, 4):
+        monkeypatch.setenv("OPF_STEP_GP", str(gp))
+        ctx = _setup(opf, g)
+        assert ctx.step_pairs() == gp
+        ctx.put_state(0, ic[0], ic[1], ic[2])
+        ctx.run(100, diag_every=1)
+        states[gp] = (ctx.get_state(0), ctx.diag(0))
+    monkeypatch.delenv("OPF_STEP_GP")
+    for gp in (1, 4):
+        for c in range(3):
+            assert np.array_equal(states[gp][0][c], states[2][0][c]), (gp, c)
+        assert np.array_equal(states[gp][1], states[2][1]), gp
+    d = states[4][1]
+    rs = g["series"]
+    assert np.max(np.abs(d[:, 2] - rs[:, 2]) / rs[:, 2]) <= TOL_DIAG
+    assert np.array_equal(d[:, 8].astype(int), rs[:, 8].astype(int))
