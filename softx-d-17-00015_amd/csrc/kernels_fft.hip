@@ -138,10 +138,15 @@ __host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 // tw: the length-L twiddle table in LDS, twg: the same table in global
 // memory, read with uniform indices for the butterflies' internal twiddles.
 // Outer twiddles are fetched four at a time to bound register use.
-template <int R>
-__device__ __forceinline__ void fft_pass(double2* buf, int L, int Lp, int B, int Ns, double sgn, const double2* tw,
+// LC, NSC: the length and Ns at compile time (0: the run-time L_, Ns_), so
+// that the index divisions become multiplications by constants.
+template <int R, int LC, int NSC>
+__device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, double sgn, const double2* tw,
                                          const double2* twg) {
   constexpr int MB = (FFT_TILE + R * FFT_THREADS - 1) / (R * FFT_THREADS);
+  const int L = LC ? LC : L_;
+  const int Ns = LC ? NSC : Ns_;
+  const int Lp = padded_len(L);
   const int nb = L / R;
   const int total = nb * B;
   const int tid = threadIdx.x;
@@ -194,35 +199,41 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L, int Lp, int B, int
 // Full transform: sgn = +1 (to physical space) or -1, unnormalised (FFTW).
 __device__ __forceinline__ void fft_lds(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
-  const int Lp = padded_len(pl.L);
   int Ns = 1;
   for (int p = 0; p < pl.nrad; ++p) {
     switch (pl.rad[p]) {
-      case 16: fft_pass<16>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 12: fft_pass<12>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 9: fft_pass<9>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 8: fft_pass<8>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 6: fft_pass<6>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 4: fft_pass<4>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      case 3: fft_pass<3>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
-      default: fft_pass<2>(buf, pl.L, Lp, B, Ns, sgn, tw, twg); break;
+      case 16: fft_pass<16, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 12: fft_pass<12, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 9: fft_pass<9, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 8: fft_pass<8, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 6: fft_pass<6, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 4: fft_pass<4, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 3: fft_pass<3, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      default: fft_pass<2, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
     }
     Ns *= pl.rad[p];
   }
 }
 
+// length of the radix sequence R0 R1 R2 (0: not fixed at compile time)
+template <int R0, int R1, int R2>
+constexpr int plan_len() {
+  return R0 * (R1 > 0 ? R1 : 1) * (R2 > 0 ? R2 : 1);
+}
+
 // Transform with the radix sequence fixed at compile time (R0 = 0: the
-// plan's own sequence).  Each pass then keeps only its own registers live.
+// plan's own sequence).  Each pass then keeps only its own registers live,
+// and lengths and strides are constants.
 template <int R0, int R1, int R2>
 __device__ __forceinline__ void run_fft(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
   if constexpr (R0 == 0) {
     fft_lds(buf, B, pl, sgn, tw, twg);
   } else {
-    const int Lp = padded_len(pl.L);
-    fft_pass<R0>(buf, pl.L, Lp, B, 1, sgn, tw, twg);
-    if constexpr (R1 > 1) fft_pass<R1>(buf, pl.L, Lp, B, R0, sgn, tw, twg);
-    if constexpr (R2 > 1) fft_pass<R2>(buf, pl.L, Lp, B, R0 * R1, sgn, tw, twg);
+    constexpr int L = plan_len<R0, R1, R2>();
+    fft_pass<R0, L, 1>(buf, L, B, 1, sgn, tw, twg);
+    if constexpr (R1 > 1) fft_pass<R1, L, R0>(buf, L, B, R0, sgn, tw, twg);
+    if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1>(buf, L, B, R0 * R1, sgn, tw, twg);
   }
 }
 
@@ -274,17 +285,21 @@ __device__ __forceinline__ void load_zcols(double2* buf, const double2* row, int
 }
 
 // column b of the tile (padded layout) = Y[z*ystride + m0 + b], one batch.
+// TMC: the full tile width at compile time (0: use tm), items of columns
+// b >= tm are skipped.
+template <int TMC>
 __device__ __forceinline__ void load_ycols(double2* buf, const double2* Y, size_t ystride, int Z, int Zp, int m0, int tm) {
-  const int total = tm * Z;
+  const int tw = TMC ? TMC : tm;
+  const int total = tw * Z;
   double2 v[ZLD];
   int d[ZLD];
 #pragma unroll
   for (int u = 0; u < ZLD; ++u) {
     const int i = threadIdx.x + u * FFT_THREADS;
-    const bool ok = i < total;
-    const int ic = ok ? i : 0;
-    const int z = ic / tm, b = ic - z * tm;
-    v[u] = Y[(size_t)z * ystride + m0 + b];
+    const int ic = i < total ? i : 0;
+    const int z = ic / tw, b = ic - z * tw;
+    const bool ok = i < total && b < tm;
+    v[u] = Y[(size_t)z * ystride + m0 + (ok ? b : 0)];
     d[u] = ok ? b * Zp + pidx(z) : -1;
   }
 #pragma unroll
@@ -316,10 +331,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
+  constexpr int LZ = plan_len<R0, R1, R2>();
+  constexpr int TMC = LZ ? FFT_TILE / LZ : 0;
   const int n = blockIdx.y / 6, f = blockIdx.y % 6;
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
-  const int Z = G.Z, K = G.K, M = G.M;
+  const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
   const double2* src = (f < 3 ? B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll
                               : B.om + (size_t)slot * 3 * G.coll + (size_t)(f - 3) * G.coll) +
                        (size_t)n * G.Hp;
@@ -331,9 +348,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
   __syncthreads();
   run_fft<R0, R1, R2>(buf, tm, T.pz, 1.0, twl, B.twZ);
   double2* Y = B.Yf + (((size_t)slot * G.N + n) * Z) * 6 * M;
-  for (int i = threadIdx.x; i < tm * Z; i += FFT_THREADS) {
-    const int z = i / tm, b = i - z * tm;
-    Y[((size_t)z * 6 + f) * M + m0 + b] = buf[b * Zp + pidx(z)];
+  const int tw = TMC ? TMC : tm;
+  for (int i = threadIdx.x; i < tw * Z; i += FFT_THREADS) {
+    const int z = i / tw, b = i - z * tw;
+    if (b < tm) Y[((size_t)z * 6 + f) * M + m0 + b] = buf[b * Zp + pidx(z)];
   }
 }
 
@@ -349,10 +367,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
+  constexpr int LT = plan_len<R0, R1, R2>();
   const int n = blockIdx.y;
   const int z0 = blockIdx.x * T.tz;
   const int tz = min(T.tz, G.Z - z0);
-  const int M = G.M, TT = G.T;
+  const int TT = LT ? LT : G.T, M = LT ? LT / 3 : G.M;   // T = 3M
   const double2* Yf = B.Yf + (((size_t)slot * G.N + n) * G.Z + z0) * 6 * M;
   const int Tp = padded_len(TT);
   double2* buf = smem;
@@ -440,16 +459,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles 
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
+  constexpr int LZ = plan_len<R0, R1, R2>();
   const int n = blockIdx.y / 3, f = blockIdx.y % 3;
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
-  const int Z = G.Z, K = G.K, M = G.M;
+  const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
   const double2* Y = B.Yb + (((size_t)slot * G.N + n) * Z) * 3 * M + (size_t)f * M;
   const int Zp = padded_len(Z);
   double2* buf = smem;
   double2* twl = smem + (size_t)T.tm * Zp;
   load_tw(twl, B.twZ, Z);
-  load_ycols(buf, Y, (size_t)3 * M, Z, Zp, m0, tm);
+  load_ycols<LZ ? FFT_TILE / LZ : 0>(buf, Y, (size_t)3 * M, Z, Zp, m0, tm);
   __syncthreads();
   run_fft<R0, R1, R2>(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = B.Nraw[c->ncur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll + (size_t)n * G.Hp;
@@ -546,7 +566,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_one(Geo G, Bufs B, FftTile
   double2* buf = smem;
   double2* twl = smem + (size_t)T.tm * Zp;
   load_tw(twl, B.twZ, Z);
-  load_ycols(buf, Y + (size_t)n * Z * M, (size_t)M, Z, Zp, m0, tm);
+  load_ycols<0>(buf, Y + (size_t)n * Z * M, (size_t)M, Z, Zp, m0, tm);
   __syncthreads();
   fft_lds(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = a + (size_t)n * G.Hp;
