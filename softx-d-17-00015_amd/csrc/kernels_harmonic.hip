@@ -629,6 +629,11 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   const double c2 = (1.0 - prm.implicit) * prm.nu / prm.Re;
   const double wnorm = 4.0 * D_PI * D_PI / prm.alpha;
   const int NI = N * L.GH;
+  // items it = tid + q * THR of the parallel stages: radial point it / GH,
+  // harmonic slot it % GH, the same slot for all of a thread's items
+  static_assert(THR % (2 * GP) == 0, "threads must cover whole radial rows");
+  const int lh0 = tid % (2 * GP);
+  const Slot S0 = slot_of(lh0, GP, np, p0, K);
 
   // ---- stage 0: u_ of the group into V[0..2]; band metadata of the group --
   if (tid < 2) {
@@ -654,8 +659,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int it = tid + q * THR;
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     const bool ok = it < NI && S.valid;
     const size_t o = ok ? (size_t)n * hs + S.h : 0;
 #pragma unroll
@@ -674,7 +679,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   for (int q = 0; q < IT; ++q) {
     const int it = tid + q * THR;
     if (it >= NI) continue;
-    const int n = it / L.GH, lh = it - n * L.GH;
+    const int n = it / L.GH, lh = lh0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) Vat(L, c, n, lh, N) = vb[q][c];
   }
@@ -698,8 +703,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     const int it = tid + q * THR;
     res[q][0] = res[q][1] = res[q][2] = make_double2(0.0, 0.0);
     if (it >= NI) continue;
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     if (!S.valid) {
       if (want_diag)
         for (int c = 0; c < 3; ++c) L.E[((size_t)c * N + n) * L.GH + lh] = 0.0;
@@ -790,7 +795,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   for (int q = 0; q < IT; ++q) {
     const int it = tid + q * THR;
     if (it >= NI) continue;
-    const int n = it / L.GH, lh = it - n * L.GH;
+    const int n = it / L.GH, lh = lh0;
     Vat(L, 0, n, lh, N) = res[q][0];
     Vat(L, 1, n, lh, N) = res[q][1];
     Vat(L, 2, n, lh, N) = res[q][2];
@@ -813,8 +818,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   // shared its memory have been summed)
   if (loader) ld_dma<GP, THR / 64 - 1>(L, B, st, 0, slot, p0, np, P, N, lane);
   for (int it = tid; it < NI; it += THR) {
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     double2 dv = make_double2(0.0, 0.0);
     if (S.valid && !S.h00 && n != 0 && n != N - 1) {
       const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)S.m;
@@ -845,8 +850,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 
   // ---- stage 4: subtract grad p (rt2pm), zero the BC rows ----------------
   for (int it = tid; it < NI; it += THR) {
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     if (!S.valid) continue;
     if (n == 0 || n == N - 1) {
       for (int c = 0; c < 3; ++c) Vat(L, c, n, lh, N) = make_double2(0.0, 0.0);
@@ -913,8 +918,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   }
   __syncthreads();
   for (int it = tid; it < NI; it += THR) {
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     if (!S.valid) continue;
     double2 up = Vat(L, 0, n, lh, N), um = Vat(L, 1, n, lh, N), uz = Vat(L, 2, n, lh, N);
     if (!S.h00) {
@@ -954,8 +959,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   __syncthreads();
   double dmax = 0.0;
   for (int it = tid; it < NI; it += THR) {
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     if (!S.valid) continue;
     double2 ur = Vat(L, 0, n, lh, N), ut = Vat(L, 1, n, lh, N), uz = Vat(L, 2, n, lh, N);
     if (S.h00) {
@@ -983,8 +988,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 
   // ---- stage 7: curl of the new velocity for the next evaluation ---------
   for (int it = tid; it < NI; it += THR) {
-    const int n = it / L.GH, lh = it - n * L.GH;
-    const Slot S = slot_of(lh, GP, np, p0, K);
+    const int n = it / L.GH, lh = lh0;
+    const Slot& S = S0;
     if (!S.valid) continue;
     const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)S.m;
     const double ak = prm.alpha * (double)S.k;
