@@ -344,6 +344,10 @@ class Ctx:
         self.Pr0 = 0.0
         self.t = 0.0
         self.step = 0
+        self.corr_dt = 0.0        # tim_corr_dt (timestep.f90:35)
+        self.cfl_dt = 0.0         # tim_cfl_dt (timestep.f90:36)
+        self.cfl_dir = 0
+        self._ntc = 0             # the saved counter i of tim_new_tstep (timestep.f90:261)
         self.matrices()
 
     # -- shapes --------------------------------------------------------------
@@ -908,6 +912,58 @@ class Ctx:
         self.dterr = 0.0
         return status
 
+    def maxtstep(self):
+        """vel_maxtstep (velocity.f90:438-481) on one rank: the CFL limits of
+        u_r (radial spacing), u_theta (2 pi / (3M Mp)) and u_z plus the base
+        flow 1 - r^2 (2 pi / (alpha 3K)), from the physical fields of the last
+        transform; tim_cfl_dir is the last direction whose limit equals the
+        minimum."""
+        N, r = self.N, self.mesh.r
+        dts = [1e99, 1e99, 1e99]
+        for n in range(N):
+            if n == 0:
+                d = r[1] - r[0]
+            elif n == N - 1:
+                d = r[N - 1] - r[N - 2]
+            else:
+                d = min(r[n] - r[n - 1], r[n + 1] - r[n])
+            mx = float(np.abs(self.p_r[n]).max())
+            if mx != 0.0:
+                dts[0] = min(dts[0], d / mx)
+            d = 2.0 * math.pi / float(3 * self.M * self.Mp)
+            mx = float(np.abs(self.p_t[n]).max())
+            if mx != 0.0:
+                dts[1] = min(dts[1], d / mx)
+            d = 2.0 * math.pi / (self.alpha * float(3 * self.K))
+            mx = float(np.abs(self.p_z[n] + (1.0 - r[n] * r[n])).max())
+            if mx != 0.0:
+                dts[2] = min(dts[2], d / mx)
+        self.cfl_dt = min(dts)
+        for i in range(3):
+            if self.cfl_dt == dts[i]:
+                self.cfl_dir = i + 1
+        return self.cfl_dt, self.cfl_dir
+
+    def new_tstep(self, maxdt=1e99, courant=0.5):
+        """tim_new_tstep (timestep.f90:259-278): grow by at most 11% (not
+        again within 99 steps of a growth), cap by the CFL and corrector
+        limits, change only beyond 5% / 10%.  Returns tim_new_dt."""
+        dt = min(self.dt * 1.11, maxdt)
+        if self.step == 0 and self.corr_dt == 0.0:
+            dt = min(dt, self.cfl_dt * 0.1)
+        if self.cfl_dt > 0.0:
+            dt = min(dt, self.cfl_dt * courant)
+        if self.corr_dt > 0.0:
+            dt = min(dt, self.corr_dt * 0.95)
+        self._ntc -= 1
+        new = (dt < self.dt * 0.95 or (dt > self.dt * 1.10 and self._ntc < 0)
+               or (dt == maxdt and self.dt < dt and self._ntc < 0))
+        if new:
+            if dt >= self.dt:
+                self._ntc = 99
+            self.dt = dt
+        return new
+
     def diagnostics(self):
         """io_write_energy (io.f90:501-524) and io_write_friction
         (io.f90:627-639)."""
@@ -928,16 +984,29 @@ class Ctx:
         Ufr = math.sqrt(abs((s - 2.0) / self.Re))
         return dict(t=self.t, E=E, Ek0=Ek0, Em0=Em0, Pr0=self.Pr0, Uc=Uc, Ufr=Ufr)
 
-    def run(self, nsteps, ncorr_fixed=None, record=None):
+    def run(self, nsteps, ncorr_fixed=None, record=None, adaptive=False, maxdt=1e99, courant=0.5):
         """The loop of main.f90:18-46 (no I/O).  With ncorr_fixed the corrector
         runs exactly that many times per step (the steporbit schedule of
-        newton.f90:743-787 uses one).  Returns the per-step series."""
+        newton.f90:743-787 uses one).  adaptive: the d_timestep < 0 block of
+        main.f90:24-29 (vel_maxtstep, tim_new_tstep, vel_matrices on a
+        change).  Returns the per-step series; adaptive runs add the
+        timestep quantities of io_write_timestep (io.f90:645-657)."""
         series = []
         while self.step < nsteps:
             self.imposesym()
             self.transform()
             self.nonlinear()
+            tq = None
+            if adaptive:
+                self.maxtstep()
+                new = self.new_tstep(maxdt, courant)
+                if new:
+                    self.matrices()
+                tq = dict(dt=self.dt, cfl_dt=self.cfl_dt, cfl_dir=self.cfl_dir, corr_dt=self.corr_dt,
+                          new_dt=int(new))
             diag = self.diagnostics()
+            if tq is not None:
+                diag.update(tq)
             self.predictor()
             self.it = 1
             ncorr = 0

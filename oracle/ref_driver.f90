@@ -11,8 +11,15 @@
 ! coll layout: Re(1:N,0:H-1) then Im(1:N,0:H-1) for ur, ut, uz.
 !
 ! usage: opf_ref_<cfg> state_in out_prefix Re alpha dt nsteps [dump steps...]
+!   dt < 0 selects the adaptive time step of main.f90:24-29 (d_timestep < 0)
+!   with |dt| as the initial tim_dt: vel_maxtstep, tim_new_tstep and, on a
+!   change, vel_matrices before every predictor.
 !   writes out_prefix_series.txt   one line per step:
 !          step t E Ek0 Em0 Pr0(=Ub) Uc Ufr ncorr dterr_last
+!   out_prefix_dt.txt              adaptive runs, one line per step after
+!                                  tim_new_tstep: step tim_dt tim_cfl_dt
+!                                  tim_cfl_dir tim_corr_dt tim_new_dt
+!                                  (io_write_timestep, io.f90:645-657)
 !   out_prefix_stage_*.bin         step-1 intermediates (N after each
 !                                  nonlinear evaluation, u after predictor
 !                                  and after each corrector)
@@ -28,7 +35,8 @@
    integer, allocatable :: dumps(:)
    integer(8)         :: c0, c1, crate
    double precision   :: dterr_last, E, Ek0, Em0, Uc, Ufr, Ub
-   integer, parameter :: useries = 41, utim = 42
+   integer, parameter :: useries = 41, utim = 42, udt = 43
+   logical            :: adaptive
 
    if(command_argument_count() < 6) stop 'usage: state_in prefix Re alpha dt nsteps [dumps]'
    call get_command_argument(1, infile)
@@ -53,11 +61,13 @@
    call tra_precompute()
    call tim_precompute()
    call vel_precompute()
-   tim_dt = dt_in
+   adaptive = dt_in < 0d0
+   tim_dt = dabs(dt_in)
    call read_state(trim(infile))
    call vel_matrices()
 
    open(useries, file=trim(prefix)//'_series.txt', status='replace')
+   if(adaptive) open(udt, file=trim(prefix)//'_dt.txt', status='replace')
    nstage = 0
    call system_clock(c0, crate)
 
@@ -66,6 +76,13 @@
       call vel_transform()
       call vel_nonlinear()
       if(tim_step==0) call dump_stage('N', vel_Nr, vel_Nt, vel_Nz)
+      if(adaptive) then                       ! main.f90:24-29
+         call vel_maxtstep()
+         call tim_new_tstep()
+         if(tim_new_dt) call vel_matrices()
+         write(udt,'(I8,2ES26.17E3,I3,ES26.17E3,I3)') tim_step, tim_dt, tim_cfl_dt,  &
+            tim_cfl_dir, tim_corr_dt, merge(1, 0, tim_new_dt)
+      end if
       call diagnostics(E, Ek0, Em0, Uc, Ufr)
       Ub = vel_Pr0            ! io_write_friction reads it here (io.f90:631)
       call vel_predictor()
@@ -94,6 +111,7 @@
 
    call system_clock(c1)
    close(useries)
+   if(adaptive) close(udt)
    open(utim, file=trim(prefix)//'_timing.txt', status='replace')
    write(utim,'(ES26.17E3,I8)') dble(c1-c0)/dble(crate), nsteps
    close(utim)

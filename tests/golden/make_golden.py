@@ -6,7 +6,8 @@ by the package's generator on the oracle's mesh, written in the reference's
 raw coll layout, and integrated by the reference executable; the fixture keeps
 the initial state, the reference's states after chosen steps, its step-1 stage
 dumps (mini only) and its per-step series (t, E, Ek0, Em0, Pr0, Uc, Ufr,
-ncorr, dterr).  Fixtures are data only: inputs and the reference's outputs.
+ncorr, dterr); adaptive runs (dt < 0) add the reference's per-step timestep
+quantities.  Fixtures are data only: inputs and the reference's outputs.
 
 usage: python tests/golden/make_golden.py [mini c0 s48 c1 ...]
 """
@@ -33,7 +34,12 @@ WORKLOADS = {
     "c0": (32, 24, 12, 2400.0, 1.25, 0.01, 1e-3, 1, 100, (1, 100), True),
     "s48": (48, 48, 24, 3000.0, 1.25, 0.01, 1e-3, 7, 50, (1,), False),
     "c1": (64, 64, 48, 5300.0, 0.5, 0.005, 1e-2, 1, 1000, (1,), False),
+    # adaptive time step (dt < 0: |dt| is the initial tim_dt, main.f90:24-29)
+    "adapt_mini": (16, 6, 4, 2400.0, 1.25, -0.01, 1e-3, 1, 120, (1, 120), True),
+    "adapt_c0": (32, 24, 12, 2400.0, 1.25, -0.01, 1e-3, 1, 200, (200,), True),
 }
+# executable of each workload (the reference is compiled per resolution)
+EXE = {"adapt_mini": "mini", "adapt_c0": "c0"}
 SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
 
 
@@ -45,7 +51,7 @@ def _unlimited_stack():
 
 def make(name):
     N, K, M, Re, alpha, dt, E0, seed, nsteps, saves, full = WORKLOADS[name]
-    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{name}")
+    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{EXE.get(name, name)}")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} missing: run oracle/build_ref.sh {name} {N} {K} {M}")
     mesh = O.build_mesh(N)
@@ -63,6 +69,8 @@ def make(name):
                        stdout=subprocess.DEVNULL, preexec_fn=_unlimited_stack)
         ser = np.loadtxt(pre + "_series.txt", ndmin=2)
         out["series"] = ser
+        if dt < 0:   # step tim_dt tim_cfl_dt tim_cfl_dir tim_corr_dt tim_new_dt
+            out["dtser"] = np.loadtxt(pre + "_dt.txt", ndmin=2)
         sel = np.arange(0, H, SAMPLE_STRIDE) if not full else np.arange(H)
         out["sel"] = sel
         if full:
@@ -86,5 +94,5 @@ def make(name):
 
 
 if __name__ == "__main__":
-    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1"]):
+    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0"]):
         make(nm)

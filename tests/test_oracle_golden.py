@@ -115,3 +115,30 @@ def test_one_step_s48_sampled():
     for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
         den = g["state_1_norm"][c]
         assert np.abs(u[:, sel] - ref[c]).max() / den <= TOL_STEP
+
+
+def test_adaptive_timestep_mini():
+    """The d_timestep < 0 loop (main.f90:24-29): vel_maxtstep and
+    tim_new_tstep of the oracle against the reference's own per-step
+    timestep quantities over 120 steps (dt changes at steps 0 and 100: the
+    first-step CFL cap and a growth after the 99-step hold), the energy
+    series and the final state."""
+    g = load_golden("adapt_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    ctx = O.Ctx(N, K, M, Re=float(g["Re"]), alpha=float(g["alpha"]), dt=abs(float(g["dt"])))
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    ser = ctx.run(int(g["nsteps"]), adaptive=True)
+    ref = g["dtser"]            # step tim_dt tim_cfl_dt tim_cfl_dir tim_corr_dt tim_new_dt
+    assert len(ser) == ref.shape[0]
+    new = np.array([s["new_dt"] for s in ser])
+    assert np.array_equal(new, ref[:, 5].astype(int))
+    assert np.nonzero(new)[0].tolist() == [0, 100]
+    assert np.array_equal([s["cfl_dir"] for s in ser], ref[:, 3].astype(int))
+    for key, col in (("dt", 1), ("cfl_dt", 2), ("corr_dt", 4)):
+        got = np.array([s[key] for s in ser])
+        assert np.max(np.abs(got - ref[:, col]) / np.maximum(np.abs(ref[:, col]), 1e-300)) <= 1e-12, key
+    E = np.array([s["E"] for s in ser])
+    assert np.max(np.abs(E - g["series"][:, 2]) / g["series"][:, 2]) <= TOL_DIAG
+    for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
+        assert rel_err(u, g["state_120"][c]) <= 1e-10, c

@@ -237,3 +237,34 @@ def test_swirl_mode_decays_at_stokes_rate():
     g = (1 / dt - (1 - 0.51) * lam / Re) / (1 / dt + 0.51 * lam / Re)
     assert abs(ratio - g ** 10) < 1e-6
     assert np.abs(ctx.ur).max() == 0.0
+
+
+def test_maxtstep_laminar_and_single_mode():
+    """vel_maxtstep: for the laminar state only the axial limit is finite,
+    2 pi / (alpha 3K (1 - r_1^2)) at the innermost point (the base flow peaks
+    there); a single radial mode u_r = a(r) cos(alpha z) (real amplitude a)
+    reaches |a(r_n)| on the grid, so the radial limit is min_n d_n / |a(r_n)|
+    with the one-sided spacing at both ends."""
+    N, K, M, alpha = 12, 4, 3, 1.25
+    c = O.Ctx(N, K, M, alpha=alpha)
+    z = np.zeros((N, c.harm.k.size), dtype=complex)
+    c.set_state(z, z.copy(), z.copy())
+    c.transform()
+    cfl, d = c.maxtstep()
+    r = c.mesh.r
+    assert d == 3
+    assert abs(cfl - 2.0 * math.pi / (alpha * 3 * K * (1.0 - r[0] ** 2))) <= 1e-15 * cfl
+    # u_r of harmonic (k=1, m=0): physical value 2 a(r) cos(alpha z)
+    a = 1e-3 * r * (1.0 - r ** 2)
+    ur = z.copy()
+    h = int(np.nonzero((c.harm.k == 1) & (c.harm.m == 0))[0][0])
+    ur[:, h] = a
+    c.ur = ur
+    c.transform()
+    c.maxtstep()
+    gaps = np.diff(r)
+    dr = np.concatenate([[gaps[0]], np.minimum(gaps[:-1], gaps[1:]), [gaps[-1]]])
+    live = a != 0.0             # the wall point r = 1 carries no velocity (mx = 0 is skipped)
+    want = np.min(dr[live] / (2.0 * np.abs(a[live])))
+    assert abs(c.cfl_dt - min(want, cfl)) <= 1e-14 * want
+    assert c.cfl_dir == (1 if want < cfl else 3)
