@@ -83,6 +83,27 @@ int opf_mesh_build(int N, const double* r_in, double* r, double* intrdr, double*
 int opf_slot_setup(opf_ctx* ctx, int slot, double Re, double alpha, double dt, double implicit,
                    int const_flux);
 
+/* ---- adaptive time step (d_timestep < 0) ------------------------------------ */
+
+/* The adaptive time step of main.f90:24-29 for one slot: before every
+ * predictor of opf_run, vel_maxtstep (velocity.f90:438-481) and
+ * tim_new_tstep (timestep.f90:259-278) run on the device and, when tim_dt
+ * changes, vel_matrices.  enable = 0 restores the fixed tim_dt.  maxdt and
+ * courant are d_maxdt and d_courant (parameters.f90:39, 41; 1e99 and 0.5 in
+ * the reference); courant also sets the corrector's fallback
+ * tim_corr_dt = tim_dt / (10 courant) (timestep.f90:239). */
+int opf_slot_adaptive(opf_ctx* ctx, int slot, int enable, double maxdt, double courant);
+/* tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir of the slot (the columns of
+ * io_write_timestep, io.f90:645-657). */
+int opf_slot_timestep(opf_ctx* ctx, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir);
+/* vel_maxtstep (velocity.f90:438-481) for a host-driven loop: the CFL limit
+ * and its direction from the physical fields of the last
+ * opf_vel_transform + opf_vel_nonlinear. */
+int opf_vel_maxtstep(opf_ctx* ctx, int slot, double* cfl_dt, int* cfl_dir);
+/* vel_matrices (velocity.f90:120-146) for a new tim_dt, the slot's other
+ * parameters unchanged (main.f90:27-28 after tim_new_tstep). */
+int opf_vel_matrices(opf_ctx* ctx, int slot, double dt);
+
 /* Load / read a state (vel_ur, vel_ut, vel_uz and tim_t); the layout of
  * io_load_state after radial interpolation (io.f90:98-165).  put applies
  * vel_imposesym (velocity.f90:61-74) and resets the step counter. */

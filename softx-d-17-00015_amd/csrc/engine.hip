@@ -87,6 +87,9 @@ struct opf_ctx {
   size_t bytes = 0;
   std::vector<int> occupied;   // slot holds a state
   std::vector<int> setup;      // slot matrices built
+  struct Adapt { int on = 0; double maxdt = 1e99, courant = 0.5; };
+  std::vector<Adapt> adapt;    // d_timestep < 0, d_maxdt, d_courant per slot
+  int n_adaptive = 0;          // slots with the adaptive time step on
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
@@ -137,6 +140,7 @@ size_t bytes_needed(int N, int K, int M, int cap, int diag_cap) {
   b += (size_t)cap * 6 * N * s.P * sizeof(double);           // Uinf
   b += (size_t)cap * 16 * s.H * sizeof(double);              // Ainv
   b += (size_t)cap * 3 * s.Hp * sizeof(double);              // diagb
+  b += (size_t)cap * 3 * N * sizeof(unsigned long long);     // cflmax
   b += (size_t)cap * diag_cap * sizeof(DiagRec);
   b += (size_t)cap * N * s.Z * 9 * M * sizeof(double2);      // Yf + Yb
   return b;
@@ -238,8 +242,25 @@ void launch_step(opf_ctx* c, int s0, int ns) {
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }
 
+// vel_matrices for slots s0 .. s0+ns-1 (gated: only slots whose control
+// block asks for it, the tim_new_dt of this step)
+void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
+  const dim3 gp((c->P + 63) / 64, 4, ns);
+  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, s0, gated);
+  launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, s0, gated);
+  launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, ns), dim3(64), 0, c->G, c->B, s0, gated);
+  launch(c, "ainv", k_ainv, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, gated);
+}
+
+// One iteration.  With adaptive slots, the predictor's evaluation is
+// followed by vel_maxtstep + tim_new_tstep and, where dt changed,
+// vel_matrices (main.f90:24-29), all decided on the device.
 void launch_iteration(opf_ctx* c, int s0, int ns) {
   launch_eval(c, s0, ns);
+  if (c->n_adaptive > 0) {
+    launch(c, "newdt", k_newdt, dim3(1, 1, ns), dim3(64), 0, c->G, c->B, s0, 0);
+    launch_matrices(c, s0, ns, 1);
+  }
   launch_step(c, s0, ns);
 }
 
@@ -338,6 +359,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   B.Ainv = c->alloc<double>((size_t)cap * 16 * s.H);
   B.Ui = c->alloc<double>((size_t)cap * N);
   B.diagb = c->alloc<double>((size_t)cap * 3 * s.Hp);
+  B.cflmax = c->alloc<unsigned long long>((size_t)cap * 3 * N);
   B.diag = c->alloc<DiagRec>((size_t)cap * c->diag_cap);
   B.prm = c->alloc<SlotParams>(cap);
   B.ctl = c->alloc<SlotCtl>(cap);
@@ -367,6 +389,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   write_ctl(c, ctl);
   c->occupied.assign(cap, 0);
   c->setup.assign(cap, 0);
+  c->adapt.assign(cap, opf_ctx::Adapt{});
 }
 
 void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
@@ -375,17 +398,15 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   SlotParams p{};
   p.Re = Re; p.alpha = alpha; p.dt = dt; p.implicit = implicit; p.nu = 1.0;
   p.dterr_tol = 1e-5;   // d_dterr (parameters.f90:40)
-  p.courant = 0.5;      // d_courant (parameters.f90:41)
+  p.courant = c->adapt[slot].courant;   // d_courant (parameters.f90:41)
   p.const_flux = cf;
+  p.adaptive = c->adapt[slot].on;        // d_timestep < 0 (parameters.f90:38)
+  p.maxdt = c->adapt[slot].maxdt;        // d_maxdt (parameters.f90:39)
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
   auto ctl = read_ctl(c);
   ctl[slot].status = 0;
   write_ctl(c, ctl);
-  const dim3 gp((c->P + 63) / 64, 4, 1);
-  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot);
-  launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, slot);
-  launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
-  launch(c, "ainv", k_ainv, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
+  launch_matrices(c, slot, 1, 0);
   ctl = read_ctl(c);
   if (ctl[slot].status != 0) {
     const int st = ctl[slot].status;
@@ -413,6 +434,11 @@ void put_state(opf_ctx* c, int slot, const double* const* f, double t) {
   k.ndiag = 0;
   k.active = 1;
   k.dterr_bits = 0;
+  k.corr_dt = 0.0;   // tim_corr_dt, tim_cfl_dt and tim_new_tstep's counter of a fresh run
+  k.cfl_dt = 0.0;
+  k.cfl_dir = 0;
+  k.ntc = 0;
+  k.refactor = 0;
   write_ctl(c, ctl);
   launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
   launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
@@ -615,6 +641,65 @@ extern "C" int opf_mesh_build(int N, const double* r_in, double* r, double* intr
 
 int opf_slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
   return guard([&] { slot_setup(c, slot, Re, alpha, dt, implicit, cf); });
+}
+
+int opf_slot_adaptive(opf_ctx* c, int slot, int enable, double maxdt, double courant) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (enable && !(maxdt > 0.0 && courant > 0.0)) throw OpfError(OPF_ERR_ARG, "maxdt and courant must be > 0");
+    opf_ctx::Adapt& a = c->adapt[slot];
+    c->n_adaptive += (enable ? 1 : 0) - a.on;
+    a.on = enable ? 1 : 0;
+    if (enable) {
+      a.maxdt = maxdt;
+      a.courant = courant;
+    }
+    if (c->setup[slot]) {   // the slot's parameters on the device follow at once
+      SlotParams p{};
+      HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+      p.adaptive = a.on;
+      p.maxdt = a.maxdt;
+      p.courant = a.courant;
+      HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
+    }
+  });
+}
+
+int opf_slot_timestep(opf_ctx* c, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir) {
+  return guard([&] {
+    check_slot(c, slot);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    SlotParams p{};
+    HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+    auto ctl = read_ctl(c);
+    if (dt) *dt = p.dt;
+    if (corr_dt) *corr_dt = ctl[slot].corr_dt;
+    if (cfl_dt) *cfl_dt = ctl[slot].cfl_dt;
+    if (cfl_dir) *cfl_dir = ctl[slot].cfl_dir;
+  });
+}
+
+int opf_vel_maxtstep(opf_ctx* c, int slot, double* cfl_dt, int* cfl_dir) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    launch(c, "newdt", k_newdt, dim3(1, 1, 1), dim3(64), 0, c->G, c->B, slot, 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto ctl = read_ctl(c);
+    if (cfl_dt) *cfl_dt = ctl[slot].cfl_dt;
+    if (cfl_dir) *cfl_dir = ctl[slot].cfl_dir;
+  });
+}
+
+int opf_vel_matrices(opf_ctx* c, int slot, double dt) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (opf_slot_setup)");
+    if (!(dt > 0.0)) throw OpfError(OPF_ERR_ARG, "dt must be > 0");
+    SlotParams p{};
+    HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+    slot_setup(c, slot, p.Re, p.alpha, dt, p.implicit, p.const_flux);
+  });
 }
 
 int opf_put_state(opf_ctx* c, int slot, const double* urr, const double* uri, const double* utr,

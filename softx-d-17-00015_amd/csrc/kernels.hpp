@@ -22,6 +22,8 @@ struct Bufs {
   double* Ainv;         // inverted influence matrices [slot][16][H]
   double* Ui;           // flux correction vector [slot][N]
   double* diagb;        // per-harmonic energy partials [slot][3][Hp]
+  unsigned long long* cflmax;  // [slot][3][N] per-plane max of |u_r|, |u_t|, |u_z + 1 - r^2| on the
+                               // physical grid of the last evaluation (bit patterns, vel_maxtstep)
   DiagRec* diag;        // diagnostics records [slot][diag_cap]
   SlotParams* prm;      // [slot]
   SlotCtl* ctl;         // [slot]
@@ -85,11 +87,12 @@ constexpr int FFT_TILE = 2048;   // complex values per transform workgroup (all 
 // slot per 16 elements, rounded up to an odd count.
 __host__ __device__ inline int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
 
-__global__ void k_lu_factor(Geo G, Bufs B, int slot0);
-__global__ void k_lu_pack(Geo G, Bufs B, int slot0);
-__global__ void k_influence(Geo G, Bufs B, int slot0);
-__global__ void k_ainv(Geo G, Bufs B, int slot0);
+__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated);
+__global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated);
+__global__ void k_influence(Geo G, Bufs B, int slot0, int gated);
+__global__ void k_ainv(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_curl(Geo G, Bufs B, int slot0);
+__global__ void k_newdt(Geo G, Bufs B, int slot0, int what);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);

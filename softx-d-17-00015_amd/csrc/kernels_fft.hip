@@ -337,6 +337,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
   const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
+  // a new evaluation: clear the slot's per-plane velocity maxima (the theta
+  // kernel that follows records them)
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int i = threadIdx.x; i < 3 * G.N; i += FFT_THREADS) B.cflmax[(size_t)slot * 3 * G.N + i] = 0ull;
   const double2* src = (f < 3 ? B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll
                               : B.om + (size_t)slot * 3 * G.coll + (size_t)(f - 3) * G.coll) +
                        (size_t)n * G.Hp;
@@ -417,18 +421,50 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   }
   __syncthreads();
   run_fft<R0, R1, R2>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
-  // products (velocity.f90:325-329), scaled as tra_phys2spec (:114-117)
+  // products (velocity.f90:325-329), scaled as tra_phys2spec (:114-117);
+  // the plane's max |u_r|, |u_t|, |u_z + 1 - r^2| for vel_maxtstep
+  // (velocity.f90:464-475)
   const double scale = 1.0 / (double)(G.Z * TT);
+  const double rn = G.mesh[MeshOff::make(G.N).r + n];
+  const double ubase = __dsub_rn(1.0, __dmul_rn(rn, rn));
+  double mr = 0.0, mt = 0.0, mz = 0.0;
   for (int i = threadIdx.x; i < tz * TT; i += FFT_THREADS) {
     const int line = i / TT, th = i - line * TT;
     double2* L0 = buf + (size_t)line * Tp + pidx(th);
     const double2 w0 = L0[0], w1 = L0[(size_t)tz * Tp], w2 = L0[(size_t)2 * tz * Tp];
     const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
+    mr = fmax(mr, fabs(ur));
+    mt = fmax(mt, fabs(ut));
+    mz = fmax(mz, fabs(__dadd_rn(uz, ubase)));
     const double Nr = ut * cz - uz * ct;
     const double Nt = uz * cr - ur * cz;
     const double Nz = ur * ct - ut * cr;
     L0[0] = make_double2(scale * Nr, scale * Nt);
     L0[(size_t)tz * Tp] = make_double2(scale * Nz, 0.0);
+  }
+  // maxima are exact and order free: non-negative doubles order like their
+  // bit patterns, so one integer atomic max per workgroup and component.
+  // Recorded only where they are read: an adaptive slot's predictor
+  // evaluation (the device loop) and host-driven evaluations.
+  const bool want_cfl = (B.prm[slot].adaptive && c->mode == MODE_PRED) || c->host_driven;
+  if (want_cfl) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      mr = fmax(mr, __shfl_xor(mr, off));
+      mt = fmax(mt, __shfl_xor(mt, off));
+      mz = fmax(mz, __shfl_xor(mz, off));
+    }
+    __shared__ double wmax[3][FFT_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) {
+      wmax[0][threadIdx.x >> 6] = mr;
+      wmax[1][threadIdx.x >> 6] = mt;
+      wmax[2][threadIdx.x >> 6] = mz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double v = 0.0;
+      for (int w = 0; w < FFT_THREADS / 64; ++w) v = fmax(v, wmax[threadIdx.x][w]);
+      atomicMax(B.cflmax + ((size_t)slot * 3 + threadIdx.x) * G.N + n, (unsigned long long)__double_as_longlong(v));
+    }
   }
   __syncthreads();
   run_fft<R0, R1, R2>(buf, 2 * tz, T.pt, -1.0, twl, B.twT);

@@ -240,11 +240,11 @@ __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alp
 // tim_lumesh_init (timestep.f90:67-96) + DGBTF2, one lane per (pair, matrix).
 // grid: x = pairs/64, y = 4 matrices, z = slot.
 // ---------------------------------------------------------------------------
-__global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
+__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
   const int slot = slot0 + blockIdx.z;
-  if (p >= G.P) return;
+  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;  // pair (|k|, m)
   const SlotParams prm = B.prm[slot];
@@ -322,11 +322,11 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0) {
 // rounded) instead of dividing by the diagonal.
 // grid: x = pairs/64, y = 4 matrices, z = slot.
 // ---------------------------------------------------------------------------
-__global__ void k_lu_pack(Geo G, Bufs B, int slot0) {
+__global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
   const int slot = slot0 + blockIdx.z;
-  if (p >= G.P) return;
+  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const double* F = lu_block(B, slot, mat, p, P, N);
   const unsigned char* pv = pv_block(B, slot, mat, p, P, N);
@@ -368,10 +368,10 @@ __global__ void k_add_coll(Geo G, double2* dst, const double2* src) {
 // vel_adjustFlux(0) (velocity.f90:160-168) on pair 0.
 // Uinf slots: 0 U1 (LDp), 1 U2 (LDm), 2 U3 (LDz), 3 U4, 4 U5, 5 p (U6 = p*alpha*k).
 // ---------------------------------------------------------------------------
-__global__ void k_influence(Geo G, Bufs B, int slot0) {
+__global__ void k_influence(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = slot0 + blockIdx.z;
-  if (p >= G.P) return;
+  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;
   const MeshOff mo = MeshOff::make(N);
@@ -418,10 +418,10 @@ __global__ void k_influence(Geo G, Bufs B, int slot0) {
 }
 
 // Inverted 4x4 influence matrices per harmonic (velocity.f90:219-225).
-__global__ void k_ainv(Geo G, Bufs B, int slot0) {
+__global__ void k_ainv(Geo G, Bufs B, int slot0, int gated) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = slot0 + blockIdx.z;
-  if (h >= G.H) return;
+  if (h >= G.H || (gated && !B.ctl[slot].refactor)) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
   double* Ai = B.Ainv + (size_t)slot * 16 * G.H + h;
@@ -636,6 +636,73 @@ __global__ void k_control(Geo G, Bufs B, int slot0) {
     c->mode = MODE_PRED;
     if (c->target_steps >= 0 && c->step >= c->target_steps) c->active = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// vel_maxtstep (velocity.f90:438-481) from the per-plane maxima that the
+// theta kernel recorded for the last evaluation, and tim_new_tstep
+// (timestep.f90:259-278).  what = 0: the device loop: for adaptive slots
+// about to run a predictor, both, with tim_new_dt left in `refactor` for the
+// gated vel_matrices launches that follow (every other slot: refactor = 0);
+// what = 1: vel_maxtstep only.  One wave per slot: lane l takes the planes
+// n = l, l+64, ...; the minima are exact, so their order is free.
+// ---------------------------------------------------------------------------
+__global__ void k_newdt(Geo G, Bufs B, int slot0, int what) {
+  const int slot = slot0 + blockIdx.z;
+  SlotCtl* c = &B.ctl[slot];
+  SlotParams* prm = &B.prm[slot];
+  const int lane = threadIdx.x;
+  if (what == 0) {
+    const bool act = c->active && c->mode == MODE_PRED && prm->adaptive;
+    if (!act) {
+      if (lane == 0) c->refactor = 0;
+      return;
+    }
+  }
+  const int N = G.N;
+  const MeshOff mo = MeshOff::make(N);
+  const double* r = G.mesh + mo.r;
+  const unsigned long long* mx = B.cflmax + (size_t)slot * 3 * N;
+  const double dth = (2.0 * D_PI) / (double)(G.T * G.Mp);
+  const double dz = (2.0 * D_PI) / (prm->alpha * (double)G.Z);
+  double t1 = 1e99, t2 = 1e99, t3 = 1e99;
+  for (int n = lane; n < N; n += 64) {
+    double d;
+    if (n == 0) d = r[1] - r[0];
+    else if (n == N - 1) d = r[N - 1] - r[N - 2];
+    else d = fmin(r[n] - r[n - 1], r[n + 1] - r[n]);
+    const double a = __longlong_as_double((long long)mx[n]);
+    const double b = __longlong_as_double((long long)mx[N + n]);
+    const double e = __longlong_as_double((long long)mx[2 * N + n]);
+    if (a != 0.0) t1 = fmin(t1, d / a);
+    if (b != 0.0) t2 = fmin(t2, dth / b);
+    if (e != 0.0) t3 = fmin(t3, dz / e);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    t1 = fmin(t1, __shfl_xor(t1, off));
+    t2 = fmin(t2, __shfl_xor(t2, off));
+    t3 = fmin(t3, __shfl_xor(t3, off));
+  }
+  if (lane != 0) return;
+  const double cfl = fmin(t1, fmin(t2, t3));
+  int dir = c->cfl_dir;
+  if (cfl == t1) dir = 1;
+  if (cfl == t2) dir = 2;
+  if (cfl == t3) dir = 3;
+  c->cfl_dt = cfl;
+  c->cfl_dir = dir;
+  if (what == 1) return;
+  // tim_new_tstep
+  const double dt0 = prm->dt;
+  double dt = fmin(dt0 * 1.11, prm->maxdt);
+  if (c->step == 0 && c->corr_dt == 0.0) dt = fmin(dt, cfl * 0.1);
+  if (cfl > 0.0) dt = fmin(dt, cfl * prm->courant);
+  if (c->corr_dt > 0.0) dt = fmin(dt, c->corr_dt * 0.95);
+  const int i = c->ntc - 1;
+  const bool nw = dt < dt0 * 0.95 || (dt > dt0 * 1.10 && i < 0) || (dt == prm->maxdt && dt0 < dt && i < 0);
+  c->ntc = (nw && dt >= dt0) ? 99 : i;
+  if (nw) prm->dt = dt;
+  c->refactor = nw ? 1 : 0;
 }
 
 // tim_lumesh_invert (timestep.f90:102-123) of one coll field x[n][Hp] with

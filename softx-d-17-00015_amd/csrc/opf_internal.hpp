@@ -43,7 +43,8 @@ struct SlotParams {
   double Re, alpha, dt, implicit, nu;
   double dterr_tol, courant;
   int const_flux;
-  int pad;
+  int adaptive;   // d_timestep < 0: vel_maxtstep + tim_new_tstep before every predictor
+  double maxdt;   // d_maxdt (parameters.f90:39)
 };
 
 // Slot modes for the next step kernel.
@@ -68,6 +69,10 @@ struct SlotCtl {
   int fresh;           // imposesym + curl needed before the next evaluation
   int last_ncorr;      // correctors used by the last completed step
   int host_driven;     // 1: the host runs tim_check_cgce and the clock (drop-in calls)
+  int ntc;             // the saved counter i of tim_new_tstep (timestep.f90:261)
+  int refactor;        // 1: vel_matrices runs before this step's predictor (tim_new_dt)
+  int cfl_dir;         // tim_cfl_dir
+  double cfl_dt;       // tim_cfl_dt
   double t, dterr, lasterr, corr_dt, Pr0, Pr0_hpf, last_dterr;
   unsigned long long dterr_bits;  // max |du|^2 of the running corrector (bit pattern)
   double flux_d1;      // vel_adjustFlux(0) d1

@@ -48,6 +48,10 @@ PROTOTYPES = {
     "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
     "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
     "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
+    "opf_slot_adaptive": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "opf_slot_timestep": (C.c_int, [vp, C.c_int, dp, dp, dp, ip]),
+    "opf_vel_maxtstep": (C.c_int, [vp, C.c_int, dp, ip]),
+    "opf_vel_matrices": (C.c_int, [vp, C.c_int, C.c_double]),
     "opf_step_config": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
     "opf_friction": (C.c_int, [vp, C.c_int, dp, dp, dp]),

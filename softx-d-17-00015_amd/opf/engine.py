@@ -219,6 +219,26 @@ class Context:
     def kernel_timing(self, enable=True):
         check(self.lib.opf_kernel_timing(self.h, int(bool(enable))))
 
+    # -- adaptive time step (main.f90:24-29) -----------------------------------------
+    def adaptive(self, slot, enable=True, maxdt=1e99, courant=0.5):
+        """d_timestep < 0 for the slot: vel_maxtstep + tim_new_tstep (and
+        vel_matrices on a change) before every predictor of run()."""
+        check(self.lib.opf_slot_adaptive(self.h, slot, int(bool(enable)), float(maxdt), float(courant)))
+
+    def timestep(self, slot):
+        """tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)."""
+        dt, cd, cf, di = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        check(self.lib.opf_slot_timestep(self.h, slot, C.byref(dt), C.byref(cd), C.byref(cf), C.byref(di)))
+        return dict(dt=dt.value, corr_dt=cd.value, cfl_dt=cf.value, cfl_dir=di.value)
+
+    def vel_maxtstep(self, slot):
+        cf, di = C.c_double(), C.c_int()
+        check(self.lib.opf_vel_maxtstep(self.h, slot, C.byref(cf), C.byref(di)))
+        return cf.value, di.value
+
+    def vel_matrices(self, slot, dt):
+        check(self.lib.opf_vel_matrices(self.h, slot, float(dt)))
+
     def step_config(self):
         """(pairs per workgroup, threads per workgroup, LDS bytes) of the step kernel."""
         gp, thr, lds = C.c_int(), C.c_int(), C.c_longlong()
