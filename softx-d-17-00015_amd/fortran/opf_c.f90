@@ -169,6 +169,45 @@ module opf_c
          integer(c_int) :: rc
       end function
 
+      ! vel_maxtstep (velocity.f90:438-481) of the last evaluation
+      function opf_vel_maxtstep(ctx, slot, cfl_dt, cfl_dir) bind(C, name="opf_vel_maxtstep") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: cfl_dt
+         integer(c_int), intent(out) :: cfl_dir
+         integer(c_int) :: rc
+      end function
+
+      ! vel_matrices (velocity.f90:120-146) for a new tim_dt
+      function opf_vel_matrices(ctx, slot, dt) bind(C, name="opf_vel_matrices") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), value :: dt
+         integer(c_int) :: rc
+      end function
+
+      ! d_timestep < 0 in the device-resident loop (opf_run)
+      function opf_slot_adaptive(ctx, slot, enable, maxdt, courant) bind(C, name="opf_slot_adaptive") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, enable
+         real(c_double), value :: maxdt, courant
+         integer(c_int) :: rc
+      end function
+
+      ! tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)
+      function opf_slot_timestep(ctx, slot, dt, corr_dt, cfl_dt, cfl_dir) bind(C, name="opf_slot_timestep") &
+            result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(out) :: dt, corr_dt, cfl_dt
+         integer(c_int), intent(out) :: cfl_dir
+         integer(c_int) :: rc
+      end function
+
       function opf_get_nonlinear(ctx, slot, nr_re, nr_im, nt_re, nt_im, nz_re, nz_im) &
             bind(C, name="opf_get_nonlinear") result(rc)
          import :: c_ptr, c_int, c_double

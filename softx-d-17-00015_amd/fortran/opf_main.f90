@@ -1,11 +1,13 @@
 !*************************************************************************
 ! opf_main.f90 -- the reference's main loop (main.f90:18-46) over the
-! procedures of opf_path, for a fixed time step.
+! procedures of opf_path.
 !
 ! usage: opf_main N K M Re alpha dt nsteps state_in state_out [series]
+!   dt < 0 selects the adaptive time step (d_timestep < 0, main.f90:24-29)
+!   with |dt| as the initial tim_dt.
 !   state_in / state_out: raw stream files holding vel_ur, vel_ut, vel_uz as
 !   Re(1:N,0:H-1) then Im(1:N,0:H-1) per component.  series: optional text
-!   file with one line per step: step, correctors used, last dterr.
+!   file with one line per step: step, correctors used, last dterr, tim_dt.
 !*************************************************************************
 program opf_main
    use opf_path
@@ -28,7 +30,8 @@ program opf_main
    want_series = command_argument_count() >= 10
    if (want_series) call get_command_argument(10, fser)
 
-   call vel_precompute(N, K, M, 1, Re, alpha, dt)
+   d_timestep = dt
+   call vel_precompute(N, K, M, 1, Re, alpha, dabs(dt))
    call vel_matrices()
    open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
    read(u) vel_ur%Re, vel_ur%Im
@@ -43,6 +46,11 @@ program opf_main
       call vel_imposesym()
       call vel_transform()
       call vel_nonlinear()
+      if (d_timestep < 0d0) then
+         call vel_maxtstep()
+         call tim_new_tstep()
+         if (tim_new_dt) call vel_matrices()
+      end if
       call vel_predictor()
       tim_it = 1
       its = 0
@@ -54,9 +62,9 @@ program opf_main
          if (want_series) dterr_dev = tim_dterr
          call tim_check_cgce()
       end do
+      if (want_series) write(u, '(I8,I4,2ES26.17E3)') tim_step + 1, its, dterr_dev, tim_dt
       tim_t = tim_t + tim_dt
       tim_step = tim_step + 1
-      if (want_series) write(u, '(I8,I4,ES26.17E3)') tim_step, its, dterr_dev
    end do
    if (want_series) close(u)
 

@@ -28,6 +28,8 @@ module opf_path
    double precision :: d_courant = 0.5d0
    integer :: i_maxtstep = -1
    logical :: b_const_flux = .true.
+   double precision :: d_timestep = 0.01d0   ! < 0: adaptive (main.f90:24-29)
+   double precision :: d_maxdt = 1d99
 
    ! the coll type of variables.f90:37-40 with run-time extents
    type coll
@@ -38,8 +40,9 @@ module opf_path
    double precision :: vel_Pr0
 
    ! timestep.f90:12-24
-   double precision :: tim_t, tim_dt, tim_dterr, tim_corr_dt
-   integer :: tim_it, tim_step, tim_pcn
+   double precision :: tim_t, tim_dt, tim_dterr, tim_corr_dt, tim_cfl_dt
+   integer :: tim_it, tim_step, tim_pcn, tim_cfl_dir
+   logical :: tim_new_dt
 
    type(c_ptr), private :: h = c_null_ptr
    integer(c_int), parameter, private :: slot = 0
@@ -79,6 +82,7 @@ contains
       d_Re = Re; d_alpha = alpha
       tim_dt = dt
       tim_t = 0d0; tim_step = 0; tim_it = 0; tim_dterr = 0d0; tim_corr_dt = 0d0; tim_pcn = -1
+      tim_cfl_dt = 0d0; tim_cfl_dir = 0; tim_new_dt = .false.
       call coll_alloc(vel_ur); call coll_alloc(vel_ut); call coll_alloc(vel_uz)
    end subroutine vel_precompute
 
@@ -126,6 +130,33 @@ contains
       call opf_check(opf_vel_corrector(h, slot, e), 'vel_corrector')
       tim_dterr = max(tim_dterr, e)
    end subroutine vel_corrector
+
+   ! vel_maxtstep (velocity.f90:438-481): the CFL limit of the physical
+   ! fields of the last vel_transform, computed on the device
+   subroutine vel_maxtstep()
+      real(c_double) :: cfl
+      integer(c_int) :: dir
+      call opf_check(opf_vel_maxtstep(h, slot, cfl, dir), 'vel_maxtstep')
+      tim_cfl_dt = cfl
+      tim_cfl_dir = dir
+   end subroutine vel_maxtstep
+
+   ! tim_new_tstep (timestep.f90:259-278): the new tim_dt and tim_new_dt
+   subroutine tim_new_tstep()
+      double precision :: dt
+      integer, save :: i = 0
+      dt = min(tim_dt * 1.11d0, d_maxdt)
+      if (tim_step == 0 .and. tim_corr_dt == 0d0) dt = min(dt, tim_cfl_dt * 0.1d0)
+      if (tim_cfl_dt > 0d0) dt = min(dt, tim_cfl_dt * d_courant)
+      if (tim_corr_dt > 0d0) dt = min(dt, tim_corr_dt * 0.95d0)
+      i = i - 1
+      tim_new_dt = (dt < tim_dt * 0.95d0 .or. (dt > tim_dt * 1.10d0 .and. i < 0) &
+                    .or. (dt == d_maxdt .and. tim_dt < dt .and. i < 0))
+      if (tim_new_dt) then
+         if (dt >= tim_dt) i = 99
+         tim_dt = dt
+      end if
+   end subroutine tim_new_tstep
 
    ! tim_check_cgce (timestep.f90:214-253): the corrector iteration control
    subroutine tim_check_cgce()

@@ -76,3 +76,30 @@ def test_fortran_host_matches_reference(name, nsteps, key):
     # correctors per step equal the reference's (series column 8: ncorr)
     ref_ncorr = g["series"][:nsteps, 8].astype(int)
     assert np.array_equal(ser[:, 1].astype(int), ref_ncorr)
+
+
+@pytest.mark.gpu
+def test_fortran_host_adaptive_matches_reference():
+    """d_timestep < 0 (main.f90:24-29) in the Fortran host: vel_maxtstep on
+    the device, the host's tim_new_tstep and vel_matrices on a change; the
+    reference's tim_dt of every step and its state after 120 steps."""
+    if not os.access(EXE, os.X_OK):
+        pytest.skip("Fortran host not built")
+    g = load_golden("adapt_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    H = (2 * K - 1) * M - (K - 1)
+    n = int(g["nsteps"])
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, fser = (os.path.join(td, x) for x in ("in.bin", "out.bin", "series.txt"))
+        _write_state(fin, g["ic"])
+        args = [EXE, str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])),
+                repr(float(g["dt"])), str(n), fin, fout, fser]
+        print("fortran host: adaptive", n, flush=True)
+        subprocess.run(args, check=True, timeout=120)
+        u = _read_state(fout, N, H)
+        ser = np.loadtxt(fser, ndmin=2)
+    ref_dt = g["dtser"][:, 1]
+    assert np.max(np.abs(ser[:, 3] - ref_dt) / ref_dt) <= 1e-12
+    assert np.array_equal(ser[:, 1].astype(int), g["series"][:, 8].astype(int))
+    for c in range(3):
+        assert rel_err(u[c], g[f"state_{n}"][c]) <= 1e-10, c
