@@ -10,9 +10,11 @@
 # directory outside the repository; the copy, the preprocessed sources and the
 # objects are deleted afterwards.  Only the executable lands in oracle/_ref/.
 #
-# usage: oracle/build_ref.sh NAME N K M [Mp]
+# usage: oracle/build_ref.sh NAME N K M [Mp] [SYMM]
+#   SYMM: one of mirrorsym, shiftrefl, shiftrott sets that b_* flag of
+#   parameters.f90:26-28 (vel_imposesym then projects every step)
 set -euo pipefail
-NAME=$1; N=$2; K=$3; M=$4; MP=${5:-1}
+NAME=$1; N=$2; K=$3; M=$4; MP=${5:-1}; SYMM=${6:-}
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
@@ -29,6 +31,10 @@ sed -e "s/\(i_N *= *\)[0-9][0-9]*/\1$N/" \
     -e "s/\(i_M *= *\)[0-9][0-9]*/\1$M/" \
     -e "s/\(i_Mp *= *\)[0-9][0-9]*/\1$MP/" \
     "$REF/parameters.f90" > "$TMP/program/parameters.f90"
+if [ -n "$SYMM" ]; then
+  sed -i -e "s/\(b_$SYMM *= *\)\.false\./\1.true./" "$TMP/program/parameters.f90"
+  grep -E "b_$SYMM +=" "$TMP/program/parameters.f90" >&2
+fi
 grep -E "i_(N|K|M|Mp) +=" "$TMP/program/parameters.f90" | head -4 >&2
 cd "$TMP/program"
 for f in mpi meshs variables transform.fftw3 timestep velocity; do

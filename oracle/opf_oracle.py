@@ -803,8 +803,45 @@ class Ctx:
         self.ur, self.ut, self.uz = ur.copy(), ut.copy(), uz.copy()
 
     def imposesym(self):
-        """vel_imposesym (velocity.f90:61-74) with only mode 0 enabled."""
+        """vel_imposesym (velocity.f90:61-74): mode 0, then the symmetries
+        in self.symm (1 mirror, 2 shift-reflect, 3 shift-rotate; the
+        b_mirrorsym, b_shiftrefl, b_shiftrott flags of parameters.f90:26-28)."""
         self.imposesym0(self.ur, self.ut, self.uz)
+        for i in sorted(getattr(self, "symm", ())):
+            self.ur, self.ut, self.uz = self.imposesym_mode(i, self.ur, self.ut, self.uz)
+
+    def imposesym_mode(self, i, r, t, z):
+        """var_imposesym(i) (variables.f90:538-618) for i = 1, 2, 3; returns
+        new fields.  1, 2: average with the image of the reflection, which
+        maps harmonic (k, m) to (-k, m) (to itself for m = 0) with factors
+        a = 1 (mirror) or (-1)^k (shift-reflect), b = -1 on the imaginary
+        part for m > 0, and -1 on u_theta; 3: zero the harmonics with k + m
+        odd.  Mode 0 (harmonic 0) is applied last."""
+        hk, hm = self.harm.k, self.harm.m
+        r, t, z = r.copy(), t.copy(), z.copy()
+        if i in (1, 2):
+            where = {(int(k), int(m)): h for h, (k, m) in enumerate(zip(hk, hm))}
+            img = np.array([h if m == 0 else where[(-int(k), int(m))] for h, (k, m) in enumerate(zip(hk, hm))])
+            a = np.ones(hk.size)
+            if i == 2:
+                a[(hk % 2) != 0] = -1.0
+            b = np.where(hm == 0, 1.0, -1.0)
+
+            def image(f, sign):
+                g = f[:, img]
+                return (g.real * (sign * a)) + 1j * (g.imag * ((sign * a) * b))
+
+            c1, c2, c3 = image(r, 1.0), image(t, -1.0), image(z, 1.0)
+            r = ((r.real + c1.real) / 2.0) + 1j * ((r.imag + c1.imag) / 2.0)
+            t = ((t.real + c2.real) / 2.0) + 1j * ((t.imag + c2.imag) / 2.0)
+            z = ((z.real + c3.real) / 2.0) + 1j * ((z.imag + c3.imag) / 2.0)
+        if i == 3:
+            odd = ((hk + hm) % 2) != 0
+            r[:, odd] = 0.0
+            t[:, odd] = 0.0
+            z[:, odd] = 0.0
+        self.imposesym0(r, t, z)
+        return r, t, z
 
     def transform(self):
         """vel_transform (velocity.f90:304-317)."""

@@ -25,6 +25,8 @@
 !                                  and after each corrector)
 !   out_prefix_state_XXXXXX.bin    state after the listed steps
 !   out_prefix_timing.txt          wall seconds for the time loop
+!   nsteps < 0: no time loop; out_prefix_sym_I.bin holds var_imposesym(I)
+!   (variables.f90:538-618) of the input state for I = 0..3.
 !*************************************************************************
  program opf_ref_driver
    use velocity
@@ -64,6 +66,10 @@
    adaptive = dt_in < 0d0
    tim_dt = dabs(dt_in)
    call read_state(trim(infile))
+   if(nsteps < 0) then
+      call dump_symmetries()
+      stop
+   end if
    call vel_matrices()
 
    open(useries, file=trim(prefix)//'_series.txt', status='replace')
@@ -168,6 +174,20 @@
       write(cnum,'(I6.6)') step
       call write_coll3(trim(prefix)//'_state_'//cnum//'.bin', vel_ur, vel_ut, vel_uz)
    end subroutine write_state
+
+   subroutine dump_symmetries()
+      type (coll) :: a, b, c
+      character(1) :: ci
+      integer :: isym
+      do isym = 0, 3
+         call var_coll_copy(vel_ur, a)
+         call var_coll_copy(vel_ut, b)
+         call var_coll_copy(vel_uz, c)
+         call var_imposesym(isym, a, b, c)
+         write(ci,'(I1)') isym
+         call write_coll3(trim(prefix)//'_sym_'//ci//'.bin', a, b, c)
+      end do
+   end subroutine dump_symmetries
 
    subroutine dump_stage(tag, a, b, c)
       character(*), intent(in) :: tag

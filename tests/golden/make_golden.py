@@ -37,9 +37,14 @@ WORKLOADS = {
     # adaptive time step (dt < 0: |dt| is the initial tim_dt, main.f90:24-29)
     "adapt_mini": (16, 6, 4, 2400.0, 1.25, -0.01, 1e-3, 1, 120, (1, 120), True),
     "adapt_c0": (32, 24, 12, 2400.0, 1.25, -0.01, 1e-3, 1, 200, (200,), True),
+    # var_imposesym(0..3) of the initial state (nsteps < 0: no time loop)
+    "sym_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, -1, (), True),
+    # shift-reflect symmetry imposed every step (b_shiftrefl, parameters.f90:27)
+    "sr_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 20, (1, 20), True),
 }
-# executable of each workload (the reference is compiled per resolution)
-EXE = {"adapt_mini": "mini", "adapt_c0": "c0"}
+# executable of each workload (the reference is compiled per resolution and
+# symmetry flag: oracle/build_ref.sh mini_sr 16 6 4 1 shiftrefl)
+EXE = {"adapt_mini": "mini", "adapt_c0": "c0", "sym_mini": "mini", "sr_mini": "mini_sr"}
 SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
 
 
@@ -67,6 +72,14 @@ def make(name):
         subprocess.run([exe, icf, pre, repr(Re), repr(alpha), repr(dt), str(nsteps)]
                        + [str(s) for s in saves], check=True, cwd=td, env=env,
                        stdout=subprocess.DEVNULL, preexec_fn=_unlimited_stack)
+        if nsteps < 0:
+            out["ic"] = np.stack([ur, ut, uz])
+            for i in range(4):
+                out[f"sym_{i}"] = np.stack(O.read_coll3(f"{pre}_sym_{i}.bin", N, H))
+            path = os.path.join(HERE, f"golden_{name}.npz")
+            np.savez_compressed(path, **out)
+            print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+            return
         ser = np.loadtxt(pre + "_series.txt", ndmin=2)
         out["series"] = ser
         if dt < 0:   # step tim_dt tim_cfl_dt tim_cfl_dir tim_corr_dt tim_new_dt
@@ -94,5 +107,5 @@ def make(name):
 
 
 if __name__ == "__main__":
-    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0"]):
+    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini"]):
         make(nm)

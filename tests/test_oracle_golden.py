@@ -142,3 +142,34 @@ def test_adaptive_timestep_mini():
     assert np.max(np.abs(E - g["series"][:, 2]) / g["series"][:, 2]) <= TOL_DIAG
     for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
         assert rel_err(u, g["state_120"][c]) <= 1e-10, c
+
+
+def test_imposesym_modes_match_reference():
+    """var_imposesym(1..3) (variables.f90:538-618) of the oracle against the
+    reference's own output on the same state, bit for bit; each mode is an
+    idempotent projection (its reflection is exact, so P(P u) == P u)."""
+    g = load_golden("sym_mini")
+    ctx = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]))
+    ic = g["ic"]
+    for i in (1, 2, 3):
+        got = ctx.imposesym_mode(i, ic[0], ic[1], ic[2])
+        for c in range(3):
+            assert np.array_equal(got[c], g[f"sym_{i}"][c]), (i, c)
+        again = ctx.imposesym_mode(i, *got)
+        for c in range(3):
+            assert np.array_equal(again[c], got[c]), (i, c)
+
+
+def test_shift_reflect_run_mini():
+    """b_shiftrefl (parameters.f90:27): vel_imposesym projects every step;
+    the reference's 20-step run compiled with the flag set."""
+    g = load_golden("sr_mini")
+    ctx = _ctx(g)
+    ctx.symm = (2,)
+    ic = g["ic"]
+    ctx.set_state(ic[0], ic[1], ic[2])
+    ser = ctx.run(int(g["nsteps"]))
+    E = np.array([s["E"] for s in ser])
+    assert np.max(np.abs(E - g["series"][:, 2]) / g["series"][:, 2]) <= TOL_DIAG
+    for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
+        assert rel_err(u, g["state_20"][c]) <= 1e-10, c
