@@ -93,6 +93,15 @@ int opf_slot_setup(opf_ctx* ctx, int slot, double Re, double alpha, double dt, d
  * the reference); courant also sets the corrector's fallback
  * tim_corr_dt = tim_dt / (10 courant) (timestep.f90:239). */
 int opf_slot_adaptive(opf_ctx* ctx, int slot, int enable, double maxdt, double courant);
+/* ---- symmetric subspaces ----------------------------------------------------- */
+
+/* The symmetries vel_imposesym imposes at the start of every step
+ * (velocity.f90:61-74): b_mirrorsym, b_shiftrefl, b_shiftrott of
+ * parameters.f90:26-28, i.e. var_imposesym modes 1, 2, 3
+ * (variables.f90:538-618), applied in that order after mode 0, in opf_run
+ * and in opf_vel_imposesym. */
+int opf_slot_symmetry(opf_ctx* ctx, int slot, int mirror, int shiftrefl, int shiftrott);
+
 /* tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir of the slot (the columns of
  * io_write_timestep, io.f90:645-657). */
 int opf_slot_timestep(opf_ctx* ctx, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir);

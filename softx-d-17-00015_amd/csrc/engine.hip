@@ -4,9 +4,10 @@
 // device arrays of `capacity` slots.  The time loop of main.f90:18-46 runs as
 // a sequence of "iterations", each one nonlinear evaluation plus one
 // predictor or corrector (per slot, as the slot's control block says):
-//   k_zfwd -> k_theta -> k_zbwd -> k_step -> k_curl_next -> k_control
-// The corrector test (tim_check_cgce) runs in k_control on the device, so the
-// host only enqueues iterations and checks progress between chunks.
+//   k_zfwd -> k_theta -> k_zbwd -> k_step -> k_control
+// (k_step ends with the curl for the next evaluation).  The corrector test
+// (tim_check_cgce) runs in k_control on the device, so the host only enqueues
+// iterations and checks progress between chunks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,6 +91,8 @@ struct opf_ctx {
   struct Adapt { int on = 0; double maxdt = 1e99, courant = 0.5; };
   std::vector<Adapt> adapt;    // d_timestep < 0, d_maxdt, d_courant per slot
   int n_adaptive = 0;          // slots with the adaptive time step on
+  std::vector<int> symm;       // symmetry bits per slot (SlotParams::symm)
+  int n_symm = 0;              // slots with a symmetry imposed
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
@@ -256,6 +259,10 @@ void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
 // followed by vel_maxtstep + tim_new_tstep and, where dt changed,
 // vel_matrices (main.f90:24-29), all decided on the device.
 void launch_iteration(opf_ctx* c, int s0, int ns) {
+  if (c->n_symm > 0) {   // vel_imposesym at the start of a step (main.f90:21), then the new curl
+    launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, s0, 1);
+    launch(c, "curl", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, 1);
+  }
   launch_eval(c, s0, ns);
   if (c->n_adaptive > 0) {
     launch(c, "newdt", k_newdt, dim3(1, 1, ns), dim3(64), 0, c->G, c->B, s0, 0);
@@ -390,6 +397,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->occupied.assign(cap, 0);
   c->setup.assign(cap, 0);
   c->adapt.assign(cap, opf_ctx::Adapt{});
+  c->symm.assign(cap, 0);
 }
 
 void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
@@ -402,6 +410,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   p.const_flux = cf;
   p.adaptive = c->adapt[slot].on;        // d_timestep < 0 (parameters.f90:38)
   p.maxdt = c->adapt[slot].maxdt;        // d_maxdt (parameters.f90:39)
+  p.symm = c->symm[slot];                // b_mirrorsym, b_shiftrefl, b_shiftrott
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
   auto ctl = read_ctl(c);
   ctl[slot].status = 0;
@@ -441,7 +450,7 @@ void put_state(opf_ctx* c, int slot, const double* const* f, double t) {
   k.refactor = 0;
   write_ctl(c, ctl);
   launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
-  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot);
+  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
   ctl = read_ctl(c);
   ctl[slot].active = 0;
@@ -665,6 +674,21 @@ int opf_slot_adaptive(opf_ctx* c, int slot, int enable, double maxdt, double cou
   });
 }
 
+int opf_slot_symmetry(opf_ctx* c, int slot, int mirror, int shiftrefl, int shiftrott) {
+  return guard([&] {
+    check_slot(c, slot);
+    const int bits = (mirror ? 1 : 0) | (shiftrefl ? 2 : 0) | (shiftrott ? 4 : 0);
+    c->n_symm += (bits != 0) - (c->symm[slot] != 0);
+    c->symm[slot] = bits;
+    if (c->setup[slot]) {
+      SlotParams p{};
+      HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+      p.symm = bits;
+      HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
+    }
+  });
+}
+
 int opf_slot_timestep(opf_ctx* c, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir) {
   return guard([&] {
     check_slot(c, slot);
@@ -780,7 +804,12 @@ int opf_diag_get(opf_ctx* c, int slot, double* recs, int maxrec, int* nrec) {
 int opf_vel_imposesym(opf_ctx* c, int slot) {
   return guard([&] {
     check_slot(c, slot);
-    launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    with_active(c, slot, -1, [&] {
+      launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, 1), dim3(256), 0, c->G, c->B, slot,
+             0);
+      launch(c, "curl", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+    });
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

@@ -91,7 +91,8 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_influence(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_ainv(Geo G, Bufs B, int slot0, int gated);
-__global__ void k_curl(Geo G, Bufs B, int slot0);
+__global__ void k_curl(Geo G, Bufs B, int slot0, int gated);
+__global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_newdt(Geo G, Bufs B, int slot0, int what);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);

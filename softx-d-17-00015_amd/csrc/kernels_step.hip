@@ -488,16 +488,72 @@ __device__ void curl_lane(const Geo& G, int h, int k, int m, double alpha,
 }
 
 // which = 0: curl of U[ucur]; used after put_state and after every step.
-__global__ void k_curl(Geo G, Bufs B, int slot0) {
+__global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = slot0 + blockIdx.z;
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active || h >= G.H) return;
+  if (gated && (c->mode != MODE_PRED || B.prm[slot].symm == 0)) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
   const double2* u = B.U[c->ucur] + (size_t)slot * 3 * G.coll;
   curl_lane(G, h, k, m, B.prm[slot].alpha, u, u + G.coll, u + 2 * G.coll,
             B.om + (size_t)slot * 3 * G.coll);
+}
+
+// vel_imposesym (velocity.f90:61-74) with the slot's symmetries on U[ucur]:
+// var_imposesym(0), then (1) mirror, (2) shift & reflect, (3) shift &
+// rotate as enabled (variables.f90:538-618; each ends with the harmonic-0
+// conditions of mode 0).  Modes 1 and 2 average a harmonic (k, m) with the
+// image of its partner (-k, m) (itself for m = 0): factor a = 1, or (-1)^k
+// for shift & reflect; the imaginary part flips for m > 0; u_theta flips.
+// Both act within a pair, mode 3 within a harmonic and every mode within a
+// component, so one thread takes (n, pair, component) and applies the
+// enabled modes in order in registers.  gated: only slots about to run a
+// predictor (the start of a step, main.f90:21) with a symmetry enabled.
+__global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated) {
+  const int slot = slot0 + blockIdx.z;
+  const SlotCtl* c = &B.ctl[slot];
+  const int symm = B.prm[slot].symm;
+  if (!c->active || (gated && (c->mode != MODE_PRED || symm == 0))) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // (comp, n, h)
+  const int h = idx % G.H, rest = idx / G.H;
+  const int n = rest % G.N, comp = rest / G.N;
+  if (comp >= 3) return;
+  int k, m;
+  harmonic_km(h, G.K, k, m);
+  if (m > 0 && k < 0) return;                 // taken by the partner (k > 0)
+  const int hp = m == 0 ? h : h - 2 * k;      // harmonic (-k, m)
+  double2* u = B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)comp * G.coll + (size_t)n * G.Hp;
+  double2 x = u[h], y = u[hp];
+  const double b = m == 0 ? 1.0 : -1.0;
+  const double sg = comp == 1 ? -1.0 : 1.0;   // u_theta changes sign
+  auto fix0 = [&]() {                         // mode 0 on harmonic 0
+    if (h == 0) {
+      if (comp == 0) x = make_double2(0.0, 0.0);
+      else x.y = 0.0;
+    }
+  };
+  fix0();
+  for (int i = 1; i <= 3; ++i) {
+    if (!(symm & (1 << (i - 1)))) continue;
+    if (i <= 2) {
+      const double a = (i == 2 && (k & 1)) ? -1.0 : 1.0;
+      const double f = sg * a;
+      // image of the partner (of x for the partner's own update)
+      const double2 ix = make_double2(y.x * f, y.y * (f * b));
+      const double2 iy = make_double2(x.x * f, x.y * (f * b));
+      x = make_double2((x.x + ix.x) / 2.0, (x.y + ix.y) / 2.0);
+      if (hp != h) y = make_double2((y.x + iy.x) / 2.0, (y.y + iy.y) / 2.0);
+      else y = x;
+    } else if ((k + m) & 1) {
+      x = make_double2(0.0, 0.0);
+      y = x;
+    }
+    fix0();
+  }
+  u[h] = x;
+  if (hp != h) u[hp] = y;
 }
 
 // var_imposesym(0) (variables.f90:611-616) on harmonic 0 of U[ucur].

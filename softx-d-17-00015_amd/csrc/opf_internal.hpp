@@ -45,6 +45,8 @@ struct SlotParams {
   int const_flux;
   int adaptive;   // d_timestep < 0: vel_maxtstep + tim_new_tstep before every predictor
   double maxdt;   // d_maxdt (parameters.f90:39)
+  int symm;       // bits 0, 1, 2: b_mirrorsym, b_shiftrefl, b_shiftrott (parameters.f90:26-28)
+  int pad;
 };
 
 // Slot modes for the next step kernel.

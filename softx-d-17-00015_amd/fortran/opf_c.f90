@@ -197,6 +197,15 @@ module opf_c
          integer(c_int) :: rc
       end function
 
+      ! b_mirrorsym, b_shiftrefl, b_shiftrott (parameters.f90:26-28)
+      function opf_slot_symmetry(ctx, slot, mirror, shiftrefl, shiftrott) bind(C, name="opf_slot_symmetry") &
+            result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, mirror, shiftrefl, shiftrott
+         integer(c_int) :: rc
+      end function
+
       ! tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)
       function opf_slot_timestep(ctx, slot, dt, corr_dt, cfl_dt, cfl_dir) bind(C, name="opf_slot_timestep") &
             result(rc)

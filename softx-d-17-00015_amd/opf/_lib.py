@@ -49,6 +49,7 @@ PROTOTYPES = {
     "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
     "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
     "opf_slot_adaptive": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "opf_slot_symmetry": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "opf_slot_timestep": (C.c_int, [vp, C.c_int, dp, dp, dp, ip]),
     "opf_vel_maxtstep": (C.c_int, [vp, C.c_int, dp, ip]),
     "opf_vel_matrices": (C.c_int, [vp, C.c_int, C.c_double]),

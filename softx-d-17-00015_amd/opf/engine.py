@@ -225,6 +225,12 @@ class Context:
         vel_matrices on a change) before every predictor of run()."""
         check(self.lib.opf_slot_adaptive(self.h, slot, int(bool(enable)), float(maxdt), float(courant)))
 
+    def symmetry(self, slot, mirror=False, shiftrefl=False, shiftrott=False):
+        """b_mirrorsym, b_shiftrefl, b_shiftrott: the symmetries vel_imposesym
+        imposes at the start of every step (var_imposesym modes 1, 2, 3)."""
+        check(self.lib.opf_slot_symmetry(self.h, slot, int(bool(mirror)), int(bool(shiftrefl)),
+                                         int(bool(shiftrott))))
+
     def timestep(self, slot):
         """tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)."""
         dt, cd, cf, di = C.c_double(), C.c_double(), C.c_double(), C.c_int()
