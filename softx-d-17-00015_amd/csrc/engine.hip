@@ -156,7 +156,7 @@ FftTiles choose_tiles(int Z, int T, int M) {
   // a workgroup transforms at most FFT_TILE complex values: tm columns of
   // length Z, or 3*tz packed lines of length T
   if (Z > FFT_TILE || 3 * T > FFT_TILE) throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
-  t.tm = std::max(1, std::min(M, FFT_TILE / Z));
+  t.tm = std::max(1, std::min(M, tile_m(Z)));
   t.tz = std::max(1, std::min(Z, FFT_TILE / (3 * T)));
   return t;
 }

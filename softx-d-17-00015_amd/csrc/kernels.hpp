@@ -85,6 +85,11 @@ constexpr int FFT_TILE = 2048;   // complex values per transform workgroup (all 
 
 // LDS stride of one transform of length L in the transform kernels: one pad
 // slot per 16 elements, rounded up to an odd count.
+// m columns per workgroup of the z transforms for length Z (as many as fit
+// in FFT_TILE values; rounding down to whole 128-byte lines of the
+// intermediates measured slower)
+__host__ __device__ constexpr int tile_m(int Z) { return FFT_TILE / Z; }
+
 __host__ __device__ inline int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
 
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated);

@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
   constexpr int LZ = plan_len<R0, R1, R2>();
-  constexpr int TMC = LZ ? FFT_TILE / LZ : 0;
+  constexpr int TMC = LZ ? tile_m(LZ) : 0;
   const int n = blockIdx.y / 6, f = blockIdx.y % 6;
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles 
   double2* buf = smem;
   double2* twl = smem + (size_t)T.tm * Zp;
   load_tw(twl, B.twZ, Z);
-  load_ycols<LZ ? FFT_TILE / LZ : 0>(buf, Y, (size_t)3 * M, Z, Zp, m0, tm);
+  load_ycols<LZ ? tile_m(LZ) : 0>(buf, Y, (size_t)3 * M, Z, Zp, m0, tm);
   __syncthreads();
   run_fft<R0, R1, R2>(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = B.Nraw[c->ncur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll + (size_t)n * G.Hp;

@@ -575,14 +575,99 @@ __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
 // the reference's order (var_coll_norm, variables.f90:727-749).
 // One workgroup per slot.
 // ---------------------------------------------------------------------------
+// The decisions of the control step on a register copy of the slot's
+// control block (recs: the slot's diagnostics records).
+__device__ __forceinline__ void control_decide(SlotCtl& k, const SlotParams& prm, DiagRec* recs, bool diag_now,
+                                               double E, double Ek0, double Em0) {
+  if (diag_now) {
+    DiagRec* rec = recs + (k.ndiag % k.diag_cap);
+    rec->E = E;
+    rec->Ek0 = Ek0;
+    rec->Em0 = Em0;
+    rec->ncorr = -1.0;
+    rec->dterr = 0.0;
+  }
+  if (k.mode == MODE_PRED) {
+    // vel_predictor done: u_ := old current, N_ := this evaluation's raw N
+    const int old_base = k.ubase;
+    k.ubase = k.ucur;
+    k.ucur = k.unext;
+    k.unext = old_base;
+    k.npred = k.ncur;
+    k.ncur = 1 - k.ncur;
+    k.it = 1;
+    k.ncorr = 0;
+    k.lasterr = 1e99;
+    k.mode = MODE_CORR;
+    if (diag_now) k.ndiag = k.ndiag + 1;
+    return;
+  }
+  // corrector done
+  const double dterr = sqrt(__longlong_as_double((long long)k.dterr_bits));
+  k.dterr_bits = 0ull;
+  k.ncorr = k.ncorr + 1;
+  const int old_cur = k.ucur;
+  k.ucur = k.unext;
+  k.unext = old_cur;
+  k.last_dterr = dterr;
+  if (k.host_driven) return;   // the caller applies tim_check_cgce itself
+  int it = k.it;
+  if (it == 1) {
+    k.corr_dt = prm.dt * sqrt(prm.dterr_tol / dterr);
+    k.lasterr = 1e99;
+  }
+  int fail = OPF_FAIL_NONE;
+  if (!(dterr == dterr)) {
+    fail = OPF_FAIL_NAN;
+    it = 0;
+  } else if (prm.dt < 1e-9 && k.step > 30) {
+    fail = OPF_FAIL_DT0;
+    it = 0;
+  } else if (it > 10) {
+    fail = OPF_FAIL_ITS;
+    it = 0;
+  } else if (dterr > k.lasterr) {
+    if (dterr > 2.0 * prm.dterr_tol) fail = OPF_FAIL_INCERR;
+    else k.corr_dt = prm.dt / (10.0 * prm.courant);
+    it = 0;
+  } else if (dterr > prm.dterr_tol) {
+    k.lasterr = dterr;
+    it = it + 1;
+  } else {
+    it = 0;
+  }
+  if (k.corr_fixed > 0) it = k.ncorr >= k.corr_fixed ? 0 : (it == 0 ? 1 : it);
+  if (fail != OPF_FAIL_NONE && k.corr_fixed <= 0) {
+    k.status = fail;
+    k.active = 0;
+    return;
+  }
+  k.it = it;
+  if (it == 0) {
+    // step complete: tim_t += dt, tim_step += 1 (main.f90:43-44)
+    if (k.diag_every > 0 && k.ndiag > 0) {
+      DiagRec* rec = recs + ((k.ndiag - 1) % k.diag_cap);
+      if ((int)rec->step == k.step) {
+        rec->ncorr = k.ncorr;
+        rec->dterr = dterr;
+      }
+    }
+    k.last_ncorr = k.ncorr;
+    k.t = k.t + prm.dt;
+    k.step = k.step + 1;
+    k.mode = MODE_PRED;
+    if (k.target_steps >= 0 && k.step >= k.target_steps) k.active = 0;
+  }
+}
+
 __global__ void k_control(Geo G, Bufs B, int slot0) {
   const int slot = slot0 + blockIdx.z;
-  SlotCtl* c = &B.ctl[slot];
-  if (!c->active) return;
+  const SlotCtl k0 = B.ctl[slot];   // one batch of loads
+  if (!k0.active) return;
   const SlotParams prm = B.prm[slot];
   __shared__ double Em[3][256];
   __shared__ double Ek0s[3];
-  const bool diag_now = c->mode == MODE_PRED && c->diag_every > 0 && (c->step % c->diag_every) == 0;
+  const bool diag_now = k0.mode == MODE_PRED && k0.diag_every > 0 && (k0.step % k0.diag_every) == 0;
   if (diag_now) {
     const double* db = B.diagb + (size_t)slot * 3 * G.Hp;
     // Em(m) accumulates the block of m in harmonic order
@@ -605,8 +690,8 @@ __global__ void k_control(Geo G, Bufs B, int slot0) {
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
+  double E = 0.0, Ek0 = 0.0, Em0 = 0.0;
   if (diag_now) {
-    double E = 0.0, Ek0 = 0.0, Em0 = 0.0;
     for (int comp = 0; comp < 3; ++comp) {
       double e = 0.0;
       for (int m = 0; m < G.M; ++m) e = e + Em[comp][m];
@@ -614,84 +699,10 @@ __global__ void k_control(Geo G, Bufs B, int slot0) {
       Ek0 = Ek0 + Ek0s[comp];
       Em0 = Em0 + Em[comp][0];
     }
-    DiagRec* rec = B.diag + (size_t)slot * c->diag_cap + (c->ndiag % c->diag_cap);
-    rec->E = E;
-    rec->Ek0 = Ek0;
-    rec->Em0 = Em0;
-    rec->ncorr = -1.0;
-    rec->dterr = 0.0;
   }
-  if (c->mode == MODE_PRED) {
-    // vel_predictor done: u_ := old current, N_ := this evaluation's raw N
-    const int old_base = c->ubase;
-    c->ubase = c->ucur;
-    c->ucur = c->unext;
-    c->unext = old_base;
-    c->npred = c->ncur;
-    c->ncur = 1 - c->ncur;
-    c->it = 1;
-    c->ncorr = 0;
-    c->lasterr = 1e99;
-    c->mode = MODE_CORR;
-    if (diag_now) c->ndiag = c->ndiag + 1;
-    return;
-  }
-  // corrector done
-  const double dterr = sqrt(__longlong_as_double((long long)c->dterr_bits));
-  c->dterr_bits = 0ull;
-  c->ncorr = c->ncorr + 1;
-  const int old_cur = c->ucur;
-  c->ucur = c->unext;
-  c->unext = old_cur;
-  c->last_dterr = dterr;
-  if (c->host_driven) return;   // the caller applies tim_check_cgce itself
-  int it = c->it;
-  if (it == 1) {
-    c->corr_dt = prm.dt * sqrt(prm.dterr_tol / dterr);
-    c->lasterr = 1e99;
-  }
-  int fail = OPF_FAIL_NONE;
-  if (!(dterr == dterr)) {
-    fail = OPF_FAIL_NAN;
-    it = 0;
-  } else if (prm.dt < 1e-9 && c->step > 30) {
-    fail = OPF_FAIL_DT0;
-    it = 0;
-  } else if (it > 10) {
-    fail = OPF_FAIL_ITS;
-    it = 0;
-  } else if (dterr > c->lasterr) {
-    if (dterr > 2.0 * prm.dterr_tol) fail = OPF_FAIL_INCERR;
-    else c->corr_dt = prm.dt / (10.0 * prm.courant);
-    it = 0;
-  } else if (dterr > prm.dterr_tol) {
-    c->lasterr = dterr;
-    it = it + 1;
-  } else {
-    it = 0;
-  }
-  if (c->corr_fixed > 0) it = c->ncorr >= c->corr_fixed ? 0 : (it == 0 ? 1 : it);
-  if (fail != OPF_FAIL_NONE && c->corr_fixed <= 0) {
-    c->status = fail;
-    c->active = 0;
-    return;
-  }
-  c->it = it;
-  if (it == 0) {
-    // step complete: tim_t += dt, tim_step += 1 (main.f90:43-44)
-    if (c->diag_every > 0 && c->ndiag > 0) {
-      DiagRec* rec = B.diag + (size_t)slot * c->diag_cap + ((c->ndiag - 1) % c->diag_cap);
-      if ((int)rec->step == c->step) {
-        rec->ncorr = c->ncorr;
-        rec->dterr = dterr;
-      }
-    }
-    c->last_ncorr = c->ncorr;
-    c->t = c->t + prm.dt;
-    c->step = c->step + 1;
-    c->mode = MODE_PRED;
-    if (c->target_steps >= 0 && c->step >= c->target_steps) c->active = 0;
-  }
+  SlotCtl k = k0;
+  control_decide(k, prm, B.diag + (size_t)slot * k.diag_cap, diag_now, E, Ek0, Em0);
+  B.ctl[slot] = k;   // one batch of stores
 }
 
 // ---------------------------------------------------------------------------
