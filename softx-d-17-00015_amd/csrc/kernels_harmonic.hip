@@ -593,7 +593,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   const bool loader = tid >= 64;
   const size_t hs = G.Hp, csz = G.coll;
   const MeshOff mo = MeshOff::make(N);
-  const double* me = G.mesh;
+  const double* __restrict__ me = G.mesh;
   const double* dr1c = me + mo.dr1 + (KL + 1);
   const SlotParams prm = B.prm[slot];
   const bool corr = ctl->mode == MODE_CORR;
@@ -743,9 +743,13 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     const double r2i = me[mo.rinv2 + n];
 #pragma unroll
     for (int dd = -KL; dd <= KL; ++dd) {
-      const int j = n + dd;
-      if (j < 0 || j > N - 1) continue;
-      const double lap = me[mo.lapb + (KL - dd) * N + j];
+      // rows outside the matrix contribute an exact zero (weight 0 on a
+      // clamped row), so the loads need no branch
+      const int jr = n + dd;
+      const bool in = jr >= 0 && jr <= N - 1;
+      const int j = in ? jr : n;
+      const double lap_ = me[mo.lapb + (KL - dd) * N + j];
+      const double lap = in ? lap_ : 0.0;
       double ap = c2 * lap, am = c2 * lap, az = c2 * lap;
       if (dd == 0) {
         ap = (ap + c2 * diag_d(r2i, G.Mp, S.m, prm.alpha, S.k, 1)) + c1;
@@ -827,10 +831,12 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
       double2 d = make_double2(0.0, 0.0);
 #pragma unroll
       for (int dd = -KL; dd <= KL; ++dd) {
-        const int j = n + dd;
-        if (j < 0 || j > N - 1) continue;
+        const int jr = n + dd;
+        const bool in = jr >= 0 && jr <= N - 1;
+        const int j = in ? jr : n;
         const double2 a = Vat(L, 0, j, lh, N), bm = Vat(L, 1, j, lh, N);
-        const double wgt = me[mo.dr1b + (KL - dd) * N + j];
+        const double w_ = me[mo.dr1b + (KL - dd) * N + j];
+        const double wgt = in ? w_ : 0.0;
         d.x = d.x + wgt * (0.5 * (a.x + bm.x));
         d.y = d.y + wgt * (0.5 * (a.y + bm.y));
       }
@@ -862,9 +868,11 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     double2 gr = make_double2(0.0, 0.0);
 #pragma unroll
     for (int dd = -KL; dd <= KL; ++dd) {
-      const int j = n + dd;
-      if (j < 0 || j > N - 1) continue;
-      const double wgt = me[mo.dr1b + (KL - dd) * N + j];
+      const int jr = n + dd;
+      const bool in = jr >= 0 && jr <= N - 1;
+      const int j = in ? jr : n;
+      const double w_ = me[mo.dr1b + (KL - dd) * N + j];
+      const double wgt = in ? w_ : 0.0;
       const double2 pj = Vat(L, 3, j, lh, N);
       gr.x = gr.x + wgt * pj.x;
       gr.y = gr.y + wgt * pj.y;
@@ -997,9 +1005,11 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
 #pragma unroll
     for (int dd = -KL; dd <= KL; ++dd) {
-      const int j = n + dd;
-      if (j < 0 || j > N - 1) continue;
-      const double wgt = me[mo.dr1b + (KL - dd) * N + j];
+      const int jr = n + dd;
+      const bool in = jr >= 0 && jr <= N - 1;
+      const int j = in ? jr : n;
+      const double w_ = me[mo.dr1b + (KL - dd) * N + j];
+      const double wgt = in ? w_ : 0.0;
       const double2 zj = Vat(L, 2, j, lh, N), tj = Vat(L, 1, j, lh, N);
       d1.x = d1.x + wgt * zj.x;
       d1.y = d1.y + wgt * zj.y;
