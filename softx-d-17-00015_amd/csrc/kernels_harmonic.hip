@@ -598,12 +598,15 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   const SlotParams prm = B.prm[slot];
   const bool corr = ctl->mode == MODE_CORR;
   const size_t s3 = (size_t)slot * 3 * csz;
-  const double2* uc = B.U[ctl->ucur] + s3;                 // current iterate
-  const double2* ub = B.U[corr ? ctl->ubase : ctl->ucur] + s3;   // u_ (vel_predictor copies u -> u_)
-  double2* un = B.U[ctl->unext] + s3;
-  const double2* Nc = B.Nraw[ctl->ncur] + s3;
-  const double2* Np = B.Nraw[ctl->npred] + s3;
-  double2* om = B.om + s3;
+  // distinct buffers (the ring of velocity roles; uc and ub coincide only in
+  // the predictor, where both are read-only), so restrict-qualified: loads
+  // may move past the stores of un and om
+  const double2* __restrict__ uc = B.U[ctl->ucur] + s3;                 // current iterate
+  const double2* __restrict__ ub = B.U[corr ? ctl->ubase : ctl->ucur] + s3;   // u_ (vel_predictor copies u -> u_)
+  double2* __restrict__ un = B.U[ctl->unext] + s3;
+  const double2* __restrict__ Nc = B.Nraw[ctl->ncur] + s3;
+  const double2* __restrict__ Np = B.Nraw[ctl->npred] + s3;
+  double2* __restrict__ om = B.om + s3;
   const bool want_diag = !corr && ctl->diag_every > 0 && (ctl->step % ctl->diag_every) == 0;
 
   extern __shared__ double2 lds2[];
@@ -894,6 +897,15 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
                                        np, P, K, N);
 
   // ---- stage 6: influence-matrix correction (vel_adjPPE(2)) --------------
+  // the 4x4 inverses of the group's harmonics into LDS, one entry per thread
+  // (the factor ring is free after the solves)
+  double* AIl = (double*)L.FC;
+  for (int e = tid; e < 16 * L.GH; e += THR) {
+    const int lh = e >> 4, ij = e & 15;
+    const Slot S = slot_of(lh, GP, np, p0, K);
+    AIl[e] = (S.valid && !S.h00) ? B.Ainv[(size_t)slot * 16 * G.H + (size_t)ij * G.H + S.h] : 0.0;
+  }
+  __syncthreads();
   if (tid < L.GH) {
     const int lh = tid;
     const Slot S = slot_of(lh, GP, np, p0, K);
@@ -907,11 +919,11 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
       }
       double BRe[4], BIm[4];
       eval_bc5(wp, wm, wz, dr1c, me[mo.rinv + N - 1], (double)(S.m * G.Mp), prm.alpha * (double)S.k, BRe, BIm);
-      const double* Ai = B.Ainv + (size_t)slot * 16 * G.H + S.h;
+      const double* Ai = AIl + lh * 16;
       for (int i = 0; i < 4; ++i) {
         double sR = 0.0, sI = 0.0;
         for (int j = 0; j < 4; ++j) {
-          const double aij = Ai[(size_t)(i * 4 + j) * G.H];
+          const double aij = Ai[i * 4 + j];
           sR = sR + aij * BRe[j];
           sI = sI + aij * BIm[j];
         }
@@ -949,14 +961,18 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     Vat(L, 1, n, lh, N) = make_double2(0.5 * (up.y - um.y), -0.5 * (up.x - um.x));
     Vat(L, 2, n, lh, N) = uz;
   }
-  __syncthreads();
-  // vel_adjustFlux(1) (velocity.f90:170-175) on harmonic (0,0): n ascending
+  // vel_adjustFlux(1) (velocity.f90:170-175) on harmonic (0,0): n ascending,
+  // the quadrature weights staged in LDS first
   const bool has00 = p0 == 0;
+  double* QW = AIl + 16 * L.GH;
+  if (has00)
+    for (int n = tid; n < N; n += THR) QW[n] = me[mo.intrdr + n];
+  __syncthreads();
   if (has00 && tid == 0) {
     double d3 = 0.0, Pr0 = ctl->Pr0_hpf;
     if (prm.const_flux) {
       double s = 0.0;
-      for (int n = 0; n < N; ++n) s = s + Vat(L, 2, n, 0, N).x * me[mo.intrdr + n];
+      for (int n = 0; n < N; ++n) s = s + Vat(L, 2, n, 0, N).x * QW[n];
       const double d2 = 2.0 * s;
       d3 = -d2 / ctl->flux_d1;
       Pr0 = Pr0 + (d3 * prm.Re) / 4.0;
@@ -1024,9 +1040,15 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   }
   if (corr) {
     // max is order independent and the bit pattern of a non-negative double
-    // orders like its value: an integer atomic max is exact
+    // orders like its value: an integer atomic max is exact.  One atomic per
+    // workgroup (all of them target the slot's one word).
     for (int off = 32; off >= 1; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off));
-    if ((tid & 63) == 0) atomicMax(&ctl->dterr_bits, (unsigned long long)__double_as_longlong(dmax));
+    if ((tid & 63) == 0) L.RED[1 + (tid >> 6)] = dmax;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < THR / 64; ++w) dmax = fmax(dmax, L.RED[1 + w]);
+      atomicMax(&ctl->dterr_bits, (unsigned long long)__double_as_longlong(dmax));
+    }
   }
 }
 
