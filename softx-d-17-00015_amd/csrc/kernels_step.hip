@@ -572,7 +572,7 @@ __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
 // ---------------------------------------------------------------------------
 // Control: tim_check_cgce (timestep.f90:214-253) per slot, buffer roles,
 // time and step counters, and the reduction of the diagnostics partials in
-// the reference's order (var_coll_norm, variables.f90:727-749).
+// var_coll_norm's blocks (variables.f90:727-749).
 // One workgroup per slot.
 // ---------------------------------------------------------------------------
 // The decisions of the control step on a register copy of the slot's
@@ -666,37 +666,65 @@ __global__ void k_control(Geo G, Bufs B, int slot0) {
   if (!k0.active) return;
   const SlotParams prm = B.prm[slot];
   __shared__ double Em[3][256];
-  __shared__ double Ek0s[3];
+  __shared__ double Ekm[3][256];   // the k = 0 term of each m block (the Ek(0) sum)
+  __shared__ double chunk[4096];
   const bool diag_now = k0.mode == MODE_PRED && k0.diag_every > 0 && (k0.step % k0.diag_every) == 0;
   if (diag_now) {
-    const double* db = B.diagb + (size_t)slot * 3 * G.Hp;
-    // Em(m) accumulates the block of m in harmonic order
-    for (int idx = threadIdx.x; idx < 3 * G.M; idx += blockDim.x) {
-      const int comp = idx / G.M, m = idx % G.M;
-      const int h0 = m == 0 ? 0 : G.K + (m - 1) * (2 * G.K - 1);
-      const int h1 = m == 0 ? G.K : h0 + 2 * G.K - 1;
-      double s = 0.0;
-      for (int h = h0; h < h1; ++h) s = s + db[comp * G.Hp + h];
-      if (m < 256) Em[comp][m] = s;
-    }
-    if (threadIdx.x < 3) {
-      double s = 0.0;  // Ek(0): harmonics with k = 0 in order, m = 0..M-1
-      for (int m = 0; m < G.M; ++m) {
-        const int h = m == 0 ? 0 : G.K + (m - 1) * (2 * G.K - 1) + (G.K - 1);
-        s = s + db[threadIdx.x * G.Hp + h];
+    // Em(m) accumulates the block of m.  Whole m blocks are staged through
+    // LDS by all threads (coalesced), then one wave per block sums it as a
+    // tree (the reference sums in harmonic order; the two differ by
+    // rounding, about 1e-16 relative).
+    const int K = G.K, M = G.M, bl = 2 * K - 1;
+    auto hstart = [&](int m) { return m == 0 ? 0 : K + (m - 1) * bl; };
+    auto hend = [&](int m) { return m == 0 ? K : K + m * bl; };
+    for (int comp = 0; comp < 3; ++comp) {
+      const double* db = B.diagb + ((size_t)slot * 3 + comp) * G.Hp;
+      for (int m0 = 0; m0 < M;) {
+        int m1 = m0 + 1;
+        while (m1 < M && hend(m1) - hstart(m0) <= 4096) ++m1;
+        const int h0 = hstart(m0), nh = hend(m1 - 1) - h0;
+        // all of a thread's loads of the chunk in one batch (one round trip)
+        for (int b0 = 0; b0 < nh; b0 += 16 * (int)blockDim.x) {
+          double v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int i = b0 + u * (int)blockDim.x + (int)threadIdx.x;
+            v[u] = i < nh ? db[h0 + i] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int i = b0 + u * (int)blockDim.x + (int)threadIdx.x;
+            if (i < nh) chunk[i] = v[u];
+          }
+        }
+        __syncthreads();
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+        for (int m = m0 + wave; m < m1; m += nw) {
+          const int a = hstart(m) - h0, n = hend(m) - hstart(m);
+          double sm = 0.0;
+          for (int i = lane; i < n; i += 64) sm = sm + chunk[a + i];
+          for (int off = 32; off >= 1; off >>= 1) sm = sm + __shfl_xor(sm, off);
+          if (lane == 0 && m < 256) {
+            Em[comp][m] = sm;
+            Ekm[comp][m] = chunk[(m == 0 ? 0 : hstart(m) + (K - 1)) - h0];
+          }
+        }
+        __syncthreads();
+        m0 = m1;
       }
-      Ek0s[threadIdx.x] = s;
     }
-    __syncthreads();
   }
   if (threadIdx.x != 0) return;
   double E = 0.0, Ek0 = 0.0, Em0 = 0.0;
   if (diag_now) {
     for (int comp = 0; comp < 3; ++comp) {
-      double e = 0.0;
-      for (int m = 0; m < G.M; ++m) e = e + Em[comp][m];
+      double e = 0.0, ek = 0.0;   // Ek(0): harmonics with k = 0 in order, m = 0..M-1
+      for (int m = 0; m < G.M; ++m) {
+        e = e + Em[comp][m];
+        ek = ek + Ekm[comp][m];
+      }
       E = E + e;
-      Ek0 = Ek0 + Ek0s[comp];
+      Ek0 = Ek0 + ek;
       Em0 = Em0 + Em[comp][0];
     }
   }
