@@ -122,6 +122,39 @@ int opf_put_state(opf_ctx* ctx, int slot, const double* ur_re, const double* ur_
 int opf_get_state(opf_ctx* ctx, int slot, double* ur_re, double* ur_im, double* ut_re,
                   double* ut_im, double* uz_re, double* uz_im, double* t);
 
+/* ---- state files (netCDF classic, io.f90) ---------------------------------
+ * The file of io_save_state (io.f90:295-392): global attributes t, Re,
+ * alpha; variables r(N), dt, dtcor, dtcfl and Ur, Ut, Uz over (r, H, ReIm)
+ * with attributes K, M, Mp.  Reading follows io_load_state / io_load_coll
+ * (io.f90:98-232): harmonics the grid does not hold are dropped, harmonics
+ * the file does not hold are zero, and a file on other radial points is
+ * interpolated by local polynomials through 2*KL+1 points (io_interp_wts,
+ * io_interp, io.f90:238-289).  The library needs no netCDF installation. */
+/* The file's grid and scalars (N from r; K, M, Mp from Ur); any pointer may
+ * be NULL. */
+int opf_state_file_info(const char* path, int* N, int* K, int* M, int* Mp, double* t,
+                        double* Re, double* alpha, double* dt);
+/* io_save_state on host arrays in the coll layout; r = the N radial points. */
+int opf_state_file_write(const char* path, int N, int K, int M, int Mp, const double* r,
+                         double t, double Re, double alpha, double dt, double dtcor,
+                         double dtcfl, const double* ur_re, const double* ur_im,
+                         const double* ut_re, const double* ut_im, const double* uz_re,
+                         const double* uz_im);
+/* io_load_state onto the grid (N, K, M, Mp, r) on the host (r == NULL: the
+ * default mesh of mes_precompute).  Scalar pointers may be NULL. */
+int opf_state_file_read(const char* path, int N, int K, int M, int Mp, const double* r,
+                        double* t, double* Re, double* alpha, double* dt, double* dtcor,
+                        double* dtcfl, double* ur_re, double* ur_im, double* ut_re,
+                        double* ut_im, double* uz_re, double* uz_im);
+/* io_save_state of a slot: its state, tim_t, Re, alpha, tim_dt, tim_corr_dt
+ * and tim_cfl_dt. */
+int opf_state_save(opf_ctx* ctx, int slot, const char* path);
+/* io_load_state into a set-up slot, then opf_put_state.  t < 0 takes tim_t
+ * from the file (d_time < 0), else t.  An adaptive slot takes tim_dt from
+ * the file (d_timestep < 0) and refactors its matrices; tim_corr_dt and
+ * tim_cfl_dt come from the file. */
+int opf_state_load(opf_ctx* ctx, int slot, const char* path, double t);
+
 /* ---- the time loop (main.f90:18-46), device resident ------------------------
  * Advance every slot that holds a state by `nsteps` steps.  corr_fixed = 0
  * iterates the corrector to convergence as main.f90:35-41 does

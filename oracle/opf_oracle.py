@@ -1089,3 +1089,75 @@ def write_coll3(path, ur, ut, uz):
         parts.append(np.ascontiguousarray(f.real.T))
         parts.append(np.ascontiguousarray(f.imag.T))
     np.concatenate([p.ravel() for p in parts]).astype("<f8").tofile(path)
+
+
+# ---------------------------------------------------------------------------
+# io.f90: loading a state onto another grid
+# ---------------------------------------------------------------------------
+def io_interp_wts(xi, xo):
+    """io_interp_wts (io.f90:238-264): for each output point, the first index
+    l of its stencil and the first row of the inverted local Taylor matrix
+    (the weights of the interpolating polynomial's value at xo)."""
+    xi = [float(v) for v in xi]
+    ni = len(xi)
+    out = []
+    for x in xo:
+        j = 0
+        while xi[j] < x - 1e-8 and j < ni - 1:
+            j += 1
+        lo = max(0, j - KL)
+        hi = min(j + KL, ni - 1)
+        nn = hi - lo + 1
+        A = [[1.0] * nn for _ in range(nn)]
+        for c in range(1, nn):
+            for i in range(nn):
+                A[i][c] = A[i][c - 1] * (xi[lo + i] - float(x)) / float(c)
+        Ainv = mat_invert(A)
+        out.append((lo, list(Ainv[0])))
+    return out
+
+
+def io_interp(wts, fi):
+    """io_interp (io.f90:270-289): fo(n) = dot_product(A(1,1:nn,n), fi(l:r))."""
+    fo = np.zeros(len(wts))
+    for n, (lo, w) in enumerate(wts):
+        s = 0.0
+        for i, wi in enumerate(w):
+            s = s + wi * float(fi[lo + i])
+        fo[n] = s
+    return fo
+
+
+def io_load_coll(a_file, Kf, Mf, r_file, K, M, Mp, r, S):
+    """io_load_coll (io.f90:171-232): the file field a_file (Nf, Hf) on
+    harmonics (Kf, Mf) and radial points r_file mapped onto (K, M) and r.
+    Harmonics are matched by (k, m); the file is interpolated when its radial
+    points differ, continued across the axis as fn(1-i) = (-1)^(m Mp + S) fn(i).
+    The file index follows the file's own harmonic order (io.f90:205 also
+    counts harmonics that neither grid holds, which differs only for a file
+    larger in one of K, M and smaller in the other)."""
+    r_file = np.asarray(r_file, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    Nf = r_file.size
+    interp = Nf != r.size or np.max(np.abs(r - r_file)) > 1e-8
+    xi = np.concatenate([-r_file[KL - 1::-1], r_file])
+    wts = io_interp_wts(xi, r) if interp else None
+    hf = Harmonics(Kf, Mf, Mp)
+    index_f = {(int(k), int(m)): i for i, (k, m) in enumerate(zip(hf.k, hf.m))}
+    hg = Harmonics(K, M, Mp)
+    out = np.zeros((r.size, hg.H), dtype=np.complex128)
+    for h, (k, m) in enumerate(zip(hg.k, hg.m)):
+        i = index_f.get((int(k), int(m)))
+        if i is None:
+            continue
+        col = a_file[:, i]
+        if not interp:
+            out[:, h] = col
+            continue
+        sgn = -1.0 if (m * Mp + S) % 2 == 1 else 1.0
+        parts = []
+        for part in (col.real, col.imag):
+            fn = np.concatenate([sgn * part[KL - 1::-1], part])
+            parts.append(io_interp(wts, fn))
+        out[:, h] = parts[0] + 1j * parts[1]
+    return out

@@ -1132,3 +1132,97 @@ extern "C" int opf_kernel_time(opf_ctx* c, const char* name, double* ms, int* la
     if (launches) *launches = cnt;
   });
 }
+
+// ---- state files (io.f90) -------------------------------------------------
+extern "C" int opf_state_file_info(const char* path, int* N, int* K, int* M, int* Mp, double* t, double* Re,
+                                   double* alpha, double* dt) {
+  return guard([&] {
+    if (!path) throw OpfError(OPF_ERR_ARG, "null path");
+    StateInfo s = read_state_info(path);
+    if (N) *N = s.N;
+    if (K) *K = s.K;
+    if (M) *M = s.M;
+    if (Mp) *Mp = s.Mp;
+    if (t) *t = s.t;
+    if (Re) *Re = s.Re;
+    if (alpha) *alpha = s.alpha;
+    if (dt) *dt = s.dt;
+  });
+}
+
+extern "C" int opf_state_file_write(const char* path, int N, int K, int M, int Mp, const double* r, double t,
+                                    double Re, double alpha, double dt, double dtcor, double dtcfl,
+                                    const double* urr, const double* uri, const double* utr, const double* uti,
+                                    const double* uzr, const double* uzi) {
+  return guard([&] {
+    const double* f[6] = {urr, uri, utr, uti, uzr, uzi};
+    if (!path || !r) throw OpfError(OPF_ERR_ARG, "null argument");
+    for (auto p : f)
+      if (!p) throw OpfError(OPF_ERR_ARG, "null field");
+    if (N <= KL || N > MAXN || K < 1 || M < 1 || Mp < 1) throw OpfError(OPF_ERR_ARG, "bad N, K, M or Mp");
+    write_state_file(path, N, K, M, Mp, t, Re, alpha, dt, dtcor, dtcfl, r, f);
+  });
+}
+
+extern "C" int opf_state_file_read(const char* path, int N, int K, int M, int Mp, const double* r, double* t,
+                                   double* Re, double* alpha, double* dt, double* dtcor, double* dtcfl, double* urr,
+                                   double* uri, double* utr, double* uti, double* uzr, double* uzi) {
+  return guard([&] {
+    double* f[6] = {urr, uri, utr, uti, uzr, uzi};
+    if (!path) throw OpfError(OPF_ERR_ARG, "null path");
+    for (auto p : f)
+      if (!p) throw OpfError(OPF_ERR_ARG, "null field");
+    if (N <= KL || N > MAXN || K < 1 || M < 1 || Mp < 1) throw OpfError(OPF_ERR_ARG, "bad N, K, M or Mp");
+    HostMesh me;
+    if (!r) build_mesh(N, nullptr, me);   // the default mesh of mes_precompute
+    read_state_file(path, N, K, M, Mp, r ? r : me.r.data(), t, Re, alpha, dt, dtcor, dtcfl, f);
+  });
+}
+
+extern "C" int opf_state_save(opf_ctx* c, int slot, const char* path) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (!path) throw OpfError(OPF_ERR_ARG, "null path");
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    SlotParams p{};
+    HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+    auto ctl = read_ctl(c);
+    const SlotCtl& k = ctl[slot];
+    const size_t nh = (size_t)c->N * c->H;
+    std::vector<double> buf(6 * nh);
+    const double2* u = c->B.U[k.ucur] + (size_t)slot * 3 * c->G.coll;
+    for (int comp = 0; comp < 3; ++comp)
+      dev_to_coll(c, u + comp * c->G.coll, &buf[2 * comp * nh], &buf[(2 * comp + 1) * nh]);
+    const double* f[6];
+    for (int i = 0; i < 6; ++i) f[i] = &buf[i * nh];
+    write_state_file(path, c->N, c->K, c->M, c->Mp, k.t, p.Re, p.alpha, p.dt, k.corr_dt, k.cfl_dt,
+                     c->mesh.r.data(), f);
+  });
+}
+
+extern "C" int opf_state_load(opf_ctx* c, int slot, const char* path, double t) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (!path) throw OpfError(OPF_ERR_ARG, "null path");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (opf_slot_setup)");
+    const size_t nh = (size_t)c->N * c->H;
+    std::vector<double> buf(6 * nh);
+    double* f[6];
+    for (int i = 0; i < 6; ++i) f[i] = &buf[i * nh];
+    double tf = 0.0, dtf = 0.0, dtcor = 0.0, dtcfl = 0.0;
+    read_state_file(path, c->N, c->K, c->M, c->Mp, c->mesh.r.data(), &tf, nullptr, nullptr, &dtf, &dtcor, &dtcfl,
+                    f);
+    if (c->adapt[slot].on && dtf > 0.0) {   // d_timestep < 0: tim_dt from the file (io.f90:145)
+      SlotParams p{};
+      HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+      if (p.dt != dtf) slot_setup(c, slot, p.Re, p.alpha, dtf, p.implicit, p.const_flux);
+    }
+    const double* cf[6] = {f[0], f[1], f[2], f[3], f[4], f[5]};
+    put_state(c, slot, cf, t < 0.0 ? tf : t);   // d_time < 0: tim_t from the file (io.f90:120)
+    auto ctl = read_ctl(c);
+    ctl[slot].corr_dt = dtcor;   // tim_corr_dt, tim_cfl_dt (io.f90:147-150)
+    ctl[slot].cfl_dt = dtcfl;
+    write_ctl(c, ctl);
+  });
+}

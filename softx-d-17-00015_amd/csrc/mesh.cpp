@@ -12,52 +12,6 @@
 #include "opf_internal.hpp"
 
 namespace opf {
-namespace {
-
-// DERIV2 (cheby.f:69-108): coefficients of df/dx from those of f.
-std::vector<double> deriv2(const std::vector<double>& a) {
-  const int m = (int)a.size();
-  std::vector<double> A(m + 1), AX(m + 1, 0.0);
-  for (int i = 0; i < m; ++i) A[i + 1] = a[i];
-  AX[m] = 0.0;
-  if (m == 1) {
-    AX[1] = 0.0;
-  } else if (m == 2) {
-    AX[1] = 2.0 * A[2];
-  } else if (m == 3) {
-    AX[1] = 2.0 * A[2];
-    AX[2] = 8.0 * A[3];
-  } else if (m == 4) {
-    AX[1] = 2.0 * A[2] + 6.0 * A[4];
-    AX[2] = 8.0 * A[3];
-    AX[3] = 12.0 * A[4];
-  } else {
-    AX[m - 1] = -((4.0 * (1.0 - m)) * A[m]);
-    AX[m - 2] = -((4.0 * (2.0 - m)) * A[m - 1]);
-    for (int j = m - 3; j >= 2; --j) AX[j] = (4.0 * j) * A[j + 1] + AX[j + 2];
-    AX[1] = 2.0 * A[2] + AX[3] / 2.0;
-  }
-  return std::vector<double>(AX.begin() + 1, AX.end());
-}
-
-// EVAL2 (cheby.f:38-67): Clenshaw evaluation on the shifted interval [0,1].
-double eval2(const std::vector<double>& a, double x) {
-  const int m = (int)a.size();
-  std::vector<double> B(m + 3, 0.0);
-  const double xa = 2.0 * x - 1.0;
-  const double xb = 2.0 * xa;
-  for (int j = m - 1; j >= 0; --j) B[j + 1] = (xb * B[j + 2] - B[j + 3]) + a[j];
-  return B[1] - xa * B[2];
-}
-
-// CHEBY (cheby.f:2-36): nd-th derivative of T_m at x.
-double cheby(int m, int nd, double x) {
-  if (m == 0) return nd == 0 ? 1.0 : 0.0;
-  std::vector<double> a(m + 1, 0.0);
-  a[m] = 1.0;
-  for (int i = 0; i < nd; ++i) a = deriv2(a);
-  return eval2(a, x);
-}
 
 // DGETF2 + DGETRI (unblocked) on a row-major n x n matrix, in place.
 void mat_invert(std::vector<double>& A, int n) {
@@ -112,6 +66,54 @@ void mat_invert(std::vector<double>& A, int n) {
       for (int i = 0; i < n; ++i) std::swap(at(i, j), at(i, jp));
   }
 }
+
+namespace {
+
+// DERIV2 (cheby.f:69-108): coefficients of df/dx from those of f.
+std::vector<double> deriv2(const std::vector<double>& a) {
+  const int m = (int)a.size();
+  std::vector<double> A(m + 1), AX(m + 1, 0.0);
+  for (int i = 0; i < m; ++i) A[i + 1] = a[i];
+  AX[m] = 0.0;
+  if (m == 1) {
+    AX[1] = 0.0;
+  } else if (m == 2) {
+    AX[1] = 2.0 * A[2];
+  } else if (m == 3) {
+    AX[1] = 2.0 * A[2];
+    AX[2] = 8.0 * A[3];
+  } else if (m == 4) {
+    AX[1] = 2.0 * A[2] + 6.0 * A[4];
+    AX[2] = 8.0 * A[3];
+    AX[3] = 12.0 * A[4];
+  } else {
+    AX[m - 1] = -((4.0 * (1.0 - m)) * A[m]);
+    AX[m - 2] = -((4.0 * (2.0 - m)) * A[m - 1]);
+    for (int j = m - 3; j >= 2; --j) AX[j] = (4.0 * j) * A[j + 1] + AX[j + 2];
+    AX[1] = 2.0 * A[2] + AX[3] / 2.0;
+  }
+  return std::vector<double>(AX.begin() + 1, AX.end());
+}
+
+// EVAL2 (cheby.f:38-67): Clenshaw evaluation on the shifted interval [0,1].
+double eval2(const std::vector<double>& a, double x) {
+  const int m = (int)a.size();
+  std::vector<double> B(m + 3, 0.0);
+  const double xa = 2.0 * x - 1.0;
+  const double xb = 2.0 * xa;
+  for (int j = m - 1; j >= 0; --j) B[j + 1] = (xb * B[j + 2] - B[j + 3]) + a[j];
+  return B[1] - xa * B[2];
+}
+
+// CHEBY (cheby.f:2-36): nd-th derivative of T_m at x.
+double cheby(int m, int nd, double x) {
+  if (m == 0) return nd == 0 ? 1.0 : 0.0;
+  std::vector<double> a(m + 1, 0.0);
+  a[m] = 1.0;
+  for (int i = 0; i < nd; ++i) a = deriv2(a);
+  return eval2(a, x);
+}
+
 
 // Integer power as the reference's compiler lowers x**n for integer n.
 double powi(double x, int n) {

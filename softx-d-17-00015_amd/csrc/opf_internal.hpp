@@ -125,4 +125,22 @@ struct MeshOff {
   }
 };
 
+// DGETF2 + DGETRI (unblocked) on a row-major n x n matrix, in place
+// (mes_mat_invert, meshs.f90:231-243).
+void mat_invert(std::vector<double>& A, int n);
+
+// State files (io_save_state / io_load_state, io.f90:98-392).  Fields are the
+// six coll planes Ur.Re, Ur.Im, Ut.Re, Ut.Im, Uz.Re, Uz.Im, each N x H with r
+// fastest.  read_state_file maps the file onto the grid (N, K, M, Mp, r);
+// any scalar pointer may be null.  Both throw std::runtime_error.
+struct StateInfo {
+  int N, K, M, Mp;
+  double t, Re, alpha, dt;
+};
+StateInfo read_state_info(const std::string& path);   // the file's grid (from Ur) and scalars
+void write_state_file(const std::string& path, int N, int K, int M, int Mp, double t, double Re, double alpha,
+                      double dt, double dtcor, double dtcfl, const double* r, const double* const f[6]);
+void read_state_file(const std::string& path, int N, int K, int M, int Mp, const double* r, double* t,
+                     double* Re, double* alpha, double* dt, double* dtcor, double* dtcfl, double* const out[6]);
+
 }  // namespace opf

@@ -32,6 +32,14 @@ PROTOTYPES = {
     "opf_slot_setup": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "opf_put_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double]),
     "opf_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+    "opf_state_file_info": (C.c_int, [C.c_char_p, ip, ip, ip, ip, dp, dp, dp, dp]),
+    "opf_state_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       dp, dp, dp, dp, dp, dp]),
+    "opf_state_file_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp,
+                                      dp, dp, dp, dp, dp, dp, dp, dp]),
+    "opf_state_save": (C.c_int, [vp, C.c_int, C.c_char_p]),
+    "opf_state_load": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_double]),
     "opf_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "opf_run_slots": (C.c_int, [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int]),
     "opf_slot_status": (C.c_int, [vp, C.c_int, ip, dp, ip, ip, dp, dp]),

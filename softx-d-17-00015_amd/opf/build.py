@@ -19,7 +19,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libopf.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["mesh.cpp", "kernels_step.hip", "kernels_harmonic.hip", "kernels_fft.hip", "engine.hip", "serve.cpp"]
+SOURCES = ["mesh.cpp", "state_io.cpp", "kernels_step.hip", "kernels_harmonic.hip", "kernels_fft.hip", "engine.hip", "serve.cpp"]
 ARCH = os.environ.get("OPF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -36,6 +36,51 @@ def _join(re, im):
     return (re + 1j * im).T.copy()
 
 
+def _path(p):
+    return str(p).encode()
+
+
+def state_file_info(path):
+    """Grid and scalars of a state file: N, K, M, Mp, t, Re, alpha, dt."""
+    lib = _lib.load()
+    iv = [C.c_int() for _ in range(4)]
+    dv = [C.c_double() for _ in range(4)]
+    check(lib.opf_state_file_info(_path(path), *[C.byref(x) for x in iv], *[C.byref(x) for x in dv]))
+    return dict(zip(("N", "K", "M", "Mp"), [x.value for x in iv]),
+                **dict(zip(("t", "Re", "alpha", "dt"), [x.value for x in dv])))
+
+
+def write_state_file(path, r, K, M, Mp, ur, ut, uz, t=0.0, Re=0.0, alpha=0.0, dt=0.0, dtcor=0.0,
+                     dtcfl=0.0):
+    """io_save_state (io.f90:295-392) of host fields a[n, h] on the radial points r."""
+    lib = _lib.load()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    N = r.size
+    H = (2 * K - 1) * M - (K - 1)
+    parts = []
+    for a in (ur, ut, uz):
+        parts.extend(_split(a, N, H))
+    check(lib.opf_state_file_write(_path(path), N, K, M, Mp, _dptr(r), float(t), float(Re), float(alpha),
+                                   float(dt), float(dtcor), float(dtcfl), *[_dptr(p) for p in parts]))
+
+
+def read_state_file(path, N, K, M, Mp=1, r=None):
+    """io_load_state (io.f90:98-232) onto the grid (N, K, M, Mp, r) on the host.
+
+    Returns a dict with ur, ut, uz (complex (N, H)) and t, Re, alpha, dt, dtcor,
+    dtcfl.  r = None is the default mesh."""
+    lib = _lib.load()
+    H = (2 * K - 1) * M - (K - 1)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    parts = [np.zeros((H, N)) for _ in range(6)]
+    sc = [C.c_double() for _ in range(6)]
+    check(lib.opf_state_file_read(_path(path), N, K, M, Mp, _dptr(rr) if rr is not None else None,
+                                  *[C.byref(x) for x in sc], *[_dptr(p) for p in parts]))
+    out = dict(zip(("t", "Re", "alpha", "dt", "dtcor", "dtcfl"), [x.value for x in sc]))
+    out.update(ur=_join(parts[0], parts[1]), ut=_join(parts[2], parts[3]), uz=_join(parts[4], parts[5]))
+    return out
+
+
 class Context:
     def __init__(self, N, K, M, Mp=1, capacity=1, r=None):
         self.lib = _lib.load()
@@ -102,6 +147,15 @@ class Context:
         check(self.lib.opf_get_state(self.h, slot, *[_dptr(p) for p in parts], C.byref(t)))
         return (_join(parts[0], parts[1]), _join(parts[2], parts[3]), _join(parts[4], parts[5]),
                 t.value)
+
+    # -- state files (io.f90) ------------------------------------------------------
+    def save_state(self, slot, path):
+        """io_save_state (io.f90:295-392) of one slot to a netCDF classic file."""
+        check(self.lib.opf_state_save(self.h, slot, _path(path)))
+
+    def load_state(self, slot, path, t=-1.0):
+        """io_load_state (io.f90:98-232) into a set-up slot; t < 0 keeps the file's time."""
+        check(self.lib.opf_state_load(self.h, slot, _path(path), float(t)))
 
     # -- time loop ------------------------------------------------------------
     def run(self, nsteps, corr_fixed=0, diag_every=0, slots=None):
