@@ -310,6 +310,55 @@ module opf_c
          integer(c_int), intent(out) :: launches
          integer(c_int) :: rc
       end function
+
+      ! state files: io_save_state / io_load_state (io.f90:98-392)
+      function opf_state_file_info(path, N, K, M, Mp, t, Re, alpha, dt) bind(C, name="opf_state_file_info") &
+            result(rc)
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), intent(out) :: N, K, M, Mp
+         real(c_double), intent(out) :: t, Re, alpha, dt
+         integer(c_int) :: rc
+      end function
+
+      function opf_state_file_write(path, N, K, M, Mp, r, t, Re, alpha, dt, dtcor, dtcfl, &
+            ur_re, ur_im, ut_re, ut_im, uz_re, uz_im) bind(C, name="opf_state_file_write") result(rc)
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: N, K, M, Mp
+         real(c_double), intent(in) :: r(*)
+         real(c_double), value :: t, Re, alpha, dt, dtcor, dtcfl
+         real(c_double), intent(in) :: ur_re(*), ur_im(*), ut_re(*), ut_im(*), uz_re(*), uz_im(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_state_file_read(path, N, K, M, Mp, r, t, Re, alpha, dt, dtcor, dtcfl, &
+            ur_re, ur_im, ut_re, ut_im, uz_re, uz_im) bind(C, name="opf_state_file_read") result(rc)
+         import :: c_int, c_double, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: N, K, M, Mp
+         real(c_double), intent(in) :: r(*)
+         real(c_double), intent(out) :: t, Re, alpha, dt, dtcor, dtcfl
+         real(c_double), intent(out) :: ur_re(*), ur_im(*), ut_re(*), ut_im(*), uz_re(*), uz_im(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_state_save(ctx, slot, path) bind(C, name="opf_state_save") result(rc)
+         import :: c_ptr, c_int, c_char
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_state_load(ctx, slot, path, t) bind(C, name="opf_state_load") result(rc)
+         import :: c_ptr, c_int, c_double, c_char
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         character(kind=c_char), intent(in) :: path(*)
+         real(c_double), value :: t
+         integer(c_int) :: rc
+      end function
    end interface
 
 contains

@@ -6,7 +6,9 @@
 !   dt < 0 selects the adaptive time step (d_timestep < 0, main.f90:24-29)
 !   with |dt| as the initial tim_dt.
 !   state_in / state_out: raw stream files holding vel_ur, vel_ut, vel_uz as
-!   Re(1:N,0:H-1) then Im(1:N,0:H-1) per component.  series: optional text
+!   Re(1:N,0:H-1) then Im(1:N,0:H-1) per component, or state files of
+!   io_save_state (netCDF classic) when the name ends in '.cdf.dat'; a state
+!   file read takes tim_t from the file (d_time < 0).  series: optional text
 !   file with one line per step: step, correctors used, last dterr, tim_dt.
 !*************************************************************************
 program opf_main
@@ -33,12 +35,16 @@ program opf_main
    d_timestep = dt
    call vel_precompute(N, K, M, 1, Re, alpha, dabs(dt))
    call vel_matrices()
-   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
-   read(u) vel_ur%Re, vel_ur%Im
-   read(u) vel_ut%Re, vel_ut%Im
-   read(u) vel_uz%Re, vel_uz%Im
-   close(u)
-   call vel_put_state()
+   if (is_state_file(fin)) then
+      call io_load_state(fin, -1d0)
+   else
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) vel_ur%Re, vel_ur%Im
+      read(u) vel_ut%Re, vel_ut%Im
+      read(u) vel_uz%Re, vel_uz%Im
+      close(u)
+      call vel_put_state()
+   end if
    if (want_series) open(newunit=u, file=trim(fser), status='replace')
 
    i_maxtstep = nsteps
@@ -69,11 +75,25 @@ program opf_main
    if (want_series) close(u)
 
    call vel_status(stepdev, ncorr, dterr_dev)
-   call vel_get_state()
-   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-   write(u) vel_ur%Re, vel_ur%Im
-   write(u) vel_ut%Re, vel_ut%Im
-   write(u) vel_uz%Re, vel_uz%Im
-   close(u)
+   if (is_state_file(fout)) then
+      call io_save_state(fout)
+   else
+      call vel_get_state()
+      open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write(u) vel_ur%Re, vel_ur%Im
+      write(u) vel_ut%Re, vel_ut%Im
+      write(u) vel_uz%Re, vel_uz%Im
+      close(u)
+   end if
    call vel_cleanup()
+
+contains
+
+   logical function is_state_file(name)
+      character(len=*), intent(in) :: name
+      integer :: n
+      n = len_trim(name)
+      is_state_file = n >= 8
+      if (is_state_file) is_state_file = name(n-7:n) == '.cdf.dat'
+   end function is_state_file
 end program opf_main

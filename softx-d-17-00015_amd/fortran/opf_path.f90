@@ -38,6 +38,7 @@ module opf_path
 
    type(coll) :: vel_ur, vel_ut, vel_uz
    double precision :: vel_Pr0
+   double precision, allocatable, target :: mes_r(:)   ! mes_D%r(:,1)
 
    ! timestep.f90:12-24
    double precision :: tim_t, tim_dt, tim_dterr, tim_corr_dt, tim_cfl_dt
@@ -84,6 +85,10 @@ contains
       tim_t = 0d0; tim_step = 0; tim_it = 0; tim_dterr = 0d0; tim_corr_dt = 0d0; tim_pcn = -1
       tim_cfl_dt = 0d0; tim_cfl_dir = 0; tim_new_dt = .false.
       call coll_alloc(vel_ur); call coll_alloc(vel_ut); call coll_alloc(vel_uz)
+      if (allocated(mes_r)) deallocate(mes_r)
+      allocate(mes_r(i_N))
+      call opf_check(opf_mesh_get(h, c_loc(mes_r), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), &
+                     'vel_precompute')
    end subroutine vel_precompute
 
    ! vel_matrices (velocity.f90:120-146): factors for tim_dt
@@ -106,6 +111,37 @@ contains
       call opf_check(opf_get_state(h, slot, vel_ur%Re, vel_ur%Im, vel_ut%Re, vel_ut%Im, vel_uz%Re, vel_uz%Im, &
                                 t), 'vel_get_state')
    end subroutine vel_get_state
+
+   ! io_load_state (io.f90:98-165): the file onto the grid and into the host
+   ! mirrors; tim_t from the file when d_time < 0, tim_dt when d_timestep < 0
+   ! (then vel_matrices), tim_corr_dt and tim_cfl_dt; then vel_put_state.
+   subroutine io_load_state(fname, d_time)
+      character(len=*), intent(in) :: fname
+      double precision, intent(in) :: d_time
+      real(c_double) :: t, Re, alpha, dt, dtcor, dtcfl
+      call opf_check(opf_state_file_read(trim(fname)//c_null_char, int(i_N, c_int), int(i_K, c_int), &
+                                         int(i_M, c_int), int(i_Mp, c_int), mes_r, t, Re, alpha, dt, dtcor, &
+                                         dtcfl, vel_ur%Re, vel_ur%Im, vel_ut%Re, vel_ut%Im, vel_uz%Re, &
+                                         vel_uz%Im), 'io_load_state')
+      if (d_time < 0d0) tim_t = t
+      if (d_timestep < 0d0 .and. dt > 0d0 .and. dt /= tim_dt) then
+         tim_dt = dt
+         call vel_matrices()
+      end if
+      tim_corr_dt = dtcor
+      tim_cfl_dt = dtcfl
+      call vel_put_state()
+   end subroutine io_load_state
+
+   ! io_save_state (io.f90:295-340) to the file fname, from the device state
+   subroutine io_save_state(fname)
+      character(len=*), intent(in) :: fname
+      call vel_get_state()
+      call opf_check(opf_state_file_write(trim(fname)//c_null_char, int(i_N, c_int), int(i_K, c_int), &
+                                          int(i_M, c_int), int(i_Mp, c_int), mes_r, tim_t, d_Re, d_alpha, &
+                                          tim_dt, tim_corr_dt, tim_cfl_dt, vel_ur%Re, vel_ur%Im, vel_ut%Re, &
+                                          vel_ut%Im, vel_uz%Re, vel_uz%Im), 'io_save_state')
+   end subroutine io_save_state
 
    subroutine vel_imposesym()                       ! velocity.f90:61-74
       call opf_check(opf_vel_imposesym(h, slot), 'vel_imposesym')

@@ -103,3 +103,34 @@ def test_fortran_host_adaptive_matches_reference():
     assert np.array_equal(ser[:, 1].astype(int), g["series"][:, 8].astype(int))
     for c in range(3):
         assert rel_err(u[c], g[f"state_{n}"][c]) <= 1e-10, c
+
+
+@pytest.mark.gpu
+def test_fortran_host_restarts_from_state_file():
+    """io_save_state / io_load_state in the Fortran host: 4 steps written to a
+    state file, 6 more steps from it, against the continuous 10-step run (bit
+    for bit: a fixed-dt run depends only on the state) and the reference's
+    state after 10 steps; the file carries tim_t = 4 dt."""
+    if not os.access(EXE, os.X_OK):
+        pytest.skip("Fortran host not built")
+    import opf
+    g = load_golden("mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    H = (2 * K - 1) * M - (K - 1)
+    common = [str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])), repr(float(g["dt"]))]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fmid, fout, fall = (os.path.join(td, x) for x in ("in.bin", "state0001.cdf.dat", "out.bin",
+                                                                "all.bin"))
+        _write_state(fin, g["ic"])
+        print("fortran host: restart", flush=True)
+        subprocess.run([EXE, *common, "4", fin, fmid], check=True, timeout=120)
+        info = opf.state_file_info(fmid)
+        assert (info["N"], info["K"], info["M"]) == (N, K, M)
+        assert abs(info["t"] - 4 * float(g["dt"])) <= 1e-15
+        subprocess.run([EXE, *common, "6", fmid, fout], check=True, timeout=120)
+        subprocess.run([EXE, *common, "10", fin, fall], check=True, timeout=120)
+        u = _read_state(fout, N, H)
+        v = _read_state(fall, N, H)
+    for c in range(3):
+        assert np.array_equal(u[c], v[c]), c
+        assert rel_err(u[c], g["state_10"][c]) <= 1e-10, c
