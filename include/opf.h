@@ -222,6 +222,23 @@ int opf_tim_lumesh_get(opf_ctx* ctx, int slot, int which, double* ab, int* ipiv)
  * BC as for that matrix, on a host coll field (in place). */
 int opf_tim_lumesh_invert(opf_ctx* ctx, int slot, int which, double* b_re, double* b_im);
 
+/* ---- Newton-Krylov clients (newton.f90) -------------------------------------
+ * var_coll_shift (variables.f90:487-528) of a slot's state: harmonic (k, m)
+ * times e^{-i(alpha k dz + m Mp dth)} (dz first, then dth, each rounded as
+ * the reference does), the vorticity recomputed for the next evaluation. */
+int opf_vel_shift(opf_ctx* ctx, int slot, double dth, double dz);
+/* steporbit (newton.f90:721-775) for `nslots` slots at once: for ndts[i] > 1,
+ * tim_dt = T[i]/ndts[i] (vel_matrices when it changes), ndts[i] steps of a
+ * predictor and exactly one corrector, vel_imposesym, then the state shifted
+ * back by the orbit's shifts, var_coll_shift(-sth[i], -sz[i]); for
+ * ndts[i] == 1, one such step with the slot's current tim_dt and no shift
+ * (the trajectory direction of multJ).  Slots must hold states and use a
+ * fixed time step.  Independent segments (multiple shooting, several
+ * searches, the perturbed and base integrations of a Jacobian product)
+ * advance together in one launch sequence. */
+int opf_steporbit(opf_ctx* ctx, int nslots, const int* slots, const int* ndts, const double* T,
+                  const double* sth, const double* sz);
+
 /* ---- request server (continuous batching of integrations) ----------------
  * A request integrates one state for `nsteps` steps with its own Re, alpha
  * and dt (the time integration steporbit hands to the stepper,

@@ -1021,6 +1021,34 @@ class Ctx:
         Ufr = math.sqrt(abs((s - 2.0) / self.Re))
         return dict(t=self.t, E=E, Ek0=Ek0, Em0=Em0, Pr0=self.Pr0, Uc=Uc, Ufr=Ufr)
 
+    def coll_shift(self, a, dth, dz):
+        """var_coll_shift (variables.f90:487-528): harmonic (k, m) times
+        e^{-i alpha k dz}, then e^{-i m Mp dth}, i.e. the field u(theta - dth,
+        z - dz); each rotation as Re c + Im s, Im c - Re s."""
+        b = a.copy()
+        for d, idx, f in ((dz, self.harm.k, self.alpha), (dth, self.harm.m, float(self.Mp))):
+            if d == 0.0:
+                continue
+            arg = (d * idx.astype(float)) * f
+            c, sn = np.cos(arg), np.sin(arg)
+            re, im = b.real.copy(), b.imag.copy()
+            b = (re * c + im * sn) + 1j * (im * c - re * sn)
+        return b
+
+    def steporbit(self, ndts, T, sth, sz):
+        """steporbit (newton.f90:721-775) on the current state: for ndts /= 1,
+        tim_dt = T/ndts and vel_matrices; ndts steps with one corrector each
+        (imposesym, predictor, corrector, imposesym); for ndts /= 1 the result
+        shifted back, var_coll_shift(-sth, -sz)."""
+        if ndts != 1:
+            self.dt = T / float(ndts)
+            self.matrices()
+        self.step = 0
+        self.run(ndts, ncorr_fixed=1)
+        self.imposesym()
+        if ndts != 1:
+            self.ur, self.ut, self.uz = (self.coll_shift(f, -sth, -sz) for f in (self.ur, self.ut, self.uz))
+
     def run(self, nsteps, ncorr_fixed=None, record=None, adaptive=False, maxdt=1e99, courant=0.5):
         """The loop of main.f90:18-46 (no I/O).  With ncorr_fixed the corrector
         runs exactly that many times per step (the steporbit schedule of

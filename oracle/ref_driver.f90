@@ -27,6 +27,12 @@
 !   out_prefix_timing.txt          wall seconds for the time loop
 !   nsteps < 0: no time loop; out_prefix_sym_I.bin holds var_imposesym(I)
 !   (variables.f90:538-618) of the input state for I = 0..3.
+!   OPF_ORBIT="ndts T sth sz" in the environment: instead of the loop, the
+!   call sequence of steporbit (newton.f90:721-775): for ndts /= 1,
+!   tim_dt = T/ndts and vel_matrices; ndts steps of imposesym, transform,
+!   nonlinear, predictor, transform, nonlinear, corrector, imposesym,
+!   tim_check_cgce with tim_it = 1; for ndts /= 1 the final state shifted by
+!   var_coll_shift(-sth, -sz).  Writes out_prefix_orbit.bin.
 !*************************************************************************
  program opf_ref_driver
    use velocity
@@ -68,6 +74,11 @@
    call read_state(trim(infile))
    if(nsteps < 0) then
       call dump_symmetries()
+      stop
+   end if
+   call get_environment_variable('OPF_ORBIT', arg, status=i)
+   if(i == 0) then
+      call run_orbit(trim(arg))
       stop
    end if
    call vel_matrices()
@@ -174,6 +185,37 @@
       write(cnum,'(I6.6)') step
       call write_coll3(trim(prefix)//'_state_'//cnum//'.bin', vel_ur, vel_ut, vel_uz)
    end subroutine write_state
+
+   subroutine run_orbit(spec)
+      character(*), intent(in) :: spec
+      integer :: ndts, ndt
+      double precision :: T, sth, sz
+      read(spec,*) ndts, T, sth, sz
+      if(ndts /= 1) tim_dt = T / dble(ndts)
+      call vel_matrices()
+      do ndt = 1, ndts
+         call vel_imposesym()
+         call vel_transform()
+         call vel_nonlinear()
+         call vel_predictor()
+         call vel_transform()
+         call vel_nonlinear()
+         call vel_corrector()
+         call vel_imposesym()
+         tim_it = 1
+         call tim_check_cgce()
+         if(ndts /= 1) then
+            tim_t    = tim_t    + tim_dt
+            tim_step = tim_step + 1
+            if(ndt == ndts) then
+               call var_coll_shift(-sth, -sz, vel_ur, vel_ur)
+               call var_coll_shift(-sth, -sz, vel_ut, vel_ut)
+               call var_coll_shift(-sth, -sz, vel_uz, vel_uz)
+            end if
+         end if
+      end do
+      call write_coll3(trim(prefix)//'_orbit.bin', vel_ur, vel_ut, vel_uz)
+   end subroutine run_orbit
 
    subroutine dump_symmetries()
       type (coll) :: a, b, c

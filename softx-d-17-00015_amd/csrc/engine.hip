@@ -93,10 +93,12 @@ struct opf_ctx {
   int n_adaptive = 0;          // slots with the adaptive time step on
   std::vector<int> symm;       // symmetry bits per slot (SlotParams::symm)
   int n_symm = 0;              // slots with a symmetry imposed
+  std::vector<double> alpha_h; // d_alpha per slot (host copy of SlotParams::alpha)
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
   double2* tC = nullptr;
+  double2* shiftTab = nullptr;   // var_coll_shift tables, (2K + M) per slot (first use)
   Timing tm;
 
   template <typename T> T* alloc(size_t n) {
@@ -398,6 +400,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->setup.assign(cap, 0);
   c->adapt.assign(cap, opf_ctx::Adapt{});
   c->symm.assign(cap, 0);
+  c->alpha_h.assign(cap, 0.0);
 }
 
 void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
@@ -412,6 +415,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   p.maxdt = c->adapt[slot].maxdt;        // d_maxdt (parameters.f90:39)
   p.symm = c->symm[slot];                // b_mirrorsym, b_shiftrefl, b_shiftrott
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
+  c->alpha_h[slot] = alpha;
   auto ctl = read_ctl(c);
   ctl[slot].status = 0;
   write_ctl(c, ctl);
@@ -516,14 +520,13 @@ int run_remaining(opf_ctx* c, const RunPlan& p) {
 
 namespace {
 
-void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
-  if (nsteps < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
-  const RunPlan p = run_begin(c, slots, std::vector<int>(slots.size(), nsteps),
-                              std::vector<int>(slots.size(), corr_fixed), diag_every, true);
+void run_plan(opf_ctx* c, const std::vector<int>& slots, const std::vector<int>& nsteps,
+              const std::vector<int>& corr_fixed, int diag_every) {
+  const RunPlan p = run_begin(c, slots, nsteps, corr_fixed, diag_every, true);
   if (p.ns == 0) return;
   // Most steps take one corrector; enqueue 2 iterations per step, then top
   // up until every slot has reached its target.
-  long chunk = 2L * nsteps;
+  long chunk = 2L * p.max_steps;
   for (int round = 0; round < 64 && chunk > 0; ++round) {
     run_enqueue(c, p, chunk);
     const int remaining = run_remaining(c, p);
@@ -536,6 +539,65 @@ void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_f
       std::snprintf(msg, sizeof msg, "tim_check_cgce failure %d in slot %d at step %d", ctl[s].status, s, ctl[s].step);
       throw OpfError(OPF_ERR_CORRECTOR, msg);
     }
+}
+
+void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
+  if (nsteps < 0) throw OpfError(OPF_ERR_ARG, "nsteps < 0");
+  run_plan(c, slots, std::vector<int>(slots.size(), nsteps), std::vector<int>(slots.size(), corr_fixed),
+           diag_every);
+}
+
+// var_coll_shift tables of slots [s0, s0 + ns) (k_shift's layout); a slot
+// not in `sel` gets none.  The host's cos / sin stand for dcos / dsin.
+void launch_shift(opf_ctx* c, int s0, int ns, const std::vector<int>& sel, const std::vector<double>& dth,
+                  const std::vector<double>& dz) {
+  const int per = 2 * c->K + c->M;
+  if (!c->shiftTab) c->shiftTab = c->alloc<double2>((size_t)c->cap * per);
+  std::vector<double2> tab((size_t)ns * per, make_double2(0.0, 0.0));
+  bool any = false;
+  for (size_t i = 0; i < sel.size(); ++i) {
+    const int j = sel[i] - s0;
+    double2* t = &tab[(size_t)j * per];
+    t[0] = make_double2(dz[i] != 0.0 ? 1.0 : 0.0, dth[i] != 0.0 ? 1.0 : 0.0);
+    any = any || dz[i] != 0.0 || dth[i] != 0.0;
+    for (int k = -(c->K - 1); k <= c->K - 1; ++k) {
+      const double a = dz[i] * (double)k * c->alpha_h[sel[i]];
+      t[k + c->K] = make_double2(std::cos(a), std::sin(a));
+    }
+    for (int m = 0; m < c->M; ++m) {
+      const double a = dth[i] * (double)m * (double)c->Mp;
+      t[2 * c->K + m] = make_double2(std::cos(a), std::sin(a));
+    }
+  }
+  if (!any) return;
+  HIPCHK(hipStreamSynchronize(c->stream));   // no kernel still reads the previous tables
+  HIPCHK(hipMemcpy(c->shiftTab, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice));
+  launch(c, "shift", k_shift, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, s0,
+         (const double2*)c->shiftTab);
+}
+
+// The end of steporbit (newton.f90:721-775) for slots `sel`: vel_imposesym
+// after the last corrector, var_coll_shift by (dth, dz), and the vorticity
+// of the result for the next evaluation.
+void orbit_finish(opf_ctx* c, const std::vector<int>& sel, const std::vector<double>& dth,
+                  const std::vector<double>& dz) {
+  if (sel.empty()) return;
+  int lo = c->cap, hi = -1;
+  for (int s : sel) { lo = std::min(lo, s); hi = std::max(hi, s); }
+  const int ns = hi - lo + 1;
+  auto ctl = read_ctl(c);
+  for (int s : sel) {
+    ctl[s].active = 1;
+    ctl[s].target_steps = -1;
+  }
+  write_ctl(c, ctl);
+  launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, lo, 0);
+  launch_shift(c, lo, ns, sel, dth, dz);
+  launch(c, "curl0", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, lo, 0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  ctl = read_ctl(c);
+  for (int s : sel) ctl[s].active = 0;
+  write_ctl(c, ctl);
 }
 
 }  // namespace
@@ -1224,5 +1286,57 @@ extern "C" int opf_state_load(opf_ctx* c, int slot, const char* path, double t) 
     ctl[slot].corr_dt = dtcor;   // tim_corr_dt, tim_cfl_dt (io.f90:147-150)
     ctl[slot].cfl_dt = dtcfl;
     write_ctl(c, ctl);
+  });
+}
+
+// ---- Newton-Krylov clients (newton.f90) ------------------------------------
+extern "C" int opf_vel_shift(opf_ctx* c, int slot, double dth, double dz) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (!c->occupied[slot]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto ctl = read_ctl(c);
+    ctl[slot].active = 1;
+    write_ctl(c, ctl);
+    launch_shift(c, slot, 1, {slot}, {dth}, {dz});
+    launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ctl = read_ctl(c);
+    ctl[slot].active = 0;
+    write_ctl(c, ctl);
+  });
+}
+
+extern "C" int opf_steporbit(opf_ctx* c, int nslots, const int* slots, const int* ndts, const double* T,
+                             const double* sth, const double* sz) {
+  return guard([&] {
+    if (!c || nslots < 0 || (nslots > 0 && (!slots || !ndts || !T || !sth || !sz)))
+      throw OpfError(OPF_ERR_ARG, "null argument");
+    std::vector<int> sel, steps, fixed;
+    std::vector<double> dth, dz;
+    std::vector<int> seen(c->cap, 0);
+    for (int i = 0; i < nslots; ++i) {
+      const int s = slots[i];
+      check_slot(c, s);
+      if (seen[s]++) throw OpfError(OPF_ERR_ARG, "slot listed twice");
+      if (!c->occupied[s]) throw OpfError(OPF_ERR_STATE, "slot holds no state");
+      if (c->adapt[s].on) throw OpfError(OPF_ERR_ARG, "steporbit needs a fixed time step (adaptive slot)");
+      if (ndts[i] < 1) throw OpfError(OPF_ERR_ARG, "ndts must be >= 1");
+      if (ndts[i] != 1) {   // tim_dt = T / ndts and vel_matrices (newton.f90:736-743)
+        if (!(T[i] > 0.0)) throw OpfError(OPF_ERR_ARG, "T must be > 0");
+        SlotParams p{};
+        HIPCHK(hipMemcpy(&p, c->B.prm + s, sizeof(p), hipMemcpyDeviceToHost));
+        const double dt = T[i] / (double)ndts[i];
+        if (p.dt != dt) slot_setup(c, s, p.Re, p.alpha, dt, p.implicit, p.const_flux);
+      }
+      sel.push_back(s);
+      steps.push_back(ndts[i]);
+      fixed.push_back(1);   // predictor and one corrector per step (newton.f90:745-757)
+      // the shift back by the orbit's shifts after the last step (:765-770)
+      dth.push_back(ndts[i] != 1 ? -sth[i] : 0.0);
+      dz.push_back(ndts[i] != 1 ? -sz[i] : 0.0);
+    }
+    run_plan(c, sel, steps, fixed, 0);
+    orbit_finish(c, sel, dth, dz);
   });
 }

@@ -100,6 +100,7 @@ __global__ void k_curl(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_newdt(Geo G, Bufs B, int slot0, int what);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
+__global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ tab);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64

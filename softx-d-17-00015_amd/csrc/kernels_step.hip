@@ -556,6 +556,37 @@ __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated) {
   if (hp != h) u[hp] = y;
 }
 
+// var_coll_shift (variables.f90:487-528) on U[ucur] of slots [slot0,
+// slot0 + gridDim.z): harmonic (k, m) rotated by e^{-i alpha k dz}, then by
+// e^{-i m Mp dth}, each in the reference's order of operations
+// (Re' = Re c + Im s, Im' = Im c - Re s).  tab holds 2K + M entries per
+// slot: [0] = (z shift on, theta shift on), [k + K] = (cos, sin)(dz k alpha)
+// for k = -(K-1)..K-1, [2K + m] = (cos, sin)(dth m Mp), evaluated on the host
+// with the C library's cos and sin as the reference's dcos and dsin.  One
+// thread per (component, n, harmonic).
+__global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ tab) {
+  const int slot = slot0 + blockIdx.z;
+  const double2* tb = tab + (size_t)blockIdx.z * (2 * G.K + G.M);
+  const double2 on = tb[0];
+  if (on.x == 0.0 && on.y == 0.0) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = idx % G.H, rest = idx / G.H;
+  const int n = rest % G.N, comp = rest / G.N;
+  if (comp >= 3) return;
+  int k, m;
+  harmonic_km(h, G.K, k, m);
+  double2* u = B.U[B.ctl[slot].ucur] + (size_t)slot * 3 * G.coll + (size_t)comp * G.coll + (size_t)n * G.Hp + h;
+  double2 x = *u;
+  auto rot = [&](const double2 cs) {
+    const double re = x.x;
+    x.x = __dadd_rn(__dmul_rn(re, cs.x), __dmul_rn(x.y, cs.y));
+    x.y = __dsub_rn(__dmul_rn(x.y, cs.x), __dmul_rn(re, cs.y));
+  };
+  if (on.x != 0.0) rot(tb[k + G.K]);
+  if (on.y != 0.0) rot(tb[2 * G.K + m]);
+  *u = x;
+}
+
 // var_imposesym(0) (variables.f90:611-616) on harmonic 0 of U[ucur].
 __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;

@@ -157,6 +157,28 @@ class Context:
         """io_load_state (io.f90:98-232) into a set-up slot; t < 0 keeps the file's time."""
         check(self.lib.opf_state_load(self.h, slot, _path(path), float(t)))
 
+    # -- Newton-Krylov clients (newton.f90) -----------------------------------------
+    def shift(self, slot, dth, dz):
+        """var_coll_shift (variables.f90:487-528) of the slot's state."""
+        check(self.lib.opf_vel_shift(self.h, slot, float(dth), float(dz)))
+
+    def steporbit(self, slots, ndts, T, sth, sz):
+        """steporbit (newton.f90:721-775) for several slots in one launch
+        sequence; scalars broadcast over the slots."""
+        slots = [int(s) for s in np.atleast_1d(slots)]
+        n = len(slots)
+
+        def vec(v, dtype):
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=dtype), (n,)))
+            return a
+
+        nd = vec(ndts, np.int32)
+        Tv, sv, zv = vec(T, np.float64), vec(sth, np.float64), vec(sz, np.float64)
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        check(self.lib.opf_steporbit(self.h, n, sl.ctypes.data_as(ip), nd.ctypes.data_as(ip), _dptr(Tv),
+                                     _dptr(sv), _dptr(zv)))
+
     # -- time loop ------------------------------------------------------------
     def run(self, nsteps, corr_fixed=0, diag_every=0, slots=None):
         """main.f90:18-46 for `nsteps` steps on the device."""

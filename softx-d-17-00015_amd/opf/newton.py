@@ -1,0 +1,449 @@
+"""Newton-Krylov-hookstep search for (relative) periodic orbits over the
+device time step: the client of newton.f90 with the solver of
+NewtonHook.f90 and GMRESm.f90.
+
+The solver (``NewtonHook``) is generic: it finds a root of f(x) = 0 with
+GMRES(m) on the Jacobian action and Viswanath's hookstep in a trust region,
+exactly as newtonhook / gmresm / hookstep do (NewtonHook.f90:48-179,
+GMRESm.f90:27-153).  ``OrbitNewton`` is newton.f90's problem: the Newton
+vector holds, per shooting segment, the theta shift, z shift and period
+(scaled by scaleT) and the scaled velocity field (rtz2vec), f(x) = F(x) - x
+with F = steporbit (newton.f90:374-393, 721-775), and the Jacobian action by
+finite differences plus the constraints of no update along the shift
+directions and the trajectory (multJ, newton.f90:399-456).
+
+The time integrations run on the device through ``opf_steporbit``: the
+segments of multiple shooting advance together in one context (one slot
+each), so getrhs costs one batched integration rather than ms sequential
+ones.  Any object with the ``Stepper`` interface can stand in for the device
+(the tests use the oracle's CPU time step to check the device path).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg
+
+from .ic import energy, harmonic_km
+
+D_PI = 3.1415926535897931
+
+
+# ---------------------------------------------------------------------------
+# NewtonHook.f90 / GMRESm.f90
+# ---------------------------------------------------------------------------
+def hookstep(h, j, beta, delta):
+    """hookstep (GMRESm.f90:121-153): the y minimising |beta e1 - H y| with
+    |y| <= delta (Viswanath 2008), from the SVD of the (j+1) x j Hessenberg
+    matrix; returns y and the predicted residual |beta e1 - H y|."""
+    a = h[:j + 1, :j]
+    u, s, vt = np.linalg.svd(a, full_matrices=True)
+    p = beta * u[0, :j]
+    mu = max(s[j - 1] * s[j - 1] * 1e-6, 1e-99)
+    qn = 1e99
+    q = np.zeros(j)
+    while qn > delta:
+        mu = mu * 1.1
+        q = p * s / (mu + s * s)
+        qn = math.sqrt(float(np.dot(q, q)))
+    y = q @ vt
+    r = -(a @ y)
+    r[0] = r[0] + beta
+    return y, math.sqrt(float(np.dot(r, r)))
+
+
+class Gmres:
+    """gmresm (GMRESm.f90:27-115): GMRES(m) with modified Gram-Schmidt and
+    least squares by LAPACK dgelsy on the Hessenberg matrix; keeps the Krylov
+    basis, H, j and beta between calls so that info = 2 recomputes the
+    hookstep for a new trust region without new products."""
+
+    def __init__(self, m, n):
+        self.m, self.n = m, n
+        self.h = np.zeros((m + 1, m))
+        self.v = np.zeros((m + 1, n))     # row i = Krylov vector i+1
+        self.j = 0
+        self.beta = 0.0
+
+    def _combine(self, y, j):
+        z = np.zeros(self.n)
+        for i in range(j):
+            z = z + y[i] * self.v[i]
+        return z
+
+    def solve(self, x, b, matvec, psolve, dotprd, res, delta, its, info):
+        """Returns (x, res, delta, its, info) as gmresm leaves its inout
+        arguments: info 0 converged, 1 breakdown (residual grew), 2 max its."""
+        m = self.m
+        if info == 2:
+            y, delta = hookstep(self.h, self.j, self.beta, delta)
+            z = psolve(self._combine(y, self.j))
+            return z, res, delta, its, 0
+        tol = res
+        imx = its
+        its = 0
+        self.v[:] = 0.0
+        while True:   # restart
+            res_ = 1e99
+            stgn = 1.0 - 1e-14
+            beta = math.sqrt(dotprd(x, x))
+            w = np.zeros(self.n) if beta == 0.0 else matvec(x)
+            w = b - w
+            beta = math.sqrt(dotprd(w, w))
+            self.beta = beta
+            self.v[0] = w / beta
+            self.h[:] = 0.0
+            for j in range(1, m + 1):
+                self.j = j
+                its += 1
+                z = psolve(self.v[j - 1].copy())
+                w = matvec(z)
+                for i in range(j):
+                    self.h[i, j - 1] = dotprd(w, self.v[i])
+                    w = w - self.h[i, j - 1] * self.v[i]
+                self.h[j, j - 1] = math.sqrt(dotprd(w, w))
+                self.v[j] = w / self.h[j, j - 1]
+                p = np.zeros(j + 1)
+                p[0] = beta
+                y = scipy.linalg.lstsq(self.h[:j + 1, :j], p, lapack_driver="gelsy")[0]
+                p = -(self.h[:j + 1, :j] @ y)
+                p[0] = p[0] + beta
+                res = math.sqrt(float(np.dot(p, p)))
+                done = res <= tol or its == imx or res > res_
+                if done or j == m:
+                    if delta > 0.0:
+                        y, delta = hookstep(self.h, j, beta, delta)
+                    z = psolve(self._combine(y, j))
+                    x = x + z
+                    if its == imx:
+                        info = 2
+                    if res > res_:
+                        info = 1
+                    if res <= tol:
+                        info = 0
+                    if done:
+                        return x, res, delta, its, info
+                    break   # j == m: restart from the new x
+                res_ = res * stgn
+
+
+class NewtonHook:
+    """newtonhook (NewtonHook.f90:48-179) with the state of its module
+    (new_x, new_fx, new_tol, new_del, new_nits, new_gits)."""
+
+    def __init__(self, x0):
+        self.new_x = np.array(x0, dtype=np.float64)
+        self.new_fx = np.zeros_like(self.new_x)
+        self.new_tol = 0.0
+        self.new_del = 0.0
+        self.new_nits = 0
+        self.new_gits = 0
+        self.log = []
+
+    def _say(self, msg):
+        self.log.append(msg)
+
+    def solve(self, f, df, dfp, sub, dp, m, gtol, tol, delta, mndl, mxdl, nits, info=0):
+        """f(x) -> f; df(x) -> Jacobian action about new_x; dfp(x) -> x
+        (preconditioner); sub() after every accepted iteration; dp(a, b).
+        Returns info: 0 converged, 2 max iterations, 3 trust region too small."""
+        n = self.new_x.size
+        gm = Gmres(m, n)
+        self.new_nits = 0
+        self.new_gits = 0
+        self.new_del = delta
+        mxdl_ = mxdl
+        ginfo = info
+        self.new_fx = f(self.new_x)
+        self.new_tol = math.sqrt(dp(self.new_fx, self.new_fx))
+        if delta < 0.0:
+            self.new_del = self.new_tol / 10.0
+            mxdl_ = 1e99
+        self._say(f"newton: nits={self.new_nits} res={self.new_tol:.6e}")
+        sub()
+        x_, fx_, tol_ = self.new_x.copy(), self.new_fx.copy(), self.new_tol
+        tol__ = 1e99
+        x__ = fx__ = None
+        del__ = 0.0
+        if self.new_tol < tol:
+            self._say("newton: input already converged")
+            return 0
+        while True:
+            if self.new_del < mndl:
+                self._say("newton: trust region too small")
+                return 3
+            s = np.zeros(n)
+            gres = gtol * self.new_tol
+            gdel = self.new_del
+            if ginfo != 2:
+                self.new_gits = m
+            s, gres, gdel, self.new_gits, ginfo = gm.solve(s, fx_, df, dfp, dp, gres, gdel, self.new_gits,
+                                                           ginfo)
+            ginfo = info
+            self.new_x = x_ - s
+            self.new_fx = f(self.new_x)
+            self.new_tol = math.sqrt(dp(self.new_fx, self.new_fx))
+            snrm = math.sqrt(dp(s, s))
+            ared = tol_ - self.new_tol
+            pred = tol_ - gdel
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ratio = float(np.float64(ared) / np.float64(pred))   # IEEE, as the Fortran division
+            self._say(f"newton: nits={self.new_nits} res={self.new_tol:.6e} gits={self.new_gits} "
+                      f"del={self.new_del:.6e} |s|={snrm:.6e} ared/pred={ratio:.4f}")
+            if self.new_tol > tol__:
+                self._say("newton: accepting previous step")
+                self.new_x, self.new_fx, self.new_tol, self.new_del = x__, fx__, tol__, del__
+            elif ared < 0.0:
+                self._say("newton: norm increased, try smaller step")
+                self.new_del = snrm * 0.5
+                ginfo = 2
+            elif ratio < 0.75:
+                self._say("newton: step ok, trying smaller step")
+                x__, fx__, tol__ = self.new_x, self.new_fx, self.new_tol
+                del__ = snrm if ratio > 0.1 else snrm * 0.5
+                self.new_del = snrm * 0.7
+                ginfo = 2
+            elif snrm < self.new_del * 0.9:
+                self._say("newton: step good, took full newton step")
+                self.new_del = min(mxdl_, snrm * 2.0)
+            elif self.new_del < mxdl_ * 0.9:
+                self._say("newton: step good, trying larger step")
+                x__, fx__, tol__, del__ = self.new_x, self.new_fx, self.new_tol, self.new_del
+                self.new_del = min(mxdl_, snrm * 2.0)
+                ginfo = 2
+            if ginfo == 2:
+                continue
+            self.new_nits += 1
+            sub()
+            x_, fx_, tol_ = self.new_x.copy(), self.new_fx.copy(), self.new_tol
+            tol__ = 1e99
+            if self.new_tol < tol:
+                self._say("newton: converged")
+                return 0
+            if self.new_nits == nits:
+                self._say("newton: reached max its")
+                return 2
+
+
+# ---------------------------------------------------------------------------
+# newton.f90: the orbit problem
+# ---------------------------------------------------------------------------
+class DeviceStepper:
+    """steporbit on the device: one slot per segment of a context whose slots
+    are set up with the run's Re, alpha (and symmetries)."""
+
+    def __init__(self, ctx, slots):
+        self.ctx = ctx
+        self.slots = list(slots)
+
+    def steporbit(self, fields, ndts, T, sth, sz, dt1):
+        """fields[i] = (ur, ut, uz) of segment i; ndts[i] == 1 entries take one
+        step of size dt1 (the tim_dt of the last integration).  Returns the
+        integrated and shifted fields."""
+        ctx = self.ctx
+        sl = self.slots[:len(fields)]
+        for s, nd, f in zip(sl, ndts, fields):
+            if nd == 1 and ctx.timestep(s)["dt"] != dt1:
+                ctx.vel_matrices(s, dt1)
+            ctx.put_state(s, *f)
+        ctx.steporbit(sl, ndts, T, sth, sz)
+        return [ctx.get_state(s)[:3] for s in sl]
+
+
+class OrbitNewton:
+    """newton.f90's search for a relative periodic orbit on ms shooting
+    segments.  x0 fields: (ur, ut, uz) per segment (or one state, segments
+    after the first are then produced by integration as load_ms_states
+    does); guesses sth, sz, T per segment (shifts.in); ndts steps per
+    segment.  symm: b_mirrorsym / b_shiftrefl flags (they freeze the theta
+    shift, newton.f90:448-450); T_fixed freezes the period."""
+
+    def __init__(self, stepper, N, K, M, Mp, alpha, intrdr, x0, sth, sz, T, ndts, T_fixed=False,
+                 freeze_theta_shift=False):
+        self.stepper = stepper
+        self.N, self.K, self.M, self.Mp, self.alpha = N, K, M, Mp, float(alpha)
+        self.H = (2 * K - 1) * M - (K - 1)
+        self.n = 8 + 6 * N * self.H
+        self.intrdr = np.asarray(intrdr, dtype=np.float64)
+        self.ndts = [int(v) for v in np.atleast_1d(ndts)]
+        self.ms = len(self.ndts)
+        self.T_fixed = bool(T_fixed)
+        self.freeze_theta = bool(freeze_theta_shift)
+        self.tim_dt = 0.0
+        ks, ms_ = harmonic_km(K, M)
+        self._k, self._m = ks, ms_
+        # rtz2vec weights (newton.f90:572-580): d = sqrt|intrdr|, w(nh)
+        self._d = np.sqrt(np.abs(self.intrdr))
+        self._w = 4.0 / (4.0 + (self.alpha * np.abs(ks)) + (float(Mp) * np.abs(ms_)))
+        states = [x0] if isinstance(x0[0], np.ndarray) and np.ndim(x0[0]) == 2 else list(x0)
+        self.get_scales(*states[0])
+        x = np.zeros(self.n * self.ms)
+        sth, sz, T = (np.broadcast_to(np.asarray(v, dtype=np.float64), (self.ms,)) for v in (sth, sz, T))
+        for p in range(self.ms):
+            x[p * self.n:p * self.n + 3] = (sth[p], sz[p], T[p])
+        x[8:self.n] = self.rtz2vec(*states[0])[8:]
+        d = self.dotprod(-1, x[:self.n], x[:self.n])
+        self.scaleT = x[2] / math.sqrt(d)            # get_scales (newton.f90:266-272)
+        for p in range(self.ms):
+            x[p * self.n:p * self.n + 3] /= self.scaleT
+        for p in range(1, self.ms):                  # load_ms_states without state files
+            if p < len(states):
+                x[p * self.n + 8:(p + 1) * self.n] = self.rtz2vec(*states[p])[8:]
+            else:
+                y = self.steporbit_many([x[(p - 1) * self.n:p * self.n]], [self.ndts[p - 1]])[0]
+                x[p * self.n + 8:(p + 1) * self.n] = y[8:]
+        self.x0 = x
+        self.history = []
+
+    # -- scaled vector <-> fields (rtz2vec / vec2rtz, newton.f90:560-652) --------
+    def get_scales(self, ur, ut, uz):
+        """get_scales (newton.f90:242-275): scaleU from the energies of the
+        components and of the mean axial flow."""
+        e = lambda a: energy([a], self.intrdr, self.K, self.M, self.alpha)  # noqa: E731
+        Er, Et, Ez = e(ur), e(ut), e(uz)
+        c1 = uz.copy()
+        c1[:, 0] = 0.0 + 1j * c1[:, 0].imag
+        Ez00 = Ez - e(c1)
+        Ez = Ez - Ez00
+        self.scaleU = np.array([math.sqrt(0.2 * Ez00 / Er), math.sqrt(0.2 * Ez00 / Et),
+                                math.sqrt(0.5 * Ez00 / Ez)])
+
+    def _scales(self):
+        s = (self._d[:, None, None] * self._w[None, :, None]) * self.scaleU[None, None, :]   # (N, H, 3)
+        s[:, 0, 2] = self._d
+        return s
+
+    def rtz2vec(self, ur, ut, uz):
+        s = self._scales()
+        v = np.zeros(self.n)
+        blk = np.empty((self.H, self.N, 6))
+        for c, a in enumerate((ur, ut, uz)):
+            blk[:, :, 2 * c] = (a.real * s[:, :, c]).T
+            blk[:, :, 2 * c + 1] = (a.imag * s[:, :, c]).T
+        v[8:] = blk.ravel()
+        return v
+
+    def vec2rtz(self, v):
+        s = self._scales()
+        blk = v[8:self.n].reshape(self.H, self.N, 6)
+        return tuple((blk[:, :, 2 * c].T / s[:, :, c]) + 1j * (blk[:, :, 2 * c + 1].T / s[:, :, c])
+                     for c in range(3))
+
+    # -- dot products (newton.f90:658-697) --------------------------------------
+    def dotprod(self, n_, a, b):
+        n1 = 8 if n_ == -1 else 0
+        return float(np.dot(a[n1:self.n], b[n1:self.n]))
+
+    def dotprod_ms(self, n_, a, b):
+        d = 0.0
+        for p in range(self.ms):
+            d = d + self.dotprod(n_, a[p * self.n:(p + 1) * self.n], b[p * self.n:(p + 1) * self.n])
+        return d
+
+    def normE(self, v):
+        """normE (newton.f90:703-715)."""
+        E = 0.0
+        for p in range(self.ms):
+            for a in self.vec2rtz(v[p * self.n:(p + 1) * self.n]):
+                E = E + energy([a], self.intrdr, self.K, self.M, self.alpha)
+        return math.sqrt(E)
+
+    # -- time integration -------------------------------------------------------
+    def steporbit_many(self, xs, ndts):
+        """steporbit (newton.f90:721-775) of several segment vectors at once."""
+        fields = [self.vec2rtz(x) for x in xs]
+        P = [x[:3] * self.scaleT for x in xs]
+        T = [p[2] for p in P]
+        for nd, t in zip(ndts, T):
+            if nd != 1:
+                self.tim_dt = t / float(nd)
+        ys = self.stepper.steporbit(fields, ndts, T, [p[0] for p in P], [p[1] for p in P], self.tim_dt)
+        out = []
+        for x, f in zip(xs, ys):
+            y = self.rtz2vec(*f)
+            y[:8] = x[:8]
+            out.append(y)
+        return out
+
+    def getshiftdir(self, i, x):
+        """getshiftdir (newton.f90:535-556): d x / d(theta shift) or d(z shift)."""
+        ds = (float(self.Mp) * self._m) if i == 1 else (self.alpha * self._k)
+        f = [(-a.imag * ds[None, :]) + 1j * (a.real * ds[None, :]) for a in self.vec2rtz(x)]
+        return self.rtz2vec(*f)
+
+    def getrhs(self, x):
+        """getrhs (newton.f90:374-393): F(x) - x over the segments, constraint rows 0."""
+        n, ms = self.n, self.ms
+        ys = self.steporbit_many([x[p * n:(p + 1) * n] for p in range(ms)], self.ndts)
+        y_ = np.zeros(n * ms)
+        for p in range(ms):
+            p1 = (p + 1) % ms
+            y_[p1 * n:(p1 + 1) * n] = ys[p]
+        y = y_ - x
+        for p in range(ms):
+            y[p * n:p * n + 8] = 0.0
+        return y
+
+    def multJ(self, nh, dx):
+        """multJ (newton.f90:399-456): the Jacobian action by finite
+        differences and the constraint rows."""
+        n, ms = self.n, self.ms
+        eps = math.sqrt(self.dotprod_ms(1, dx, dx))
+        if eps == 0.0:
+            raise RuntimeError("multJ: eps=0 (1)")
+        eps = self.epsJ * math.sqrt(self.dotprod_ms(1, nh.new_x, nh.new_x)) / eps
+        if eps == 0.0:
+            raise RuntimeError("multJ: eps=0 (2)")
+        y = nh.new_x + eps * dx
+        s = self.getrhs(y)
+        y = (s - nh.new_fx) / eps
+        xs = [nh.new_x[p * n:(p + 1) * n] for p in range(ms)]
+        traj = self.steporbit_many(xs, [1] * ms)
+        for p in range(ms):
+            xp, dxp = xs[p], dx[p * n:(p + 1) * n]
+            y[p * n + 0] = self.dotprod(-1, self.getshiftdir(1, xp), dxp)
+            y[p * n + 1] = self.dotprod(-1, self.getshiftdir(2, xp), dxp)
+            st = (traj[p] - xp) / self.tim_dt
+            y[p * n + 2] = self.dotprod(-1, st, dxp)
+            if self.freeze_theta:
+                y[p * n + 0] = dxp[0]
+            if self.T_fixed:
+                y[p * n + 2] = dxp[2]
+        return y
+
+    # -- the search (newton.f90:46-121) -----------------------------------------
+    def solve(self, m=100, nits=20, rel_err=1e-8, delta=-1.0, mndl=1e-12, mxdl=1e3, gtol=1e-3, epsJ=1e-7,
+              on_iteration=None):
+        """newtonhook on this problem with the tolerances of params.in scaled
+        by |x0| as newton.f90:91-95 does.  Returns (info, NewtonHook)."""
+        self.epsJ = float(epsJ)
+        d = self.dotprod_ms(-1, self.x0, self.x0)
+        tol = rel_err * math.sqrt(d)
+        scale = math.sqrt(d)
+        nh = NewtonHook(self.x0)
+
+        def sub():   # saveorbit (newton.f90:462-529): the newton.dat quantities
+            norm_x = math.sqrt(self.dotprod_ms(-1, nh.new_x, nh.new_x))
+            rec = dict(nits=nh.new_nits, gits=nh.new_gits, tol=nh.new_tol, delta=nh.new_del,
+                       rel=nh.new_tol / norm_x, norm_x=norm_x,
+                       normE_ratio=self.normE(nh.new_fx) / self.normE(nh.new_x),
+                       shifts=[tuple(nh.new_x[p * self.n:p * self.n + 3] * self.scaleT)
+                               for p in range(self.ms)])
+            self.history.append(rec)
+            if on_iteration is not None:
+                on_iteration(self, nh)
+
+        info = nh.solve(self.getrhs, lambda v: self.multJ(nh, v), lambda v: v, sub,
+                        lambda a, b: self.dotprod_ms(1, a, b), m, gtol, tol,
+                        delta * scale if delta > 0 else delta, mndl * scale, mxdl * scale, nits)
+        self.result = nh
+        return info, nh
+
+    def fields(self, x, p=0):
+        """The velocity fields of segment p of a Newton vector."""
+        return self.vec2rtz(x[p * self.n:(p + 1) * self.n])
+
+    def shifts(self, x, p=0):
+        """(sth, sz, T) of segment p, unscaled."""
+        return tuple(x[p * self.n:p * self.n + 3] * self.scaleT)

@@ -47,6 +47,70 @@ WORKLOADS = {
 EXE = {"adapt_mini": "mini", "adapt_c0": "c0", "sym_mini": "mini", "sr_mini": "mini_sr"}
 SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
 
+# steporbit (newton.f90:721-775) runs of the reference's procedures from the
+# seeded state of a workload: name: (workload, executable, [(ndts, T, sth, sz)]);
+# ndts = 1 is multJ's one step with the workload's dt, no shift.
+ORBITS = {
+    "orbit_mini": ("mini", "mini", [(20, 0.2, 0.3, 0.7), (1, 0.0, 0.0, 0.0), (25, 0.2, 0.0, -0.4)]),
+    "orbit_sr_mini": ("sr_mini", "mini_sr", [(20, 0.2, 0.0, 0.5)]),
+}
+
+
+# newtonhook (NewtonHook.f90, GMRESm.f90) of the reference on a small algebraic
+# system (oracle/newton_driver.f90; built by oracle/build_newton_ref.sh):
+# (n, m, nits, gtol, tol, del, mndl, mxdl, epsJ, amp) per case.
+NEWTON_CASES = [
+    (10, 8, 30, 1e-3, 1e-12, 0.5, 1e-10, 10.0, 1e-7, 1.5),
+    (10, 8, 40, 1e-3, 1e-12, 0.5, 1e-10, 10.0, 1e-7, 4.0),
+    (12, 6, 40, 1e-2, 1e-12, -1.0, 1e-10, 10.0, 1e-7, 3.0),
+    (10, 20, 40, 1e-3, 1e-12, 0.2, 1e-10, 0.4, 1e-7, 6.0),
+]
+
+
+def make_newton(name):
+    exe = os.path.join(REPO, "oracle", "_ref", "newton_ref")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_newton_ref.sh")
+    out = dict(cases=np.array(NEWTON_CASES, dtype=np.float64))
+    with tempfile.TemporaryDirectory() as td:
+        for i, c in enumerate(NEWTON_CASES):
+            f = os.path.join(td, f"nt{i}.txt")
+            env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+            subprocess.run([exe, f] + [repr(v) for v in c], check=True, cwd=td, env=env,
+                           stdout=subprocess.DEVNULL)
+            lines = open(f).read().strip().splitlines()
+            out[f"info_{i}"] = np.array(int(lines[-1].split()[1]))
+            out[f"iters_{i}"] = np.array([[float(v) for v in ln.split()] for ln in lines[:-1]])
+    path = os.path.join(HERE, f"golden_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+def make_orbit(name):
+    base, exe_name, cases = ORBITS[name]
+    N, K, M, Re, alpha, dt, E0, seed = WORKLOADS[base][:8]
+    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{exe_name}")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_ref.sh")
+    mesh = O.build_mesh(N)
+    ur, ut, uz = ic.seeded_perturbation(mesh.r, mesh.intrdr, K, M, alpha, seed, E0)
+    H = ur.shape[1]
+    out = dict(N=N, K=K, M=M, Re=Re, alpha=alpha, dt=dt, E0=E0, seed=seed, r=mesh.r,
+               ic=np.stack([ur, ut, uz]), cases=np.array(cases, dtype=np.float64))
+    with tempfile.TemporaryDirectory() as td:
+        icf = os.path.join(td, "ic.bin")
+        O.write_coll3(icf, ur, ut, uz)
+        for i, (ndts, T, sth, sz) in enumerate(cases):
+            pre = os.path.join(td, f"{name}_{i}")
+            env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1",
+                       OPF_ORBIT=f"{ndts} {T!r} {sth!r} {sz!r}")
+            subprocess.run([exe, icf, pre, repr(Re), repr(alpha), repr(dt), "0"], check=True, cwd=td,
+                           env=env, stdout=subprocess.DEVNULL, preexec_fn=_unlimited_stack)
+            out[f"orbit_{i}"] = np.stack(O.read_coll3(pre + "_orbit.bin", N, H))
+    path = os.path.join(HERE, f"golden_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
 
 def _unlimited_stack():
     # The reference keeps large automatic coll temporaries (e.g. velocity.f90:417)
@@ -107,5 +171,9 @@ def make(name):
 
 
 if __name__ == "__main__":
-    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini"]):
-        make(nm)
+    for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini",
+                                "orbit_mini", "orbit_sr_mini", "newton_alg"]):
+        if nm == "newton_alg":
+            make_newton(nm)
+        else:
+            (make_orbit if nm in ORBITS else make)(nm)

@@ -1,0 +1,148 @@
+"""The Newton-Krylov client's device path through the C ABI:
+opf_steporbit against the reference's own steporbit call sequence
+(golden_orbit_*: oracle/ref_driver.f90 with OPF_ORBIT), batched segments
+against the same segments run alone, opf_vel_shift against the oracle's
+var_coll_shift, and the orbit search (opf/newton.py) with the device time
+step against the same search with the oracle's CPU time step."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import opf_oracle as O
+from conftest import load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(opf, g, capacity=1, symm=False):
+    ctx = opf.Context(int(g["N"]), int(g["K"]), int(g["M"]), capacity=capacity)
+    for s in range(capacity):
+        if symm:
+            ctx.symmetry(s, shiftrefl=True)
+        ctx.setup(s, float(g["Re"]), float(g["alpha"]), float(g["dt"]))
+    return ctx
+
+
+@pytest.mark.parametrize("name,symm", [("orbit_mini", False), ("orbit_sr_mini", True)])
+def test_steporbit_matches_reference(name, symm):
+    import opf
+    g = load_golden(name)
+    ctx = _ctx(opf, g, symm=symm)
+    for i, (ndts, T, sth, sz) in enumerate(g["cases"]):
+        ctx.setup(0, float(g["Re"]), float(g["alpha"]), float(g["dt"]))   # tim_dt of the workload
+        ctx.put_state(0, *g["ic"])
+        print("steporbit", name, i, int(ndts), T, sth, sz, flush=True)
+        ctx.steporbit([0], int(ndts), T, sth, sz)
+        u = ctx.get_state(0)
+        ref = g[f"orbit_{i}"]
+        for c in range(3):
+            assert rel_err(u[c], ref[c]) <= 1e-11, (i, c)
+
+
+def test_steporbit_batched_equals_alone():
+    """Three segments with different step counts, periods and shifts in one
+    call give the same bits as each run alone."""
+    import opf
+    g = load_golden("orbit_mini")
+    cases = [(20, 0.2, 0.3, 0.7), (25, 0.2, -0.1, 0.4), (12, 0.15, 0.0, -0.2)]
+    rng = np.random.default_rng(9)
+    states = [[a * (1.0 + 0.1 * rng.standard_normal()) for a in g["ic"]] for _ in cases]
+    many = _ctx(opf, g, capacity=3)
+    for s, st in enumerate(states):
+        many.put_state(s, *st)
+    many.steporbit([0, 1, 2], [c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases],
+                   [c[3] for c in cases])
+    for s, (st, c) in enumerate(zip(states, cases)):
+        one = _ctx(opf, g)
+        one.put_state(0, *st)
+        one.steporbit([0], *c)
+        a, b = many.get_state(s), one.get_state(0)
+        for k in range(3):
+            assert np.array_equal(a[k], b[k]), (s, k)
+
+
+def test_vel_shift_matches_oracle():
+    import opf
+    g = load_golden("orbit_mini")
+    ctx = _ctx(opf, g)
+    ctx.put_state(0, *g["ic"])
+    ctx.shift(0, 0.37, -1.1)
+    u = ctx.get_state(0)
+    c = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), alpha=float(g["alpha"]))
+    for k in range(3):
+        want = c.coll_shift(g["ic"][k], 0.37, -1.1)
+        assert np.abs(u[k] - want).max() <= 1e-15 * np.abs(want).max(), k
+    ctx.shift(0, -0.37, 1.1)
+    u = ctx.get_state(0)
+    for k in range(3):
+        assert rel_err(u[k], g["ic"][k]) <= 1e-14
+
+
+class OracleStepper:
+    """steporbit of the oracle (CPU) behind the client's Stepper interface."""
+
+    def __init__(self, N, K, M, Re, alpha):
+        self.args = (N, K, M)
+        self.Re, self.alpha = Re, alpha
+
+    def steporbit(self, fields, ndts, T, sth, sz, dt1):
+        out = []
+        for f, nd, t, a, b in zip(fields, ndts, T, sth, sz):
+            c = O.Ctx(*self.args, Re=self.Re, alpha=self.alpha, dt=dt1 if nd == 1 else t / nd)
+            c.set_state(*f)
+            c.steporbit(nd, t, a, b)
+            out.append((c.ur, c.ut, c.uz))
+        return out
+
+
+def test_orbit_search_device_matches_oracle_stepper():
+    """Two Newton iterations of the orbit search at (16, 6, 4), T = 0.1 in 10
+    steps: the device-stepped search and the oracle-stepped search take the
+    same decisions (GMRES counts, trust regions) and reach the same iterate.
+    The finite-difference Jacobian amplifies rounding differences by about
+    1/epsJ, hence the 1e-5 tolerance."""
+    import opf
+    from opf.newton import DeviceStepper, OrbitNewton
+    g = load_golden("orbit_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    Re, alpha = float(g["Re"]), float(g["alpha"])
+    intrdr = O.build_mesh(N).intrdr
+    ctx = _ctx(opf, g)
+    runs = []
+    for stepper in (DeviceStepper(ctx, [0]), OracleStepper(N, K, M, Re, alpha)):
+        on = OrbitNewton(stepper, N, K, M, 1, alpha, intrdr, g["ic"], 0.0, 0.0, 0.1, [10])
+        print("orbit search with", type(stepper).__name__, flush=True)
+        info, nh = on.solve(m=8, nits=2, rel_err=1e-10, delta=-1.0, epsJ=1e-7)
+        runs.append((info, on.history, nh.new_x))
+    (i0, h0, x0), (i1, h1, x1) = runs
+    assert i0 == i1 == 2
+    assert [r["gits"] for r in h0] == [r["gits"] for r in h1]
+    for a, b in zip(h0, h1):
+        assert abs(a["tol"] - b["tol"]) <= 1e-5 * b["tol"]
+        assert abs(a["delta"] - b["delta"]) <= 1e-5 * b["delta"]
+    assert h0[-1]["tol"] < h0[0]["tol"]
+    assert np.abs(x0 - x1).max() <= 1e-5 * np.abs(x1).max()
+
+
+def test_multiple_shooting_segments_share_one_launch():
+    """ms = 2: getrhs integrates both segments in one batched call; the
+    result equals the segments stepped one at a time."""
+    import opf
+    from opf.newton import DeviceStepper, OrbitNewton
+    g = load_golden("orbit_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    alpha = float(g["alpha"])
+    intrdr = O.build_mesh(N).intrdr
+    both = _ctx(opf, g, capacity=2)
+    on2 = OrbitNewton(DeviceStepper(both, [0, 1]), N, K, M, 1, alpha, intrdr, g["ic"], 0.1, 0.2, 0.1, [10, 10])
+    f2 = on2.getrhs(on2.x0)
+    single = _ctx(opf, g)
+    n = on2.n
+    for p in range(2):
+        on1 = OrbitNewton(DeviceStepper(single, [0]), N, K, M, 1, alpha, intrdr, g["ic"], 0.1, 0.2, 0.1, [10])
+        on1.scaleU, on1.scaleT = on2.scaleU, on2.scaleT
+        yp = on1.steporbit_many([on2.x0[p * n:(p + 1) * n]], [10])[0]
+        p1 = (p + 1) % 2
+        want = yp - on2.x0[p1 * n:(p1 + 1) * n]
+        assert np.array_equal(f2[p1 * n + 8:(p1 + 1) * n], want[8:]), p

@@ -1,0 +1,141 @@
+"""The Newton-Krylov-hookstep solver of the orbit client (opf/newton.py)
+against the reference's own newtonhook / gmresm / hookstep (NewtonHook.f90,
+GMRESm.f90), compiled from its sources by oracle/build_newton_ref.sh and run
+on a small algebraic system by oracle/newton_driver.f90
+(tests/golden/golden_newton_alg.npz): every accepted iterate (Newton and
+GMRES counts, residual, trust region, x) and the exit code.  Plus the pieces
+of the orbit problem that need no time step: the scaled vector round trip,
+the shift directions against a finite shift, and the hookstep's constraint.
+No GPU is used."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+
+def _system(n, amp):
+    """oracle/newton_driver.f90's f_i(x) = x_i + 0.4 sin(2 x_{i+1}) + 0.2 x_i^3 - b_i."""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    xs = np.cos(i)
+
+    def f0(x):
+        return x + 0.4 * np.sin(2.0 * np.roll(x, -1)) + 0.2 * x ** 3
+
+    b = f0(xs)
+    return (lambda x: f0(x) - b), xs + amp * np.sin(3.0 * i)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_newtonhook_matches_reference(case):
+    from opf.newton import NewtonHook
+    g = load_golden("newton_alg")
+    n, m, nits, gtol, tol, dl, mndl, mxdl, epsJ, amp = g["cases"][case]
+    n, m, nits = int(n), int(m), int(nits)
+    f, x0 = _system(n, amp)
+    nh = NewtonHook(x0)
+    rows = []
+
+    def df(v):   # multJ's finite difference (newton.f90:411-418)
+        eps = math.sqrt(float(v @ v))
+        eps = epsJ * math.sqrt(float(nh.new_x @ nh.new_x)) / eps
+        return (f(nh.new_x + eps * v) - nh.new_fx) / eps
+
+    def sub():
+        rows.append([nh.new_nits, nh.new_gits, nh.new_tol, nh.new_del, *nh.new_x])
+
+    info = nh.solve(f, df, lambda v: v, sub, lambda a, b: float(a @ b), m, gtol, tol, dl, mndl, mxdl, nits)
+    ref = g[f"iters_{case}"]
+    got = np.array(rows)
+    assert info == int(g[f"info_{case}"])
+    assert got.shape == ref.shape
+    assert np.array_equal(got[:, :2], ref[:, :2])                  # Newton and GMRES iteration counts
+    big = ref[:, 2] > 1e-6
+    assert np.allclose(got[big, 2], ref[big, 2], rtol=1e-6, atol=0)   # residuals
+    assert np.allclose(got[:, 3], ref[:, 3], rtol=1e-6, atol=1e-12)   # trust regions
+    assert np.abs(got[:, 4:] - ref[:, 4:]).max() <= 1e-7              # iterates
+
+
+def test_hookstep_respects_trust_region():
+    """hookstep: |y| <= delta, and a delta above the unconstrained least-squares
+    step returns that step (mu -> 0 limit within its 10% growth)."""
+    from opf.newton import hookstep
+    rng = np.random.default_rng(2)
+    j = 6
+    h = np.zeros((j + 1, j))
+    for c in range(j):
+        h[:c + 2, c] = rng.standard_normal(c + 2)
+    beta = 1.7
+    for delta in (1e-3, 0.1, 1.0):
+        y, pred = hookstep(h, j, beta, delta)
+        assert np.linalg.norm(y) <= delta * (1 + 1e-12)
+        r = -(h[:j + 1, :j] @ y)
+        r[0] += beta
+        assert abs(pred - np.linalg.norm(r)) <= 1e-12 * max(1.0, pred)
+    e1 = np.zeros(j + 1)
+    e1[0] = beta
+    yls = np.linalg.lstsq(h[:j + 1, :j], e1, rcond=None)[0]
+    y, _ = hookstep(h, j, beta, 10 * np.linalg.norm(yls))
+    assert np.linalg.norm(y - yls) <= 1e-5 * np.linalg.norm(yls)
+
+
+def test_gmres_solves_linear_system():
+    """gmresm without a trust region on a well-conditioned system: the
+    residual reached is the one reported and below the tolerance."""
+    from opf.newton import Gmres
+    rng = np.random.default_rng(4)
+    n = 30
+    A = np.eye(n) * 3 + rng.standard_normal((n, n)) * 0.3
+    b = rng.standard_normal(n)
+    gm = Gmres(40, n)
+    x, res, _, its, info = gm.solve(np.zeros(n), b, lambda v: A @ v, lambda v: v, lambda a, c: float(a @ c),
+                                    1e-10 * np.linalg.norm(b), 0.0, 40, 0)
+    assert info == 0 and its <= 40
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+
+
+def _orbit_problem(ms=1):
+    import opf_oracle as O
+    from opf import ic
+    from opf.newton import OrbitNewton
+    N, K, M, alpha = 16, 6, 4, 1.25
+    me = O.build_mesh(N)
+    u = ic.seeded_perturbation(me.r, me.intrdr, K, M, alpha, 3, 1e-3)
+
+    class NoStep:
+        def steporbit(self, fields, ndts, T, sth, sz, dt1):
+            return fields
+
+    return OrbitNewton(NoStep(), N, K, M, 1, alpha, me.intrdr, [u] * ms, 0.1, 0.2, 2.0, [10] * ms), u, O
+
+
+def test_orbit_vector_round_trip_and_scales():
+    """rtz2vec / vec2rtz invert each other; the period entry is scaled to the
+    norm of the velocity part (get_scales, newton.f90:266-272)."""
+    on, u, _ = _orbit_problem()
+    v = on.rtz2vec(*u)
+    back = on.vec2rtz(v)
+    for a, b in zip(back, u):
+        assert np.abs(a - b).max() <= 1e-15 * np.abs(b).max()
+    d = on.dotprod(-1, on.x0, on.x0)
+    assert abs(on.x0[2] - math.sqrt(d)) <= 1e-14 * math.sqrt(d)
+    assert on.shifts(on.x0) == pytest.approx((0.1, 0.2, 2.0), rel=1e-15)
+
+
+def test_shift_directions_match_finite_shift():
+    """getshiftdir(i, x) is the derivative of x under a theta / z shift:
+    against the oracle's var_coll_shift by a small angle (central
+    difference; the sign follows the field u(theta - s, z - s))."""
+    on, u, O = _orbit_problem()
+    c = O.Ctx(on.N, on.K, on.M, alpha=on.alpha)
+    x = on.rtz2vec(*u)
+    h = 1e-6
+    for i, (dth, dz) in ((1, (h, 0.0)), (2, (0.0, h))):
+        fp = [c.coll_shift(a, -dth, -dz) for a in u]
+        fm = [c.coll_shift(a, dth, dz) for a in u]
+        fd = (on.rtz2vec(*fp) - on.rtz2vec(*fm)) / (2 * h)
+        sd = on.getshiftdir(i, x)
+        assert np.abs(fd[8:] - sd[8:]).max() <= 1e-6 * np.abs(sd[8:]).max()

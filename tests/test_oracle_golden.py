@@ -173,3 +173,20 @@ def test_shift_reflect_run_mini():
     assert np.max(np.abs(E - g["series"][:, 2]) / g["series"][:, 2]) <= TOL_DIAG
     for c, u in enumerate((ctx.ur, ctx.ut, ctx.uz)):
         assert rel_err(u, g["state_20"][c]) <= 1e-10, c
+
+
+@pytest.mark.parametrize("name,symm", [("orbit_mini", ()), ("orbit_sr_mini", (2,))])
+def test_steporbit_matches_reference(name, symm):
+    """steporbit (newton.f90:721-775): the reference's own call sequence
+    (oracle/ref_driver.f90, OPF_ORBIT) for new and unchanged tim_dt, with and
+    without the final shift, and in the shift-reflect subspace."""
+    g = load_golden(name)
+    for i, (ndts, T, sth, sz) in enumerate(g["cases"]):
+        c = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), Re=float(g["Re"]), alpha=float(g["alpha"]),
+                  dt=float(g["dt"]))
+        c.symm = symm
+        c.set_state(*g["ic"])
+        c.steporbit(int(ndts), T, sth, sz)
+        ref = g[f"orbit_{i}"]
+        for k, u in enumerate((c.ur, c.ut, c.uz)):
+            assert rel_err(u, ref[k]) <= 1e-12, (i, k)

@@ -268,3 +268,22 @@ def test_maxtstep_laminar_and_single_mode():
     want = np.min(dr[live] / (2.0 * np.abs(a[live])))
     assert abs(c.cfl_dt - min(want, cfl)) <= 1e-14 * want
     assert c.cfl_dir == (1 if want < cfl else 3)
+
+
+def test_coll_shift_is_a_grid_translation():
+    """var_coll_shift(dth, dz) gives the field u(theta - dth, z - dz): a shift
+    by one grid spacing of the 3/2 grid (2 pi/(alpha 3K) in z, 2 pi/(Mp 3M) in
+    theta) rolls the physical field by one point; shifting back restores it."""
+    ctx = O.Ctx(6, 4, 3, Mp=2, alpha=1.3)
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((6, ctx.H)) + 1j * rng.standard_normal((6, ctx.H))
+    a[:, 0] = a[:, 0].real
+    p = ctx.spec2phys(a)
+    Z, T = 3 * ctx.K, 3 * ctx.M
+    dz = 2 * np.pi / (ctx.alpha * Z)
+    dth = 2 * np.pi / (ctx.Mp * T)
+    assert np.allclose(ctx.spec2phys(ctx.coll_shift(a, 0.0, dz)), np.roll(p, 1, axis=1), atol=1e-12)
+    assert np.allclose(ctx.spec2phys(ctx.coll_shift(a, dth, 0.0)), np.roll(p, 1, axis=2), atol=1e-12)
+    both = ctx.coll_shift(a, 2 * dth, -3 * dz)
+    assert np.allclose(ctx.spec2phys(both), np.roll(np.roll(p, 2, axis=2), -3, axis=1), atol=1e-12)
+    assert np.allclose(ctx.coll_shift(both, -2 * dth, 3 * dz), a, atol=1e-14)
