@@ -1,11 +1,14 @@
 #!/usr/bin/env bash
-# Build the reference's Newton-Krylov-hookstep solver (test oracle).
+# Build the reference's Newton-Krylov-hookstep solver and Arnoldi routine
+# (test oracles).
 #
-# Test infrastructure only.  Compiles NewtonHook.f90 and GMRESm.f90 from the
-# sources where they lie (REF, default /root/reference) with amdflang, links
-# them with oracle/newton_driver.f90 against MKL's LAPACK (dgelsy, dgesvd;
-# /opt/conda/lib/libmkl_rt.so).  Objects go to a temporary directory; only
-# the executable lands in oracle/_ref/newton_ref.
+# Test infrastructure only.  Compiles NewtonHook.f90, GMRESm.f90 and
+# arnoldi.f from the sources where they lie (REF, default /root/reference)
+# with amdflang, links them with oracle/newton_driver.f90 and
+# oracle/arnoldi_driver.f90 against MKL's LAPACK (dgelsy, dgesvd, dgebal,
+# dgehrd, dhseqr, dhsein, dgebak; /opt/conda/lib/libmkl_rt.so).  Objects go
+# to a temporary directory; only the executables land in
+# oracle/_ref/newton_ref and oracle/_ref/arnoldi_ref.
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -21,4 +24,8 @@ $FC -O2 -c "$REF/GMRESm.f90" -o GMRESm.o
 $FC -O2 -c "$HERE/newton_driver.f90" -o newton_driver.o
 $FC -o "$OUT/newton_ref" newton_driver.o NewtonHook.o GMRESm.o -L/opt/conda/lib -lmkl_rt \
     -Wl,-rpath,/opt/conda/lib
-echo "built $OUT/newton_ref" >&2
+# newton.f90 includes arnoldi.f into free-form source
+$FC -O2 -ffree-form -c "$REF/arnoldi.f" -o arnoldi.o
+$FC -O2 -c "$HERE/arnoldi_driver.f90" -o arnoldi_driver.o
+$FC -o "$OUT/arnoldi_ref" arnoldi_driver.o arnoldi.o -L/opt/conda/lib -lmkl_rt -Wl,-rpath,/opt/conda/lib
+echo "built $OUT/newton_ref $OUT/arnoldi_ref" >&2

@@ -227,6 +227,116 @@ class NewtonHook:
 
 
 # ---------------------------------------------------------------------------
+# arnoldi.f
+# ---------------------------------------------------------------------------
+class Arnoldi:
+    """arnold (arnoldi.f:22-246) in reverse communication: call ``step(sv)``
+    with sv = A q_k (the start vector on the first call); it returns
+    (ifail, sv) with ifail 0: apply A to the returned sv and call again;
+    -1: the ncgd eigenvalues of largest magnitude converged (consecutive
+    Hessenberg estimates closer than tol), ``wr2``/``wi2`` hold them and
+    ``b`` the eigenvectors (a real column per real eigenvalue, the real and
+    imaginary parts for a complex pair); 1: kmax reached; 2: invariant
+    subspace.  Gram-Schmidt with one reorthogonalisation, as arnold does."""
+
+    SMALL = 1e-80
+
+    def __init__(self, n, kmax, ncgd, dp, tol):
+        self.n, self.kmax, self.ncgd, self.dp, self.tol = n, kmax, ncgd, dp, tol
+        self.k = 0
+        self.q = np.zeros((kmax, n))
+        self.h = np.zeros((kmax, kmax))
+        self.wr2 = np.zeros(ncgd)
+        self.wi2 = np.zeros(ncgd)
+        self.b = np.zeros((2 * ncgd, n))
+
+    def step(self, sv):
+        k, q, h = self.k, self.q, self.h
+        if k >= self.kmax:
+            return 1, sv
+        q[k] = sv
+        for j in range(k):                  # modified Gram-Schmidt
+            h[j, k - 1] = self.dp(q[k], q[j])
+            q[k] = q[k] - h[j, k - 1] * q[j]
+        for j in range(k):                  # reorthogonalise
+            nm = self.dp(q[k], q[j])
+            q[k] = q[k] - nm * q[j]
+            h[j, k - 1] = h[j, k - 1] + nm
+        norm = math.sqrt(self.dp(q[k], q[k]))
+        if norm < self.SMALL:
+            return 2, sv
+        q[k] = q[k] / norm
+        sv = q[k].copy()
+        ifail = 0
+        if k == 0:
+            h[:] = 0.0
+        else:
+            h[k, k - 1] = norm
+            if k >= self.ncgd:
+                ifail = self._eigen(k)
+        self.k = k + 1
+        return ifail, sv
+
+    def _eigen(self, k):
+        """Eigenvalues of the k x k Hessenberg matrix (LAPACK's balanced QR),
+        the ncgd largest in magnitude compared with the last estimates."""
+        w, vr = scipy.linalg.eig(self.h[:k, :k])
+        wr, wi = w.real, w.imag
+        sel = np.zeros(k, dtype=bool)
+        p = 0
+        picked = []
+        for i in range(self.ncgd):
+            best, m = -1.0, -1
+            for j in range(k):
+                if not sel[j]:
+                    nm = wr[j] * wr[j] + wi[j] * wi[j]
+                    if nm > best:
+                        best, m = nm, j
+            sel[m] = True
+            if abs(wi[m]) > self.SMALL:     # the conjugate partner
+                partner = m + 1 if m + 1 < k and wi[m + 1] == -wi[m] else int(np.argmin(np.abs(w - np.conj(w[m]))))
+                sel[partner] = True
+            if math.sqrt((wr[m] - self.wr2[i]) ** 2 + (wi[m] - self.wi2[i]) ** 2) < self.tol:
+                p += 1
+            self.wr2[i], self.wi2[i] = wr[m], wi[m]
+            picked.append(m)
+        if p < self.ncgd:
+            return 0
+        cols = []
+        Qk = self.q[:k]
+        for m in picked:
+            v = vr[:, m]
+            if abs(wi[m]) > self.SMALL:
+                if wi[m] < 0:
+                    v = np.conj(v)
+                cols.append(v.real @ Qk)
+                cols.append(v.imag @ Qk)
+            else:
+                cols.append(v.real @ Qk)
+        self.b = np.array(cols)
+        return -1
+
+
+def logeig(wr, wi, t):
+    """logeig (arnoldi.f:256-302): Floquet multipliers wr + i wi = e^{sigma t}
+    to exponents sigma; the angle lies in [-pi, pi) (a negative real
+    multiplier gives -pi) as in arnoldi.f."""
+    sr, si = np.zeros(len(wr)), np.zeros(len(wr))
+    for i, (re, im) in enumerate(zip(wr, wi)):
+        if abs(re) < Arnoldi.SMALL:
+            if abs(im) < Arnoldi.SMALL:
+                raise ValueError("logeig: zero multiplier")
+            th = 0.5 * D_PI if im > 0 else -0.5 * D_PI
+        elif re > 0:
+            th = math.atan(im / re)
+        else:
+            th = D_PI + math.atan(im / re) if im > 0 else -D_PI + math.atan(im / re)
+        sr[i] = math.log(math.sqrt(re * re + im * im)) / t
+        si[i] = th / t
+    return sr, si
+
+
+# ---------------------------------------------------------------------------
 # newton.f90: the orbit problem
 # ---------------------------------------------------------------------------
 class DeviceStepper:
@@ -250,6 +360,14 @@ class DeviceStepper:
         ctx.steporbit(sl, ndts, T, sth, sz)
         return [ctx.get_state(s)[:3] for s in sl]
 
+    def imposesym(self, fields):
+        """vel_imposesym (velocity.f90:61-74) of host fields with the first
+        slot's symmetries."""
+        s = self.slots[0]
+        self.ctx.put_state(s, *fields)
+        self.ctx.imposesym(s)
+        return self.ctx.get_state(s)[:3]
+
 
 class OrbitNewton:
     """newton.f90's search for a relative periodic orbit on ms shooting
@@ -260,7 +378,7 @@ class OrbitNewton:
     shift, newton.f90:448-450); T_fixed freezes the period."""
 
     def __init__(self, stepper, N, K, M, Mp, alpha, intrdr, x0, sth, sz, T, ndts, T_fixed=False,
-                 freeze_theta_shift=False):
+                 freeze_theta_shift=False, r=None):
         self.stepper = stepper
         self.N, self.K, self.M, self.Mp, self.alpha = N, K, M, Mp, float(alpha)
         self.H = (2 * K - 1) * M - (K - 1)
@@ -275,6 +393,11 @@ class OrbitNewton:
         self._k, self._m = ks, ms_
         # rtz2vec weights (newton.f90:572-580): d = sqrt|intrdr|, w(nh)
         self._d = np.sqrt(np.abs(self.intrdr))
+        if r is None:
+            from .engine import host_mesh
+            r = host_mesh(N)["r"]
+        r = np.asarray(r, dtype=np.float64)
+        self._r2 = r * r                                  # mes_D%r(:,2)
         self._w = 4.0 / (4.0 + (self.alpha * np.abs(ks)) + (float(Mp) * np.abs(ms_)))
         states = [x0] if isinstance(x0[0], np.ndarray) and np.ndim(x0[0]) == 2 else list(x0)
         self.get_scales(*states[0])
@@ -439,6 +562,78 @@ class OrbitNewton:
                         delta * scale if delta > 0 else delta, mndl * scale, mxdl * scale, nits)
         self.result = nh
         return info, nh
+
+    # -- Floquet analysis (getEigen, newton.f90:850-968) -------------------------
+    def multA(self, x, y0, epsA):
+        """multA (newton.f90:814-842): the linearised map over all segments
+        about new_x by finite differences."""
+        n = self.n
+        y = x.copy()
+        for p in range(self.ms):
+            xp = self.xn[p * n:(p + 1) * n]
+            eps = math.sqrt(self.dotprod(-1, y, y))
+            if eps == 0.0:
+                raise RuntimeError("multA: eps=0 (1)")
+            eps = epsA * math.sqrt(self.dotprod(-1, xp, xp)) / eps
+            if eps == 0.0:
+                raise RuntimeError("multA: eps=0 (2)")
+            y = xp + eps * y
+            y[:8] = xp[:8]
+            y = self.steporbit_many([y], [self.ndts[p]])[0]
+            y = (y - y0[p * n:(p + 1) * n]) / eps
+        return y
+
+    def get_eigen(self, x=None, ncgd=4, kmax=60, epsA=1e-7, rel_err=1e-8, gtol=1e-3):
+        """getEigen (newton.f90:850-968): the ncgd leading Floquet multipliers
+        of the orbit x (default: the last Newton iterate) by Arnoldi on multA.
+        Returns dict(k, ifail, multipliers (complex), exponents (complex),
+        vectors: (ur, ut, uz) per eigenvector column scaled to the orbit's
+        energy as the saved states are)."""
+        n = self.n
+        self.xn = np.array(self.result.new_x if x is None else x, dtype=np.float64)
+        rerr = min(rel_err, gtol)
+        t = float(np.sum(self.xn[2::n][:self.ms])) * self.scaleT
+        y0 = np.concatenate(self.steporbit_many([self.xn[p * n:(p + 1) * n] for p in range(self.ms)],
+                                                self.ndts))
+        # initial input: the orbit's state shifted by 0.3 of each period, with
+        # (1 - r^2) r^2 put into harmonics (1, 0) and (0, 1) of u_r and u_theta
+        ur, ut, uz = self.vec2rtz(self.xn[:n])
+        st, sz = 0.3 * (2.0 * D_PI), 0.3 * (2.0 * D_PI / self.alpha)
+        ph = np.exp(-1j * ((sz * self._k.astype(float)) * self.alpha)) * \
+            np.exp(-1j * ((st * self._m.astype(float)) * float(self.Mp)))
+        ur, ut, uz = (a * ph[None, :] for a in (ur, ut, uz))
+        r2 = self._r2
+        prof = (1.0 - r2) * r2
+        for h in (1, 2 * self.K - 1):    # var_harmonic(1, 0), var_harmonic(0, 1)
+            ur[:, h] = prof + 1j * ur[:, h].imag
+            ut[:, h] = prof + 1j * ut[:, h].imag
+        if hasattr(self.stepper, "imposesym"):
+            ur, ut, uz = self.stepper.imposesym((ur, ut, uz))
+        sv = self.rtz2vec(ur, ut, uz)
+        ar = Arnoldi(n, kmax, ncgd, lambda a, b: self.dotprod(-1, a, b), rerr)
+        while True:
+            sv[:8] = 0.0
+            ifail, sv = ar.step(sv)
+            if ifail != 0:
+                break
+            sv = self.multA(sv, y0, epsA)
+        mult = ar.wr2 + 1j * ar.wi2
+        # converged (-1) or at kmax (1): the last estimates, as getEigen writes them
+        sr, si = logeig(ar.wr2, ar.wi2, t) if ifail in (-1, 1) else (np.full(ncgd, np.nan), np.full(ncgd, np.nan))
+        E0 = sum(energy([a], self.intrdr, self.K, self.M, self.alpha) for a in self.vec2rtz(self.xn[:n]))
+        vecs = []
+        if ifail == -1:
+            j = 0
+            for i in range(ncgd):
+                cols = ar.b[j:j + (2 if ar.wi2[i] != 0.0 else 1)]
+                E = sum(energy([a], self.intrdr, self.K, self.M, self.alpha)
+                        for c in cols for a in self.vec2rtz(c))
+                for c in cols:
+                    v = c * math.sqrt(E0 / E) if E > 0.0 else c
+                    vecs.append(self.vec2rtz(v))
+                j += len(cols)
+        return dict(k=ar.k, ifail=ifail, t=t, multipliers=mult, exponents=sr + 1j * si, vectors=vecs,
+                    arnoldi=ar)
 
     def fields(self, x, p=0):
         """The velocity fields of segment p of a Newton vector."""

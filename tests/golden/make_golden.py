@@ -86,6 +86,48 @@ def make_newton(name):
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
 
 
+def arnoldi_matrix(n=40, seed=5):
+    """A = Q T Q^T with a known dominant spectrum: 0.95 e^{+-0.4i}, 0.9,
+    0.8 e^{+-1.1i}, 0.7, the rest in [0, 0.5); T upper block triangular."""
+    rng = np.random.default_rng(seed)
+    T = np.triu(rng.standard_normal((n, n)) * 0.05, 1)
+
+    def blk(r, th):
+        return np.array([[r * np.cos(th), -r * np.sin(th)], [r * np.sin(th), r * np.cos(th)]])
+
+    T[0:2, 0:2] = blk(0.95, 0.4)
+    T[2, 2] = 0.9
+    T[3:5, 3:5] = blk(0.8, 1.1)
+    T[5, 5] = 0.7
+    for i in range(6, n):
+        T[i, i] = 0.5 * rng.random()
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return Q @ T @ Q.T
+
+
+def make_arnoldi(name):
+    """arnold + logeig of the reference (oracle/arnoldi_driver.f90) on a fixed matrix."""
+    exe = os.path.join(REPO, "oracle", "_ref", "arnoldi_ref")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_newton_ref.sh")
+    n, kmax, ncgd, tol = 40, 30, 4, 1e-10
+    A = arnoldi_matrix(n)
+    out = dict(A=A, n=n, kmax=kmax, ncgd=ncgd, tol=tol)
+    with tempfile.TemporaryDirectory() as td:
+        fa = os.path.join(td, "A.bin")
+        A.T.astype("<f8").tofile(fa)                       # column-major
+        env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+        subprocess.run([exe, fa, str(n), str(kmax), str(ncgd), repr(tol), "ar.txt", "ar.bin"], check=True,
+                       cwd=td, env=env, stdout=subprocess.DEVNULL)
+        lines = open(os.path.join(td, "ar.txt")).read().split("\n")
+        out["k"], out["ifail"] = (np.array(int(v)) for v in lines[0].split())
+        out["eig"] = np.array([[float(v) for v in ln.split()] for ln in lines[1:1 + ncgd]])
+        out["vecs"] = np.fromfile(os.path.join(td, "ar.bin"), dtype="<f8").reshape(2 * ncgd, n)
+    path = os.path.join(HERE, f"golden_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
 def make_orbit(name):
     base, exe_name, cases = ORBITS[name]
     N, K, M, Re, alpha, dt, E0, seed = WORKLOADS[base][:8]
@@ -172,8 +214,10 @@ def make(name):
 
 if __name__ == "__main__":
     for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini",
-                                "orbit_mini", "orbit_sr_mini", "newton_alg"]):
+                                "orbit_mini", "orbit_sr_mini", "newton_alg", "arnoldi_mat"]):
         if nm == "newton_alg":
             make_newton(nm)
+        elif nm == "arnoldi_mat":
+            make_arnoldi(nm)
         else:
             (make_orbit if nm in ORBITS else make)(nm)

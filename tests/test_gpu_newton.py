@@ -146,3 +146,75 @@ def test_multiple_shooting_segments_share_one_launch():
         p1 = (p + 1) % 2
         want = yp - on2.x0[p1 * n:(p1 + 1) * n]
         assert np.array_equal(f2[p1 * n + 8:(p1 + 1) * n], want[8:]), p
+
+
+def test_floquet_arnoldi_device_matches_oracle_stepper():
+    """getEigen's Arnoldi on multA (newton.f90:814-905) about the seeded state
+    for T = 0.1 in 10 steps, 16 iterations with the device time step and with
+    the oracle's: the same Hessenberg matrix and multiplier estimates (finite
+    differences amplify rounding by about 1/epsA: 1e-5)."""
+    import opf
+    from opf.newton import DeviceStepper, OrbitNewton
+    g = load_golden("orbit_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    Re, alpha = float(g["Re"]), float(g["alpha"])
+    me = O.build_mesh(N)
+    ctx = _ctx(opf, g)
+    res = []
+    for stepper in (DeviceStepper(ctx, [0]), OracleStepper(N, K, M, Re, alpha)):
+        on = OrbitNewton(stepper, N, K, M, 1, alpha, me.intrdr, g["ic"], 0.0, 0.0, 0.1, [10], r=me.r)
+        print("getEigen with", type(stepper).__name__, flush=True)
+        res.append(on.get_eigen(on.x0, ncgd=2, kmax=16, epsA=1e-7, rel_err=1e-7))
+    a, b = res
+    assert a["ifail"] == b["ifail"] == 1 and a["k"] == b["k"] == 16
+    ha, hb = a["arnoldi"].h, b["arnoldi"].h
+    assert np.abs(ha - hb).max() <= 1e-5 * np.abs(hb).max()
+    assert np.abs(a["multipliers"] - b["multipliers"]).max() <= 1e-5
+
+
+def test_floquet_eigenpairs_satisfy_the_device_map():
+    """A converged getEigen on the device about a near-laminar state (energy
+    1e-10 at Re = 100, so the map is the linear evolution about Hagen-
+    Poiseuille flow; T = 1 in 100 steps): every leading multiplier lies
+    inside the unit circle (the flow is linearly stable), and each leading
+    multiplier mu and its eigenvector v satisfy multA v = mu v, checked with
+    one more device integration.  The finite-difference map carries noise of
+    about 1e-7 relative, so the eigenvalue tolerance is 1e-6 and the
+    eigenpair residual is checked to 1e-3."""
+    import opf
+    from opf import ic
+    from opf.newton import DeviceStepper, OrbitNewton
+    N, K, M, Re, alpha = 16, 6, 4, 100.0, 1.25
+    me = O.build_mesh(N)
+    u = ic.seeded_perturbation(me.r, me.intrdr, K, M, alpha, 4, 1e-10)
+    ctx = opf.Context(N, K, M)
+    ctx.setup(0, Re, alpha, 0.01)
+    on = OrbitNewton(DeviceStepper(ctx, [0]), N, K, M, 1, alpha, me.intrdr, u, 0.0, 0.0, 1.0, [100], r=me.r)
+    r = on.get_eigen(on.x0, ncgd=2, kmax=100, epsA=1e-7, rel_err=1e-6)
+    print("getEigen:", r["k"], r["ifail"], r["multipliers"], flush=True)
+    assert r["ifail"] == -1
+    assert np.all(np.abs(r["multipliers"]) < 1.0)
+    ar = r["arnoldi"]
+    y0 = on.steporbit_many([on.x0], [100])[0]
+    j = 0
+    for i in range(2):
+        mu = ar.wr2[i] + 1j * ar.wi2[i]
+        if ar.wi2[i] != 0.0:
+            vr, vi = ar.b[j], ar.b[j + 1]
+            j += 2
+            Avr, Avi = on.multA(vr, y0, 1e-7), on.multA(vi, y0, 1e-7)
+            res = (Avr + 1j * Avi) - mu * (vr + 1j * vi)
+            scale = np.linalg.norm((vr + 1j * vi)[8:])
+        else:
+            v = ar.b[j]
+            j += 1
+            res = on.multA(v, y0, 1e-7) - mu * v
+            scale = np.linalg.norm(v[8:])
+        print("eigenpair", i, mu, np.linalg.norm(res[8:]) / scale, flush=True)
+        assert np.linalg.norm(res[8:]) <= 1e-3 * scale, i
+    assert np.allclose(r["exponents"].real, np.log(np.abs(r["multipliers"])) / 1.0, rtol=1e-12)
+    # the least damped Stokes mode of the pipe: sigma = -j_{1,1}^2 / Re (the
+    # axisymmetric swirl, and the k = 0, m = 1 mode of the same rate), to the
+    # discretisation error at N = 16, dt = 0.01
+    j11 = 3.8317059702075123
+    assert np.abs(r["exponents"].real + j11 * j11 / Re).max() <= 1e-4 * j11 * j11 / Re

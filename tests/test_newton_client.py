@@ -139,3 +139,53 @@ def test_shift_directions_match_finite_shift():
         fd = (on.rtz2vec(*fp) - on.rtz2vec(*fm)) / (2 * h)
         sd = on.getshiftdir(i, x)
         assert np.abs(fd[8:] - sd[8:]).max() <= 1e-6 * np.abs(sd[8:]).max()
+
+
+def test_arnoldi_matches_reference():
+    """arnold + logeig (arnoldi.f) of the reference on a fixed matrix with a
+    known dominant spectrum (golden_arnoldi_mat): the same iteration count
+    and exit code, the eigenvalues and exponents to 1e-12, and eigenvectors
+    that span the same directions (each compared after removing its complex
+    scale, which inverse iteration and the QR algorithm fix differently)."""
+    from opf.newton import Arnoldi, logeig
+    g = load_golden("arnoldi_mat")
+    A, n, kmax, ncgd, tol = g["A"], int(g["n"]), int(g["kmax"]), int(g["ncgd"]), float(g["tol"])
+    ar = Arnoldi(n, kmax, ncgd, lambda a, b: float(a @ b), tol)
+    sv = 1.0 + np.sin(np.arange(1, n + 1, dtype=np.float64))
+    while True:
+        ifail, sv = ar.step(sv)
+        if ifail != 0:
+            break
+        sv = A @ sv
+    assert (ar.k, ifail) == (int(g["k"]), int(g["ifail"]))
+    eig = g["eig"]
+    assert np.abs(ar.wr2 - eig[:, 0]).max() <= 1e-12
+    assert np.abs(ar.wi2 - eig[:, 1]).max() <= 1e-12
+    sr, si = logeig(ar.wr2, ar.wi2, 1.0)
+    assert np.abs(sr - eig[:, 2]).max() <= 1e-12 and np.abs(si - eig[:, 3]).max() <= 1e-12
+    ref = g["vecs"]
+    col = 0
+    for i in range(ncgd):
+        if abs(eig[i, 1]) > 0:
+            v, w = ar.b[col] + 1j * ar.b[col + 1], ref[col] + 1j * ref[col + 1]
+            col += 2
+        else:
+            v, w = ar.b[col] + 0j, ref[col] + 0j
+            col += 1
+        c = np.vdot(v, w) / np.vdot(v, v)
+        assert np.abs(c * v - w).max() <= 1e-8 * np.abs(w).max(), i
+        lam = eig[i, 0] + 1j * eig[i, 1]
+        assert np.abs(A @ v - lam * v).max() <= 1e-8 * np.abs(v).max(), i
+
+
+def test_logeig_quadrants():
+    """logeig's angle equals atan2 in all four quadrants and on the axes,
+    except on the negative real axis, where arnoldi.f:296-300 gives -pi."""
+    from opf.newton import logeig
+    wr = np.array([0.5, -0.5, -0.5, 0.5, 0.0, 0.0, -0.7])
+    wi = np.array([0.3, 0.3, -0.3, -0.3, 0.2, -0.2, 0.0])
+    sr, si = logeig(wr, wi, 2.0)
+    want = np.arctan2(wi, wr)
+    want[-1] = -np.pi
+    assert np.allclose(si, want / 2.0, atol=1e-15)
+    assert np.allclose(sr, np.log(np.hypot(wr, wi)) / 2.0, atol=1e-15)
