@@ -102,6 +102,15 @@ int opf_slot_adaptive(opf_ctx* ctx, int slot, int enable, double maxdt, double c
  * and in opf_vel_imposesym. */
 int opf_slot_symmetry(opf_ctx* ctx, int slot, int mirror, int shiftrefl, int shiftrott);
 
+/* ---- forcing plugins (var_null(2) after vel_nonlinear, main.f90:23,38) ------ */
+
+/* The Coriolis force of the Coriolis plugin (Coriolis.f90:54-71, parameters of
+ * coriolis.in): N += -1/(4 Ro) Omega^ x (U + u) with Ro = Re Ek0 and Omega^ =
+ * (sin(phi) cos(theta), -sin(phi) sin(theta), cos(phi)), evaluated on the
+ * physical grid inside the nonlinear products of every evaluation.  Needs
+ * Mp = 1.  enable = 0 removes it. */
+int opf_slot_coriolis(opf_ctx* ctx, int slot, int enable, double Ek0, double phi);
+
 /* tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir of the slot (the columns of
  * io_write_timestep, io.f90:645-657). */
 int opf_slot_timestep(opf_ctx* ctx, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir);

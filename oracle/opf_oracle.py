@@ -854,7 +854,9 @@ class Ctx:
         self.p_cz = self.spec2phys(c3)
 
     def nonlinear(self):
-        """vel_nonlinear + vel_addHPF (velocity.f90:323-361)."""
+        """vel_nonlinear + vel_addHPF (velocity.f90:323-361), then the
+        var_null(2) hook of main.f90:23,38 (the Coriolis plugin when
+        self.coriolis = (Ek0, phi))."""
         p = self.p_t * self.p_cz - self.p_z * self.p_ct
         self.Nr = self.phys2spec(p)
         p = self.p_z * self.p_cr - self.p_r * self.p_cz
@@ -862,6 +864,31 @@ class Ctx:
         p = self.p_r * self.p_ct - self.p_t * self.p_cr
         self.Nz = self.phys2spec(p)
         self.add_hpf()
+        if getattr(self, "coriolis", None) is not None:
+            self.add_coriolis(*self.coriolis)
+
+    def add_coriolis(self, Ek0, phi):
+        """util(2) of Coriolis.f90:54-71: N += -1/(4 Ro) Omega^ x (U + u) on
+        the physical grid, Ro = Re Ek0, Omega^ = (sin phi cos theta,
+        -sin phi sin theta, cos phi); each component transformed and added."""
+        if self.Mp != 1:
+            raise ValueError("add_coriolis: i_Mp/=1")
+        T = 3 * self.M
+        d1 = 2.0 * D_PI / float(T)
+        th = d1 * np.arange(T, dtype=np.float64)
+        Or = math.sin(phi) * np.cos(th)
+        Ot = -math.sin(phi) * np.sin(th)
+        Oz = math.cos(phi)
+        Ro = self.Re * Ek0
+        d1 = -1.0 / (4.0 * Ro)
+        U = self.U[:, None, None]
+        uz = self.p_z + U
+        Nr = d1 * (Ot[None, None, :] * uz - Oz * self.p_t)
+        Nt = d1 * (Oz * self.p_r - Or[None, None, :] * uz)
+        Nz = d1 * (Or[None, None, :] * self.p_t - Ot[None, None, :] * self.p_r)
+        self.Nr = self.Nr + self.phys2spec(Nr)
+        self.Nt = self.Nt + self.phys2spec(Nt)
+        self.Nz = self.Nz + self.phys2spec(Nz)
 
     def add_hpf(self):
         """vel_addHPF (velocity.f90:338-361)."""

@@ -94,6 +94,8 @@ struct opf_ctx {
   std::vector<int> symm;       // symmetry bits per slot (SlotParams::symm)
   int n_symm = 0;              // slots with a symmetry imposed
   std::vector<double> alpha_h; // d_alpha per slot (host copy of SlotParams::alpha)
+  struct Cor { int on = 0; double Ek0 = 0.0, phi = 0.0; };
+  std::vector<Cor> cor;        // coriolis.in per slot (Coriolis.f90:36-39)
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
@@ -401,6 +403,15 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->adapt.assign(cap, opf_ctx::Adapt{});
   c->symm.assign(cap, 0);
   c->alpha_h.assign(cap, 0.0);
+  c->cor.assign(cap, opf_ctx::Cor{});
+}
+
+// The Coriolis parameters of Coriolis.f90:49-50, 58: Ro = Re Ek0, d1 = -1/(4 Ro).
+void set_coriolis(SlotParams& p, const opf_ctx::Cor& k) {
+  p.coriolis = k.on;
+  p.cor_d1 = k.on ? -1.0 / (4.0 * (p.Re * k.Ek0)) : 0.0;
+  p.cor_sphi = k.on ? std::sin(k.phi) : 0.0;
+  p.cor_cphi = k.on ? std::cos(k.phi) : 0.0;
 }
 
 void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double implicit, int cf) {
@@ -414,6 +425,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   p.adaptive = c->adapt[slot].on;        // d_timestep < 0 (parameters.f90:38)
   p.maxdt = c->adapt[slot].maxdt;        // d_maxdt (parameters.f90:39)
   p.symm = c->symm[slot];                // b_mirrorsym, b_shiftrefl, b_shiftrott
+  set_coriolis(p, c->cor[slot]);
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
   c->alpha_h[slot] = alpha;
   auto ctl = read_ctl(c);
@@ -746,6 +758,24 @@ int opf_slot_symmetry(opf_ctx* c, int slot, int mirror, int shiftrefl, int shift
       SlotParams p{};
       HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
       p.symm = bits;
+      HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
+    }
+  });
+}
+
+int opf_slot_coriolis(opf_ctx* c, int slot, int enable, double Ek0, double phi) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (enable && c->Mp != 1) throw OpfError(OPF_ERR_ARG, "add_coriolis: i_Mp/=1");
+    if (enable && !(Ek0 > 0.0)) throw OpfError(OPF_ERR_ARG, "Ek0 must be > 0");
+    opf_ctx::Cor& k = c->cor[slot];
+    k.on = enable ? 1 : 0;
+    k.Ek0 = enable ? Ek0 : 0.0;
+    k.phi = enable ? phi : 0.0;
+    if (c->setup[slot]) {
+      SlotParams p{};
+      HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+      set_coriolis(p, k);
       HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
     }
   });

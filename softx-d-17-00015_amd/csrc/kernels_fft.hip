@@ -427,6 +427,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   const double scale = 1.0 / (double)(G.Z * TT);
   const double rn = G.mesh[MeshOff::make(G.N).r + n];
   const double ubase = __dsub_rn(1.0, __dmul_rn(rn, rn));
+  const bool cor = B.prm[slot].coriolis != 0;
+  const double cor_d1 = B.prm[slot].cor_d1, cor_s = B.prm[slot].cor_sphi, cor_c = B.prm[slot].cor_cphi;
   double mr = 0.0, mt = 0.0, mz = 0.0;
   for (int i = threadIdx.x; i < tz * TT; i += FFT_THREADS) {
     const int line = i / TT, th = i - line * TT;
@@ -436,9 +438,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     mr = fmax(mr, fabs(ur));
     mt = fmax(mt, fabs(ut));
     mz = fmax(mz, fabs(__dadd_rn(uz, ubase)));
-    const double Nr = ut * cz - uz * ct;
-    const double Nt = uz * cr - ur * cz;
-    const double Nz = ur * ct - ut * cr;
+    double Nr = ut * cz - uz * ct;
+    double Nt = uz * cr - ur * cz;
+    double Nz = ur * ct - ut * cr;
+    if (cor) {   // Coriolis.f90:62-68: N += -1/(4 Ro) Omega^ x (U + u)
+      const double2 cs = twl[th];   // (cos, sin)(2 pi th / T)
+      const double Or = cor_s * cs.x, Ot = -cor_s * cs.y;
+      const double uzU = uz + ubase;
+      Nr += cor_d1 * (Ot * uzU - cor_c * ut);
+      Nt += cor_d1 * (cor_c * ur - Or * uzU);
+      Nz += cor_d1 * (Or * ut - Ot * ur);
+    }
     L0[0] = make_double2(scale * Nr, scale * Nt);
     L0[(size_t)tz * Tp] = make_double2(scale * Nz, 0.0);
   }

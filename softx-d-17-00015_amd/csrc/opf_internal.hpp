@@ -46,7 +46,9 @@ struct SlotParams {
   int adaptive;   // d_timestep < 0: vel_maxtstep + tim_new_tstep before every predictor
   double maxdt;   // d_maxdt (parameters.f90:39)
   int symm;       // bits 0, 1, 2: b_mirrorsym, b_shiftrefl, b_shiftrott (parameters.f90:26-28)
-  int pad;
+  int coriolis;   // 1: the Coriolis force of Coriolis.f90:54-71 is added to N
+  double cor_d1;  // -1 / (4 Ro), Ro = Re Ek0
+  double cor_sphi, cor_cphi;   // sin(phi), cos(phi): the rotation axis in the (x, z) plane
 };
 
 // Slot modes for the next step kernel.

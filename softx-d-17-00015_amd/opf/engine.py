@@ -307,6 +307,12 @@ class Context:
         check(self.lib.opf_slot_symmetry(self.h, slot, int(bool(mirror)), int(bool(shiftrefl)),
                                          int(bool(shiftrott))))
 
+    def coriolis(self, slot, Ek0=None, phi=0.0):
+        """The Coriolis plugin (Coriolis.f90:54-71) for the slot: Ekman number
+        Ek0 and angle phi between pipe and rotation axes; Ek0=None removes it."""
+        on = Ek0 is not None
+        check(self.lib.opf_slot_coriolis(self.h, slot, int(on), float(Ek0 or 0.0), float(phi)))
+
     def timestep(self, slot):
         """tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)."""
         dt, cd, cf, di = C.c_double(), C.c_double(), C.c_double(), C.c_int()

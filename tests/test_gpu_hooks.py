@@ -81,3 +81,36 @@ def test_add_nonlinear_is_added_to_the_evaluation():
     n2 = ctx.get_nonlinear(0)
     for c in range(3):
         assert np.max(np.abs(n2[c] - (n1[c] + d[c]))) <= 1e-15 * max(1.0, np.abs(n1[c]).max())
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.6])
+def test_coriolis_plugin_matches_oracle(phi):
+    """The Coriolis plugin (Coriolis.f90:54-71) fused into the device's
+    products against the oracle's util(2) after vel_nonlinear: 1 step to
+    1e-12 and 10 steps to 1e-10 at (16, 6, 4), Ek0 = 0.01 (a force of
+    order 1/(4 Re Ek0) ~ 0.01 against N of order 1e-3)."""
+    import opf_oracle as O
+    g = load_golden("mini")
+    ctx = _ctx(g)
+    ctx.coriolis(0, Ek0=0.01, phi=phi)
+    ctx.put_state(0, *g["ic"])
+    o = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), Re=float(g["Re"]), alpha=float(g["alpha"]),
+              dt=float(g["dt"]))
+    o.coriolis = (0.01, phi)
+    o.set_state(*g["ic"])
+    for n, tol in ((1, 1e-12), (10, 1e-10)):
+        print("coriolis", phi, n, flush=True)
+        ctx.run(n - o.step)
+        o.run(n)
+        u = ctx.get_state(0)
+        for c, ref in enumerate((o.ur, o.ut, o.uz)):
+            assert rel_err(u[c], ref) <= tol, (n, c)
+    # the plugin changes the flow: against the run without it
+    plain = _ctx(g)
+    plain.run(10)
+    assert rel_err(plain.get_state(0)[0], o.ur) > 1e-6
+    ctx.coriolis(0, None)                      # removed again
+    with pytest.raises(Exception):
+        import opf
+        bad = opf.Context(16, 6, 4, Mp=2)
+        bad.coriolis(0, Ek0=0.01)              # io: 'add_coriolis: i_Mp/=1'

@@ -287,3 +287,48 @@ def test_coll_shift_is_a_grid_translation():
     both = ctx.coll_shift(a, 2 * dth, -3 * dz)
     assert np.allclose(ctx.spec2phys(both), np.roll(np.roll(p, 2, axis=2), -3, axis=1), atol=1e-12)
     assert np.allclose(ctx.coll_shift(both, -2 * dth, 3 * dz), a, atol=1e-14)
+
+
+def _coriolis_delta(ctx, ur, ut, uz, Ek0, phi):
+    """The term util(2) adds, alone (the N it is added to set to zero)."""
+    ctx.set_state(ur, ut, uz)
+    ctx.transform()
+    ctx.Nr, ctx.Nt, ctx.Nz = ctx.zeros(), ctx.zeros(), ctx.zeros()
+    ctx.add_coriolis(Ek0, phi)
+    return ctx.Nr, ctx.Nt, ctx.Nz
+
+
+def test_coriolis_about_the_pipe_axis_is_a_rotation_of_u():
+    """phi = 0 (rotation about the pipe axis): Omega^ x (U + u) = (-u_t, u_r, 0),
+    so the added N is -1/(4 Ro) (-u_t, u_r, 0) harmonic by harmonic."""
+    ctx = O.Ctx(8, 4, 3, Re=1500.0, alpha=1.1)
+    rng = np.random.default_rng(12)
+    u = [rng.standard_normal((8, ctx.H)) + 1j * rng.standard_normal((8, ctx.H)) for _ in range(3)]
+    O.Ctx.imposesym0(*u)
+    Ek0 = 0.02
+    d1 = -1.0 / (4.0 * 1500.0 * Ek0)
+    dr, dt_, dz = _coriolis_delta(ctx, *u, Ek0, 0.0)
+    scale = abs(d1) * max(np.abs(u[0]).max(), np.abs(u[1]).max())
+    assert np.abs(dr - (-d1) * u[1]).max() <= 1e-13 * scale
+    assert np.abs(dt_ - d1 * u[0]).max() <= 1e-13 * scale
+    assert np.abs(dz).max() <= 1e-13 * scale
+
+
+def test_coriolis_on_laminar_flow_is_a_single_harmonic():
+    """u = 0, tilted axis: the force d1 (Ot U, -Or U, 0) with Or = sin(phi)
+    cos(theta), Ot = -sin(phi) sin(theta) lives in harmonic (k, m) = (0, 1):
+    N_r = i d1 sin(phi) U / 2, N_t = -d1 sin(phi) U / 2, N_z = 0."""
+    ctx = O.Ctx(10, 3, 4, Re=2000.0, alpha=1.25)
+    z = ctx.zeros()
+    Ek0, phi = 0.05, 0.7
+    d1 = -1.0 / (4.0 * 2000.0 * Ek0)
+    dr, dt_, dz = _coriolis_delta(ctx, z, z.copy(), z.copy(), Ek0, phi)
+    h = int(np.nonzero((ctx.harm.k == 0) & (ctx.harm.m == 1))[0][0])
+    U = ctx.U
+    s = abs(d1) * math.sin(phi)
+    assert np.abs(dr[:, h] - 1j * d1 * math.sin(phi) * U / 2).max() <= 1e-14 * s
+    assert np.abs(dt_[:, h] - (-d1 * math.sin(phi) * U / 2)).max() <= 1e-14 * s
+    others = np.ones(ctx.H, dtype=bool)
+    others[h] = False
+    assert np.abs(dr[:, others]).max() <= 1e-14 * s and np.abs(dt_[:, others]).max() <= 1e-14 * s
+    assert np.abs(dz).max() <= 1e-14 * s
