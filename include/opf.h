@@ -111,6 +111,16 @@ int opf_slot_symmetry(opf_ctx* ctx, int slot, int mirror, int shiftrefl, int shi
  * Mp = 1.  enable = 0 removes it. */
 int opf_slot_coriolis(opf_ctx* ctx, int slot, int enable, double Ek0, double phi);
 
+/* The non-Newtonian plugin (nonnewtonian.f90, params_model.in with model 1,
+ * Carreau-Yasuda): after every evaluation the strain rate of u + U, the
+ * viscosity nu = mu_inf/mu_0 + (1 - mu_inf/mu_0)(1 + (lam sr)^a)^(-n/a) with
+ * lam = Re lam_CY mu_0 / (rho radius^2), and N += [(nu - 1) Lap(u + U) +
+ * grad(nu) . 2S] / Re (get_nu, update_nonlin_terms, :93-244); the slot then
+ * takes exactly one corrector per step (tim_pcn = 1, :70).  enable = 0
+ * removes it. */
+int opf_slot_nonnewtonian(opf_ctx* ctx, int slot, int enable, double rho, double radius,
+                          double mu_0, double mu_inf, double lam_CY, double n_CY, double a_CY);
+
 /* tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir of the slot (the columns of
  * io_write_timestep, io.f90:645-657). */
 int opf_slot_timestep(opf_ctx* ctx, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir);

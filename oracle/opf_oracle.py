@@ -866,6 +866,51 @@ class Ctx:
         self.add_hpf()
         if getattr(self, "coriolis", None) is not None:
             self.add_coriolis(*self.coriolis)
+        if getattr(self, "nonnewtonian", None) is not None:
+            self.add_nonnewtonian(**self.nonnewtonian)
+
+    def add_nonnewtonian(self, rho, radius, mu_0, mu_inf, lam_CY, n_CY, a_CY):
+        """util(2) of the non-Newtonian plugin with the Carreau-Yasuda model:
+        get_nu (nonnewtonian.f90:93-193) and update_nonlin_terms (:203-244),
+        vel_nu = 1.  The velocity includes the base flow U on the (0, 0) mode."""
+        ur, ut, uz = self.ur.copy(), self.ut.copy(), self.uz.copy()
+        uz[:, 0] = (uz[:, 0].real + self.U) + 1j * uz[:, 0].imag
+        lam_ = self.Re * lam_CY * mu_0 / (rho * radius ** 2)
+        dr1 = self.mesh.dr[1]
+        rows = [[self.meshmult(dr1, ur), self.meshmult(dr1, ut), self.meshmult(dr1, uz)]]
+        ri = self.mesh.rp[-1][:, None]
+        mm = (self.harm.m * self.Mp).astype(float)[None, :]
+        rows.append([ri * (-(ur.imag * mm) - ut.real) + 1j * (ri * (ur.real * mm - ut.imag)),
+                     ri * (-(ut.imag * mm) + ur.real) + 1j * (ri * (ut.real * mm + ur.imag)),
+                     ri * (-(uz.imag * mm)) + 1j * (ri * (uz.real * mm))])
+        d1 = self.alpha * self.harm.k.astype(float)[None, :]
+        rows.append([-(a.imag * d1) + 1j * (a.real * d1) for a in (ur, ut, uz)])
+        S = [[self.spec2phys(rows[i][j]) for j in range(3)] for i in range(3)]
+        for i, j in ((0, 1), (0, 2), (1, 2)):
+            S[i][j] = 0.5 * (S[i][j] + S[j][i])
+            S[j][i] = S[i][j]
+        sr = S[0][0] ** 2 + S[1][1] ** 2 + S[2][2] ** 2 + 2.0 * (S[0][1] ** 2 + S[0][2] ** 2 + S[1][2] ** 2)
+        sr = np.sqrt(2.0 * sr)
+        d1 = mu_inf / mu_0
+        d2 = -n_CY / a_CY
+        nu = d1 + (1.0 - d1) * (1.0 + (lam_ * sr) ** a_CY) ** d2
+        nuc = self.phys2spec(nu)
+        self.nn_nu = nu
+        self.nn_nuc = nuc
+        g = [self.spec2phys(a) for a in self.grad(nuc)]
+        q = [self.spec2phys(a) for a in self.curl(*self.curl(ur, ut, uz))]
+        nu_ = nu - 1.0
+        out = []
+        for i in range(3):
+            qi = nu_ * (-q[i])
+            for j in range(3):
+                qi = qi + g[j] * S[j][i] * 2.0
+            if i == 2:
+                qi = qi + (1.0 - 1.0) * (-4.0)
+            out.append(self.phys2spec(qi / self.Re))
+        self.Nr = self.Nr + out[0]
+        self.Nt = self.Nt + out[1]
+        self.Nz = self.Nz + out[2]
 
     def add_coriolis(self, Ek0, phi):
         """util(2) of Coriolis.f90:54-71: N += -1/(4 Ro) Omega^ x (U + u) on

@@ -96,6 +96,12 @@ struct opf_ctx {
   std::vector<double> alpha_h; // d_alpha per slot (host copy of SlotParams::alpha)
   struct Cor { int on = 0; double Ek0 = 0.0, phi = 0.0; };
   std::vector<Cor> cor;        // coriolis.in per slot (Coriolis.f90:36-39)
+  struct NN { int on = 0; double rho = 0, radius = 0, mu0 = 0, muinf = 0, lam = 0, n = 0, a = 0; };
+  std::vector<NN> nn;          // params_model.in per slot (nonnewtonian.f90:67-86)
+  int n_nn = 0;                // slots with the non-Newtonian viscosity on
+  std::vector<double> Re_h;    // d_Re per slot (host copy of SlotParams::Re)
+  double* nnP = nullptr;       // non-Newtonian physical fields (16 of N*Z*T), first use
+  double2* nnC = nullptr;      // non-Newtonian coll temporaries (4 of coll), first use
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
@@ -262,12 +268,17 @@ void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
 // One iteration.  With adaptive slots, the predictor's evaluation is
 // followed by vel_maxtstep + tim_new_tstep and, where dt changed,
 // vel_matrices (main.f90:24-29), all decided on the device.
+void launch_nonnewtonian(opf_ctx* c, int slot);
+
 void launch_iteration(opf_ctx* c, int s0, int ns) {
   if (c->n_symm > 0) {   // vel_imposesym at the start of a step (main.f90:21), then the new curl
     launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, s0, 1);
     launch(c, "curl", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, 1);
   }
   launch_eval(c, s0, ns);
+  if (c->n_nn > 0)   // var_null(2) after vel_nonlinear (main.f90:23,38)
+    for (int s = s0; s < s0 + ns; ++s)
+      if (c->nn[s].on) launch_nonnewtonian(c, s);
   if (c->n_adaptive > 0) {
     launch(c, "newdt", k_newdt, dim3(1, 1, ns), dim3(64), 0, c->G, c->B, s0, 0);
     launch_matrices(c, s0, ns, 1);
@@ -301,6 +312,61 @@ void ensure_temps(opf_ctx* c) {
   c->tY = c->alloc<double2>((size_t)c->N * c->Z * c->M);
   c->tP = c->alloc<double>((size_t)c->N * c->Z * c->T);
   c->tC = c->alloc<double2>(c->G.coll);
+}
+
+// Single-field transforms on device buffers (tra_spec2phys / tra_phys2spec
+// of one coll field, transform.fftw3.f90:65-147).
+void coll2phys_dev(opf_ctx* c, const double2* src, double* dst) {
+  const FftTiles& t = c->tiles;
+  launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+         c->B, t, src, c->tY);
+  launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
+         (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, (const double2*)c->tY, dst);
+}
+void phys2coll_dev(opf_ctx* c, const double* src, double2* dst) {
+  const FftTiles& t = c->tiles;
+  HIPCHK(hipMemsetAsync(dst, 0, sizeof(double2) * c->G.coll, c->stream));
+  launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
+         (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, src, c->tY);
+  launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+         c->B, t, (const double2*)c->tY, dst);
+}
+
+// util(2) of the non-Newtonian plugin for one slot: get_nu and
+// update_nonlin_terms (nonnewtonian.f90:93-244) on the slot's current
+// evaluation, vel_nu = 1.
+void launch_nonnewtonian(opf_ctx* c, int slot) {
+  ensure_temps(c);
+  const size_t np = (size_t)c->N * c->Z * c->T;
+  if (!c->nnP) c->nnP = c->alloc<double>(16 * np);
+  if (!c->nnC) c->nnC = c->alloc<double2>(4 * c->G.coll);
+  NnPhys P{};
+  for (int i = 0; i < 9; ++i) P.S[i] = c->nnP + (size_t)i * np;
+  P.nu = c->nnP + 9 * np;
+  for (int i = 0; i < 3; ++i) P.g[i] = c->nnP + (size_t)(10 + i) * np;
+  for (int i = 0; i < 3; ++i) P.q[i] = c->nnP + (size_t)(13 + i) * np;
+  double2* c3 = c->nnC;                       // three coll components
+  double2* nuc = c->nnC + 3 * c->G.coll;
+  const dim3 gh((c->H + 63) / 64), b64(64);
+  for (int row = 0; row < 3; ++row) {          // A(row, j), velocity-gradient rows
+    launch(c, "nn_grad", k_nn_grad, gh, b64, 0, c->G, c->B, slot, row, c3);
+    for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.S[row * 3 + j]);
+  }
+  const opf_ctx::NN& q = c->nn[slot];
+  const double lam = c->Re_h[slot] * q.lam * q.mu0 / (q.rho * (q.radius * q.radius));   // lam_ (:77)
+  const dim3 gp((unsigned)((np + 255) / 256)), b256(256);
+  launch(c, "nn_visc", k_nn_visc, gp, b256, 0, P, np, lam, q.muinf / q.mu0, -q.n / q.a, q.a);
+  phys2coll_dev(c, P.nu, nuc);
+  launch(c, "nn_gradp", k_nn_gradp, gh, b64, 0, c->G, c->B, slot, (const double2*)nuc, c3);
+  for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.g[j]);
+  launch(c, "nn_curlcurl", k_nn_curlcurl, gh, b64, 0, c->G, c->B, slot, c3);
+  for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.q[j]);
+  launch(c, "nn_force", k_nn_force, gp, b256, 0, P, np, 1.0, c->Re_h[slot]);
+  for (int j = 0; j < 3; ++j) {
+    phys2coll_dev(c, P.q[j], c->tC);
+    launch(c, "nn_addN", k_nn_addN, dim3((c->H + 63) / 64, c->N, 1), b64, 0, c->G, c->B, slot, j,
+           (const double2*)c->tC);
+  }
 }
 
 std::vector<double2> twiddles(int L) {
@@ -404,6 +470,8 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->symm.assign(cap, 0);
   c->alpha_h.assign(cap, 0.0);
   c->cor.assign(cap, opf_ctx::Cor{});
+  c->nn.assign(cap, opf_ctx::NN{});
+  c->Re_h.assign(cap, 0.0);
 }
 
 // The Coriolis parameters of Coriolis.f90:49-50, 58: Ro = Re Ek0, d1 = -1/(4 Ro).
@@ -428,6 +496,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   set_coriolis(p, c->cor[slot]);
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
   c->alpha_h[slot] = alpha;
+  c->Re_h[slot] = Re;
   auto ctl = read_ctl(c);
   ctl[slot].status = 0;
   write_ctl(c, ctl);
@@ -500,7 +569,7 @@ RunPlan run_begin(opf_ctx* c, const std::vector<int>& slots, const std::vector<i
     k.active = 1;
     k.mode = MODE_PRED;
     k.target_steps = k.step + nsteps[i];
-    k.corr_fixed = corr_fixed[i];
+    k.corr_fixed = c->nn[s].on ? 1 : corr_fixed[i];   // the plugin sets tim_pcn = 1 (nonnewtonian.f90:70)
     k.diag_every = diag_every;
     k.host_driven = 0;
     p.slots.push_back(s);
@@ -781,6 +850,19 @@ int opf_slot_coriolis(opf_ctx* c, int slot, int enable, double Ek0, double phi) 
   });
 }
 
+int opf_slot_nonnewtonian(opf_ctx* c, int slot, int enable, double rho, double radius, double mu_0,
+                          double mu_inf, double lam_CY, double n_CY, double a_CY) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (enable && !(rho > 0.0 && radius > 0.0 && mu_0 > 0.0 && mu_inf >= 0.0 && lam_CY >= 0.0 && a_CY > 0.0))
+      throw OpfError(OPF_ERR_ARG, "bad Carreau-Yasuda parameters");
+    opf_ctx::NN& q = c->nn[slot];
+    c->n_nn += (enable ? 1 : 0) - q.on;
+    q = opf_ctx::NN{};
+    if (enable) q = opf_ctx::NN{1, rho, radius, mu_0, mu_inf, lam_CY, n_CY, a_CY};
+  });
+}
+
 int opf_slot_timestep(opf_ctx* c, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir) {
   return guard([&] {
     check_slot(c, slot);
@@ -925,6 +1007,7 @@ int opf_vel_nonlinear(opf_ctx* c, int slot) {
              c->B, t, slot);
       launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
+      if (c->nn[slot].on) launch_nonnewtonian(c, slot);   // var_null(2)
     });
   });
 }

@@ -101,6 +101,19 @@ __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_newdt(Geo G, Bufs B, int slot0, int what);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
 __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ tab);
+// non-Newtonian plugin: physical-grid fields of one slot (row-major S(i,j) at S[3i+j])
+struct NnPhys {
+  double* S[9];
+  double* nu;
+  double* g[3];
+  double* q[3];
+};
+__global__ void k_nn_grad(Geo G, Bufs B, int slot, int row, double2* __restrict__ out);
+__global__ void k_nn_gradp(Geo G, Bufs B, int slot, const double2* __restrict__ p, double2* __restrict__ out);
+__global__ void k_nn_curlcurl(Geo G, Bufs B, int slot, double2* __restrict__ out);
+__global__ void k_nn_visc(NnPhys P, size_t np, double lam, double d1, double d2, double a);
+__global__ void k_nn_force(NnPhys P, size_t np, double nu_r, double Re);
+__global__ void k_nn_addN(Geo G, Bufs B, int slot, int comp, const double2* __restrict__ src);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64

@@ -556,6 +556,144 @@ __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated) {
   if (hp != h) u[hp] = y;
 }
 
+// ---------------------------------------------------------------------------
+// Non-Newtonian viscosity (the plugin of nonnewtonian.f90:93-244): the
+// velocity-gradient tensor of the total flow u + U(r) z^, the strain rate and
+// the Carreau-Yasuda viscosity nu on the physical grid, and the forcing
+// N' = [(nu - 1) Lap(u) + grad(nu) . 2S] / Re added to the raw N of the
+// evaluation.  Spectral pieces run per harmonic (one thread per h, all n),
+// physical pieces per grid point; the transforms between them are the
+// single-field transforms of tra_coll2phys / tra_phys2coll.
+// ---------------------------------------------------------------------------
+// One row of the gradient tensor A of the slot's current iterate (get_nu,
+// nonnewtonian.f90:108-147): row 0 the radial derivatives by
+// var_coll_meshmult with dr(1), row 1 the theta terms (1/r)(d/dtheta u -
+// e_theta x u), row 2 the z derivatives; out holds the three components.
+__global__ void k_nn_grad(Geo G, Bufs B, int slot, int row, double2* __restrict__ out) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= G.H) return;
+  int k, m;
+  harmonic_km(h, G.K, k, m);
+  const double2* u = B.U[B.ctl[slot].ucur] + (size_t)slot * 3 * G.coll;
+  const MeshOff mo = MeshOff::make(G.N);
+  const double* me = G.mesh;
+  const size_t hs = G.Hp;
+  const int N = G.N;
+  auto at = [&](int comp, int n) {   // u, with vel_U added to the (0, 0) mode of u_z
+    double2 v = u[(size_t)comp * G.coll + (size_t)n * hs + h];
+    if (comp == 2 && h == 0) v.x = v.x + me[mo.Ubase + n];
+    return v;
+  };
+  for (int n = 0; n < N; ++n) {
+    const size_t o = (size_t)n * hs + h;
+    if (row == 0) {
+      const int j0 = n - KL < 0 ? 0 : n - KL;
+      const int j1 = n + KL > N - 1 ? N - 1 : n + KL;
+      for (int comp = 0; comp < 3; ++comp) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int j = j0; j <= j1; ++j) {
+          const double a = me[mo.dr1b + (KL + n - j) * N + j];
+          const double2 v = at(comp, j);
+          acc.x = acc.x + a * v.x;
+          acc.y = acc.y + a * v.y;
+        }
+        out[(size_t)comp * G.coll + o] = acc;
+      }
+    } else if (row == 1) {
+      const double ri = me[mo.rinv + n];
+      const double mm = (double)(m * G.Mp);
+      const double2 vr = at(0, n), vt = at(1, n), vz = at(2, n);
+      out[o] = make_double2(ri * (-(vr.y * mm) - vt.x), ri * (vr.x * mm - vt.y));
+      out[G.coll + o] = make_double2(ri * (-(vt.y * mm) + vr.x), ri * (vt.x * mm + vr.y));
+      out[2 * G.coll + o] = make_double2(ri * (-(vz.y * mm)), ri * (vz.x * mm));
+    } else {
+      const double d1 = B.prm[slot].alpha * (double)k;
+      for (int comp = 0; comp < 3; ++comp) {
+        const double2 v = at(comp, n);
+        out[(size_t)comp * G.coll + o] = make_double2(-(v.y * d1), v.x * d1);
+      }
+    }
+  }
+}
+
+// var_coll_grad (variables.f90:442-459) of a scalar coll field p into out.
+__global__ void k_nn_gradp(Geo G, Bufs B, int slot, const double2* __restrict__ p, double2* __restrict__ out) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= G.H) return;
+  int k, m;
+  harmonic_km(h, G.K, k, m);
+  const MeshOff mo = MeshOff::make(G.N);
+  const double* me = G.mesh;
+  const double ak = B.prm[slot].alpha * (double)k;
+  for (int n = 0; n < G.N; ++n) {
+    const size_t o = (size_t)n * G.Hp + h;
+    out[o] = band_row(me + mo.dr1b, G.N, n, p + h, G.Hp, make_double2(0.0, 0.0));
+    const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
+    const double2 v = p[o];
+    out[G.coll + o] = make_double2(-(v.y * mr), v.x * mr);
+    out[2 * G.coll + o] = make_double2(-(v.y * ak), v.x * ak);
+  }
+}
+
+// curl(curl(u + U z^)) of the current iterate: the curl of the slot's
+// vorticity om = curl u, plus curl curl (U z^) = -Lap(U) z^ = 4 z^ on the
+// (0, 0) mode (U = 1 - r^2).
+__global__ void k_nn_curlcurl(Geo G, Bufs B, int slot, double2* __restrict__ out) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= G.H) return;
+  int k, m;
+  harmonic_km(h, G.K, k, m);
+  const double2* om = B.om + (size_t)slot * 3 * G.coll;
+  curl_lane(G, h, k, m, B.prm[slot].alpha, om, om + G.coll, om + 2 * G.coll, out);
+  if (h == 0)
+    for (int n = 0; n < G.N; ++n) out[2 * G.coll + (size_t)n * G.Hp].x += 4.0;
+}
+
+// get_nu's algebra per grid point (nonnewtonian.f90:148-172): S = (A + A^T)/2
+// written back over A, strain rate sr = (2 S:S)^(1/2) and the Carreau-Yasuda
+// viscosity nu = d1 + (1 - d1)(1 + (lam sr)^a)^d2.
+__global__ void k_nn_visc(NnPhys P, size_t np, double lam, double d1, double d2, double a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double* const* S = P.S;
+  const double s12 = 0.5 * (S[1][i] + S[3][i]);
+  const double s13 = 0.5 * (S[2][i] + S[6][i]);
+  const double s23 = 0.5 * (S[5][i] + S[7][i]);
+  S[1][i] = s12; S[3][i] = s12;
+  S[2][i] = s13; S[6][i] = s13;
+  S[5][i] = s23; S[7][i] = s23;
+  const double s11 = S[0][i], s22 = S[4][i], s33 = S[8][i];
+  double sr = s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23);
+  sr = sqrt(2.0 * sr);
+  P.nu[i] = d1 + (1.0 - d1) * pow(1.0 + pow(lam * sr, a), d2);
+}
+
+// update_nonlin_terms' algebra per grid point (nonnewtonian.f90:227-236):
+// q_i = [(nu - nu_r)(-curlcurl(u)_i) + sum_j grad(nu)_j S_ji 2 + (nu_r - 1)(-4) delta_i3] / Re.
+__global__ void k_nn_force(NnPhys P, size_t np, double nu_r, double Re) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double nu_ = P.nu[i] - nu_r;
+  for (int c = 0; c < 3; ++c) {
+    double q = nu_ * (-P.q[c][i]);
+    for (int j = 0; j < 3; ++j) q = q + P.g[j][i] * P.S[j * 3 + c][i] * 2.0;
+    if (c == 2) q = q + (nu_r - 1.0) * (-4.0);
+    P.q[c][i] = q / Re;
+  }
+}
+
+// N += N': one component added to the raw nonlinear terms of the slot's
+// current evaluation.
+__global__ void k_nn_addN(Geo G, Bufs B, int slot, int comp, const double2* __restrict__ src) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
+  if (h >= G.H || n >= G.N) return;
+  double2* dst = B.Nraw[B.ctl[slot].ncur] + (size_t)slot * 3 * G.coll + (size_t)comp * G.coll;
+  const size_t o = (size_t)n * G.Hp + h;
+  const double2 a = dst[o], b = src[o];
+  dst[o] = make_double2(a.x + b.x, a.y + b.y);
+}
+
 // var_coll_shift (variables.f90:487-528) on U[ucur] of slots [slot0,
 // slot0 + gridDim.z): harmonic (k, m) rotated by e^{-i alpha k dz}, then by
 // e^{-i m Mp dth}, each in the reference's order of operations

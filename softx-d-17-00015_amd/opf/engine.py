@@ -313,6 +313,13 @@ class Context:
         on = Ek0 is not None
         check(self.lib.opf_slot_coriolis(self.h, slot, int(on), float(Ek0 or 0.0), float(phi)))
 
+    def nonnewtonian(self, slot, rho=None, radius=1.0, mu_0=1.0, mu_inf=0.0, lam_CY=0.0, n_CY=0.0, a_CY=2.0):
+        """The non-Newtonian plugin (nonnewtonian.f90, Carreau-Yasuda model of
+        params_model.in) for the slot; rho=None removes it."""
+        on = rho is not None
+        check(self.lib.opf_slot_nonnewtonian(self.h, slot, int(on), float(rho or 0.0), float(radius), float(mu_0),
+                                             float(mu_inf), float(lam_CY), float(n_CY), float(a_CY)))
+
     def timestep(self, slot):
         """tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)."""
         dt, cd, cf, di = C.c_double(), C.c_double(), C.c_double(), C.c_int()

@@ -114,3 +114,29 @@ def test_coriolis_plugin_matches_oracle(phi):
         import opf
         bad = opf.Context(16, 6, 4, Mp=2)
         bad.coriolis(0, Ek0=0.01)              # io: 'add_coriolis: i_Mp/=1'
+
+
+def test_nonnewtonian_plugin_matches_oracle():
+    """The non-Newtonian plugin (nonnewtonian.f90:93-244, Carreau-Yasuda)
+    on the device against the oracle's util(2): 1 step to 1e-12 and 10
+    steps to 1e-10 at (16, 6, 4), both with one corrector per step
+    (tim_pcn = 1); the viscosity falls to about a third of mu_0 at the wall."""
+    import opf_oracle as O
+    g = load_golden("mini")
+    prm = dict(rho=1000.0, radius=0.01, mu_0=0.056, mu_inf=0.0056, lam_CY=3e-3, n_CY=0.6, a_CY=2.0)
+    ctx = _ctx(g)
+    ctx.nonnewtonian(0, **prm)
+    ctx.put_state(0, *g["ic"])
+    o = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), Re=float(g["Re"]), alpha=float(g["alpha"]),
+              dt=float(g["dt"]))
+    o.nonnewtonian = prm
+    o.set_state(*g["ic"])
+    for n, tol in ((1, 1e-12), (10, 1e-10)):
+        print("nonnewtonian", n, flush=True)
+        ctx.run(n - o.step)
+        o.run(n, ncorr_fixed=1)
+        u = ctx.get_state(0)
+        for c, ref in enumerate((o.ur, o.ut, o.uz)):
+            assert rel_err(u[c], ref) <= tol, (n, c)
+    assert ctx.status(0)["last_ncorr"] == 1
+    assert 0.2 < o.nn_nu.min() < 0.5

@@ -332,3 +332,53 @@ def test_coriolis_on_laminar_flow_is_a_single_harmonic():
     others[h] = False
     assert np.abs(dr[:, others]).max() <= 1e-14 * s and np.abs(dt_[:, others]).max() <= 1e-14 * s
     assert np.abs(dz).max() <= 1e-14 * s
+
+
+NN_PARAMS = dict(rho=1000.0, radius=0.01, mu_0=0.056, mu_inf=0.0056, lam_CY=3e-3, n_CY=0.6, a_CY=2.0)
+
+
+def _nn_term(ctx, u, **prm):
+    """The term the non-Newtonian util(2) adds, alone."""
+    ctx.set_state(*u)
+    ctx.Nr, ctx.Nt, ctx.Nz = ctx.zeros(), ctx.zeros(), ctx.zeros()
+    ctx.add_nonnewtonian(**prm)
+    return ctx.Nr, ctx.Nt, ctx.Nz
+
+
+def test_nonnewtonian_on_laminar_flow_closed_form():
+    """On Hagen-Poiseuille flow U = 1 - r^2 the strain rate is 2r, nu(r) =
+    d1 + (1 - d1)(1 + (2 lam r)^a)^(-n/a), and the added force is axial and
+    axisymmetric: N_z = (1/r) d/dr(r (nu - 1) U') / Re = (-4 (nu - 1) - 2 r
+    nu') / Re.  nu' comes from finite differences of the transformed nu, so
+    the comparison holds to the derivative's discretisation error, which
+    falls from 2e-4 at N = 32 to 2.5e-6 at 64 and 6e-8 at 96."""
+    N = 96
+    ctx = O.Ctx(N, 3, 3, Re=2400.0, alpha=1.25)
+    z = ctx.zeros()
+    nr, nt, nz = _nn_term(ctx, (z, z.copy(), z.copy()), **NN_PARAMS)
+    p = NN_PARAMS
+    lam = 2400.0 * p["lam_CY"] * p["mu_0"] / (p["rho"] * p["radius"] ** 2)
+    d1, d2, a = p["mu_inf"] / p["mu_0"], -p["n_CY"] / p["a_CY"], p["a_CY"]
+    r = ctx.mesh.r
+    x = 2.0 * lam * r
+    nu = d1 + (1 - d1) * (1 + x ** a) ** d2
+    dnu = (1 - d1) * d2 * (1 + x ** a) ** (d2 - 1) * a * x ** (a - 1) * 2.0 * lam
+    want = (-4.0 * (nu - 1.0) - 2.0 * r * dnu) / 2400.0
+    assert np.abs(ctx.nn_nuc[:, 0].real - nu).max() <= 2e-13
+    scale = np.abs(want).max()
+    assert np.abs(nz[:, 0].real - want).max() <= 2e-7 * scale
+    assert np.abs(nz[:, 0].imag).max() <= 1e-14 * scale
+    assert np.abs(nz[:, 1:]).max() <= 1e-14 * scale
+    assert np.abs(nr).max() <= 1e-14 * scale and np.abs(nt).max() <= 1e-14 * scale
+
+
+def test_nonnewtonian_with_unit_viscosity_adds_nothing():
+    """mu_inf = mu_0 gives nu = 1 everywhere: the plugin's force vanishes
+    (to the rounding of differentiating a constant)."""
+    ctx = O.Ctx(16, 4, 3, Re=2400.0, alpha=1.25)
+    rng = np.random.default_rng(6)
+    u = [1e-2 * (rng.standard_normal((16, ctx.H)) + 1j * rng.standard_normal((16, ctx.H))) for _ in range(3)]
+    O.Ctx.imposesym0(*u)
+    prm = dict(NN_PARAMS, mu_inf=NN_PARAMS["mu_0"])
+    out = _nn_term(ctx, u, **prm)
+    assert max(np.abs(a).max() for a in out) <= 1e-13
