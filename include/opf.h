@@ -121,6 +121,17 @@ int opf_slot_coriolis(opf_ctx* ctx, int slot, int enable, double Ek0, double phi
 int opf_slot_nonnewtonian(opf_ctx* ctx, int slot, int enable, double rho, double radius,
                           double mu_0, double mu_inf, double lam_CY, double n_CY, double a_CY);
 
+/* The LES plugin (LES.f90, params_model.in): eddy viscosity nu_t by
+ * Smagorinsky (model 1, filters 1-3, van Driest damping with A_plus) or
+ * Vreman (model 2, filters 1-2, damping when A_plus > 0) from the strain of
+ * u + U after every evaluation; before each predictor vel_nu = 1.1 (1 + Re
+ * max nu_t) when it left 7 % of that value, the matrices refactored on the
+ * device (update_nu_r, :264-273); N += (1/Re) nu' Lap(u + U) - 4 nu_t z^ +
+ * grad(nu_t) . 2S with nu' = Re nu_t + 1 - vel_nu (:280-320); one corrector
+ * per step (tim_pcn = 1).  enable = 0 removes it (and restores vel_nu = 1). */
+int opf_slot_les(opf_ctx* ctx, int slot, int enable, int model, double C_s, int filter,
+                 double A_plus);
+
 /* tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir of the slot (the columns of
  * io_write_timestep, io.f90:645-657). */
 int opf_slot_timestep(opf_ctx* ctx, int slot, double* dt, double* corr_dt, double* cfl_dt, int* cfl_dir);

@@ -868,6 +868,114 @@ class Ctx:
             self.add_coriolis(*self.coriolis)
         if getattr(self, "nonnewtonian", None) is not None:
             self.add_nonnewtonian(**self.nonnewtonian)
+        if getattr(self, "les", None) is not None:
+            self.add_les(**self.les)
+
+    def _total_velocity_gradient(self):
+        """get_nu's copies u, with vel_U on the (0, 0) mode of u_z, and the
+        gradient tensor A(i, j) on the physical grid (nonnewtonian.f90:108-147,
+        LES.f90:114-150)."""
+        ur, ut, uz = self.ur.copy(), self.ut.copy(), self.uz.copy()
+        uz[:, 0] = (uz[:, 0].real + self.U) + 1j * uz[:, 0].imag
+        dr1 = self.mesh.dr[1]
+        rows = [[self.meshmult(dr1, ur), self.meshmult(dr1, ut), self.meshmult(dr1, uz)]]
+        ri = self.mesh.rp[-1][:, None]
+        mm = (self.harm.m * self.Mp).astype(float)[None, :]
+        rows.append([ri * (-(ur.imag * mm) - ut.real) + 1j * (ri * (ur.real * mm - ut.imag)),
+                     ri * (-(ut.imag * mm) + ur.real) + 1j * (ri * (ut.real * mm + ur.imag)),
+                     ri * (-(uz.imag * mm)) + 1j * (ri * (uz.real * mm))])
+        d1 = self.alpha * self.harm.k.astype(float)[None, :]
+        rows.append([-(a.imag * d1) + 1j * (a.real * d1) for a in (ur, ut, uz)])
+        return (ur, ut, uz), [[self.spec2phys(rows[i][j]) for j in range(3)] for i in range(3)]
+
+    def les_lengths(self, filter):
+        """Rdth, dz, Delta_r and the Smagorinsky width Delta (LES.f90:81-89, 199-206)."""
+        r = self.mesh.r
+        N = self.N
+        Rdth = (2.0 * D_PI / float(self.Mp * 3 * self.M)) * (3.0 / 2.0)
+        dz = (2.0 * D_PI / (self.alpha * float(3 * self.K))) * (3.0 / 2.0)
+        Dr = np.zeros(N)
+        Dr[0] = (r[1] - 0.0) / 2.0
+        for n in range(1, N - 1):
+            Dr[n] = (r[n + 1] - r[n - 1]) / 2.0
+        Dr[N - 1] = 1.0 - r[N - 2]
+        if filter == 1:
+            Delta = (Dr * Rdth * dz) ** (1.0 / 3.0)
+        elif filter == 2:
+            Delta = (Dr * r * Rdth * dz) ** (1.0 / 3.0)
+        else:
+            Delta = np.sqrt((Dr ** 2 + (r * Rdth) ** 2 + dz ** 2) / 3.0)
+        return Rdth, dz, Dr, Delta
+
+    def add_les(self, model, C_s, filter, A_plus):
+        """util(2) of the LES plugin (LES.f90:100-320): the eddy viscosity of
+        Smagorinsky (model 1) or Vreman (model 2), update_nu_r at the
+        predictor evaluation (tim_it = 0: vel_nu and vel_matrices) and
+        update_nonlin_terms."""
+        (ur, ut, uz), A = self._total_velocity_gradient()
+        S = [[A[i][j] for j in range(3)] for i in range(3)]
+        for i, j in ((0, 1), (0, 2), (1, 2)):
+            S[i][j] = 0.5 * (A[i][j] + A[j][i])
+            S[j][i] = S[i][j]
+        sr = S[0][0] ** 2 + S[1][1] ** 2 + S[2][2] ** 2 + 2.0 * (S[0][1] ** 2 + S[0][2] ** 2 + S[1][2] ** 2)
+        sr = np.sqrt(2.0 * sr)
+        N = self.N
+        w = self.mesh.dr1[:, 1]                      # mes_D%dr1(:,1)
+        d1 = 0.0
+        for i in range(KL + 1):
+            d1 = d1 + self.uz[N - 1 - KL + i, 0].real * w[i]
+        u_tau = math.sqrt(abs((d1 - 2.0) / self.Re))
+        y_plus = (self.Re * u_tau) * (1.0 - self.mesh.r)
+        Rdth, dz, Dr, Delta = self.les_lengths(filter)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            vanDr = 1.0 - np.exp(-((y_plus / A_plus) ** 2))
+        if model == 1:
+            nu = (vanDr * (C_s * Delta) ** 2)[:, None, None] * sr
+        else:
+            AA = np.zeros_like(sr)
+            for i in range(3):
+                for j in range(3):
+                    AA = AA + A[i][j] ** 2
+            dl = [Dr[:, None, None], (self.mesh.r * Rdth if filter == 2 else np.full(N, Rdth))[:, None, None], dz]
+            Bm = [[None] * 3 for _ in range(3)]
+            for i in range(3):
+                for j in range(3):
+                    b = np.zeros_like(sr)
+                    for m in range(3):
+                        b = b + (dl[m] ** 2) * A[m][i] * A[m][j]
+                    Bm[i][j] = b
+            P = (Bm[0][0] * Bm[1][1] - Bm[0][1] ** 2 + Bm[0][0] * Bm[2][2] - Bm[0][2] ** 2
+                 + Bm[1][1] * Bm[2][2] - Bm[1][2] ** 2)
+            P = np.where(AA < 1e-12, 0.0, P)
+            P = np.maximum(P, 0.0)
+            AA = np.maximum(AA, 1e-12)
+            nu = C_s * np.sqrt(P / AA)
+            if A_plus > 0.0:
+                dd = 0.03 / C_s
+                dd = dd + (1.0 - dd) * vanDr
+                nu = dd[:, None, None] * nu
+        nuc = self.phys2spec(nu)
+        self.les_nu, self.les_nuc = nu, nuc
+        if getattr(self, "it", 0) == 0:              # update_nu_r (LES.f90:264-273)
+            d1 = 1.0 + self.Re * nu.max()
+            d1 = 1.10 * d1
+            if self.nu < 0.93 * d1 or self.nu > 1.07 * d1:
+                self.nu = d1
+                self.matrices()
+        g = [self.spec2phys(a) for a in self.grad(nuc)]
+        q = [self.spec2phys(a) for a in self.curl(*self.curl(ur, ut, uz))]
+        nu_ = self.Re * nu + (1.0 - self.nu)
+        out = []
+        for i in range(3):
+            qi = (1.0 / self.Re) * nu_ * (-q[i])
+            if i == 2:
+                qi = qi + nu * (-4.0)
+            for j in range(3):
+                qi = qi + g[j] * S[j][i] * 2.0
+            out.append(self.phys2spec(qi))
+        self.Nr = self.Nr + out[0]
+        self.Nt = self.Nt + out[1]
+        self.Nz = self.Nz + out[2]
 
     def add_nonnewtonian(self, rho, radius, mu_0, mu_inf, lam_CY, n_CY, a_CY):
         """util(2) of the non-Newtonian plugin with the Carreau-Yasuda model:
@@ -1130,6 +1238,7 @@ class Ctx:
         timestep quantities of io_write_timestep (io.f90:645-657)."""
         series = []
         while self.step < nsteps:
+            self.it = 0               # tim_it at the predictor's evaluation
             self.imposesym()
             self.transform()
             self.nonlinear()

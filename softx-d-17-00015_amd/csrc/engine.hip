@@ -100,6 +100,11 @@ struct opf_ctx {
   std::vector<NN> nn;          // params_model.in per slot (nonnewtonian.f90:67-86)
   int n_nn = 0;                // slots with the non-Newtonian viscosity on
   std::vector<double> Re_h;    // d_Re per slot (host copy of SlotParams::Re)
+  struct LES { int on = 0, model = 1, filter = 1; double Cs = 0, Aplus = 0; };
+  std::vector<LES> les;        // params_model.in per slot (LES.f90:70-95)
+  int n_les = 0;               // slots with the LES eddy viscosity on
+  double* lesD = nullptr;      // per slot [Delta, Delta_r, L_theta, coef] x N
+  unsigned long long* lesMax = nullptr;   // per slot nu_max (bit pattern)
   double* nnP = nullptr;       // non-Newtonian physical fields (16 of N*Z*T), first use
   double2* nnC = nullptr;      // non-Newtonian coll temporaries (4 of coll), first use
   // granular transform temporaries (allocated on first use)
@@ -269,6 +274,7 @@ void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
 // followed by vel_maxtstep + tim_new_tstep and, where dt changed,
 // vel_matrices (main.f90:24-29), all decided on the device.
 void launch_nonnewtonian(opf_ctx* c, int slot);
+void launch_les(opf_ctx* c, int slot);
 
 void launch_iteration(opf_ctx* c, int s0, int ns) {
   if (c->n_symm > 0) {   // vel_imposesym at the start of a step (main.f90:21), then the new curl
@@ -276,9 +282,11 @@ void launch_iteration(opf_ctx* c, int s0, int ns) {
     launch(c, "curl", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, 1);
   }
   launch_eval(c, s0, ns);
-  if (c->n_nn > 0)   // var_null(2) after vel_nonlinear (main.f90:23,38)
-    for (int s = s0; s < s0 + ns; ++s)
+  if (c->n_nn > 0 || c->n_les > 0)   // var_null(2) after vel_nonlinear (main.f90:23,38)
+    for (int s = s0; s < s0 + ns; ++s) {
       if (c->nn[s].on) launch_nonnewtonian(c, s);
+      if (c->les[s].on) launch_les(c, s);
+    }
   if (c->n_adaptive > 0) {
     launch(c, "newdt", k_newdt, dim3(1, 1, ns), dim3(64), 0, c->G, c->B, s0, 0);
     launch_matrices(c, s0, ns, 1);
@@ -332,10 +340,11 @@ void phys2coll_dev(opf_ctx* c, const double* src, double2* dst) {
          c->B, t, (const double2*)c->tY, dst);
 }
 
-// util(2) of the non-Newtonian plugin for one slot: get_nu and
-// update_nonlin_terms (nonnewtonian.f90:93-244) on the slot's current
-// evaluation, vel_nu = 1.
-void launch_nonnewtonian(opf_ctx* c, int slot) {
+// The variable-viscosity plugins share their transforms: the gradient
+// tensor of u + U on the physical grid (into P.S), then, once nu is known on
+// the grid, grad(nu) (P.g) and curl curl (u + U) (P.q), and finally the
+// forcing in P.q added to the slot's raw N.
+NnPhys visc_fields(opf_ctx* c) {
   ensure_temps(c);
   const size_t np = (size_t)c->N * c->Z * c->T;
   if (!c->nnP) c->nnP = c->alloc<double>(16 * np);
@@ -345,28 +354,97 @@ void launch_nonnewtonian(opf_ctx* c, int slot) {
   P.nu = c->nnP + 9 * np;
   for (int i = 0; i < 3; ++i) P.g[i] = c->nnP + (size_t)(10 + i) * np;
   for (int i = 0; i < 3; ++i) P.q[i] = c->nnP + (size_t)(13 + i) * np;
-  double2* c3 = c->nnC;                       // three coll components
-  double2* nuc = c->nnC + 3 * c->G.coll;
-  const dim3 gh((c->H + 63) / 64), b64(64);
+  return P;
+}
+void visc_gradient(opf_ctx* c, int slot, const NnPhys& P) {
+  double2* c3 = c->nnC;
   for (int row = 0; row < 3; ++row) {          // A(row, j), velocity-gradient rows
-    launch(c, "nn_grad", k_nn_grad, gh, b64, 0, c->G, c->B, slot, row, c3);
+    launch(c, "nn_grad", k_nn_grad, dim3((c->H + 63) / 64), dim3(64), 0, c->G, c->B, slot, row, c3);
     for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.S[row * 3 + j]);
   }
-  const opf_ctx::NN& q = c->nn[slot];
-  const double lam = c->Re_h[slot] * q.lam * q.mu0 / (q.rho * (q.radius * q.radius));   // lam_ (:77)
-  const dim3 gp((unsigned)((np + 255) / 256)), b256(256);
-  launch(c, "nn_visc", k_nn_visc, gp, b256, 0, P, np, lam, q.muinf / q.mu0, -q.n / q.a, q.a);
+}
+void visc_derivatives(opf_ctx* c, int slot, const NnPhys& P) {
+  double2* c3 = c->nnC;
+  double2* nuc = c->nnC + 3 * c->G.coll;
+  const dim3 gh((c->H + 63) / 64), b64(64);
   phys2coll_dev(c, P.nu, nuc);
   launch(c, "nn_gradp", k_nn_gradp, gh, b64, 0, c->G, c->B, slot, (const double2*)nuc, c3);
   for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.g[j]);
   launch(c, "nn_curlcurl", k_nn_curlcurl, gh, b64, 0, c->G, c->B, slot, c3);
   for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.q[j]);
-  launch(c, "nn_force", k_nn_force, gp, b256, 0, P, np, 1.0, c->Re_h[slot]);
+}
+void visc_add(opf_ctx* c, int slot, const NnPhys& P) {
   for (int j = 0; j < 3; ++j) {
     phys2coll_dev(c, P.q[j], c->tC);
-    launch(c, "nn_addN", k_nn_addN, dim3((c->H + 63) / 64, c->N, 1), b64, 0, c->G, c->B, slot, j,
+    launch(c, "nn_addN", k_nn_addN, dim3((c->H + 63) / 64, c->N, 1), dim3(64), 0, c->G, c->B, slot, j,
            (const double2*)c->tC);
   }
+}
+
+// util(2) of the non-Newtonian plugin for one slot: get_nu and
+// update_nonlin_terms (nonnewtonian.f90:93-244) on the slot's current
+// evaluation, vel_nu = 1.
+void launch_nonnewtonian(opf_ctx* c, int slot) {
+  const NnPhys P = visc_fields(c);
+  const size_t np = (size_t)c->N * c->Z * c->T;
+  visc_gradient(c, slot, P);
+  const opf_ctx::NN& q = c->nn[slot];
+  const double lam = c->Re_h[slot] * q.lam * q.mu0 / (q.rho * (q.radius * q.radius));   // lam_ (:77)
+  const dim3 gp((unsigned)((np + 255) / 256)), b256(256);
+  launch(c, "nn_visc", k_nn_visc, gp, b256, 0, P, np, lam, q.muinf / q.mu0, -q.n / q.a, q.a);
+  visc_derivatives(c, slot, P);
+  launch(c, "nn_force", k_nn_force, gp, b256, 0, P, np, 1.0, c->Re_h[slot]);
+  visc_add(c, slot, P);
+}
+
+void launch_matrices(opf_ctx* c, int s0, int ns, int gated);
+
+// util(2) of the LES plugin for one slot (LES.f90:100-320): get_nu with the
+// slot's eddy-viscosity model, update_nu_r before a predictor (vel_matrices
+// on the device where vel_nu changed), update_nonlin_terms.
+void launch_les(opf_ctx* c, int slot) {
+  const NnPhys P = visc_fields(c);
+  const size_t np = (size_t)c->N * c->Z * c->T;
+  if (!c->lesD) c->lesD = c->alloc<double>((size_t)c->cap * 4 * c->N);
+  if (!c->lesMax) c->lesMax = c->alloc<unsigned long long>(c->cap);
+  double* D = c->lesD + (size_t)slot * 4 * c->N;
+  const opf_ctx::LES& q = c->les[slot];
+  visc_gradient(c, slot, P);
+  launch(c, "les_prep", k_les_prep, dim3(1), dim3(64), 0, c->G, c->B, slot, q.model, q.Cs, q.Aplus, D, c->lesMax);
+  const double dz = (2.0 * D_PI / (c->alpha_h[slot] * (double)c->Z)) * (3.0 / 2.0);
+  const dim3 gp((unsigned)((np + 255) / 256)), b256(256);
+  launch(c, "les_visc", k_les_visc, gp, b256, 0, P, np, c->N, c->Z * c->T, q.model, q.Cs, dz, (const double*)D,
+         c->lesMax + slot);
+  launch(c, "les_nur", k_les_nur, dim3(1), dim3(1), 0, c->B, slot, (const unsigned long long*)c->lesMax);
+  launch_matrices(c, slot, 1, 1);
+  visc_derivatives(c, slot, P);
+  launch(c, "les_force", k_les_force, gp, b256, 0, P, np, c->B, slot);
+  visc_add(c, slot, P);
+}
+
+// The filter lengths of init_params_model and Smagorinsky (LES.f90:81-89,
+// 199-206) for the slot's alpha, uploaded once.
+void les_geometry(opf_ctx* c, int slot) {
+  if (!c->lesD) c->lesD = c->alloc<double>((size_t)c->cap * 4 * c->N);
+  const int N = c->N;
+  const std::vector<double>& r = c->mesh.r;
+  const opf_ctx::LES& q = c->les[slot];
+  const double Rdth = (2.0 * D_PI / (double)(c->Mp * c->T)) * (3.0 / 2.0);
+  const double dz = (2.0 * D_PI / (c->alpha_h[slot] * (double)c->Z)) * (3.0 / 2.0);
+  std::vector<double> h((size_t)4 * N, 0.0);
+  double* Delta = h.data();
+  double* Dr = h.data() + N;
+  double* Lt = h.data() + 2 * N;
+  Dr[0] = (r[1] - 0.0) / 2.0;
+  for (int n = 1; n < N - 1; ++n) Dr[n] = (r[n + 1] - r[n - 1]) / 2.0;
+  Dr[N - 1] = 1.0 - r[N - 2];
+  for (int n = 0; n < N; ++n) {
+    if (q.filter == 1) Delta[n] = std::pow(Dr[n] * Rdth * dz, 1.0 / 3.0);
+    else if (q.filter == 2) Delta[n] = std::pow(Dr[n] * r[n] * Rdth * dz, 1.0 / 3.0);
+    else Delta[n] = std::sqrt((Dr[n] * Dr[n] + (r[n] * Rdth) * (r[n] * Rdth) + dz * dz) / 3.0);
+    Lt[n] = q.filter == 2 ? r[n] * Rdth : Rdth;   // Vreman's azimuthal length (:229-230)
+  }
+  HIPCHK(hipMemcpy(c->lesD + (size_t)slot * 4 * N, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 std::vector<double2> twiddles(int L) {
@@ -471,6 +549,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->alpha_h.assign(cap, 0.0);
   c->cor.assign(cap, opf_ctx::Cor{});
   c->nn.assign(cap, opf_ctx::NN{});
+  c->les.assign(cap, opf_ctx::LES{});
   c->Re_h.assign(cap, 0.0);
 }
 
@@ -486,7 +565,13 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   check_slot(c, slot);
   if (!(dt > 0.0) || !(Re > 0.0) || alpha == 0.0) throw OpfError(OPF_ERR_ARG, "Re, dt must be > 0 and alpha != 0");
   SlotParams p{};
-  p.Re = Re; p.alpha = alpha; p.dt = dt; p.implicit = implicit; p.nu = 1.0;
+  p.Re = Re; p.alpha = alpha; p.dt = dt; p.implicit = implicit; p.nu = 1.0;   // vel_nu (velocity.f90:51)
+  if (c->setup[slot] && c->les[slot].on) {   // vel_nu as update_nu_r left it
+    HIPCHK(hipStreamSynchronize(c->stream));
+    SlotParams old{};
+    HIPCHK(hipMemcpy(&old, c->B.prm + slot, sizeof(old), hipMemcpyDeviceToHost));
+    p.nu = old.nu;
+  }
   p.dterr_tol = 1e-5;   // d_dterr (parameters.f90:40)
   p.courant = c->adapt[slot].courant;   // d_courant (parameters.f90:41)
   p.const_flux = cf;
@@ -497,6 +582,7 @@ void slot_setup(opf_ctx* c, int slot, double Re, double alpha, double dt, double
   HIPCHK(hipMemcpy(c->B.prm + slot, &p, sizeof(p), hipMemcpyHostToDevice));
   c->alpha_h[slot] = alpha;
   c->Re_h[slot] = Re;
+  if (c->les[slot].on) les_geometry(c, slot);
   auto ctl = read_ctl(c);
   ctl[slot].status = 0;
   write_ctl(c, ctl);
@@ -569,7 +655,8 @@ RunPlan run_begin(opf_ctx* c, const std::vector<int>& slots, const std::vector<i
     k.active = 1;
     k.mode = MODE_PRED;
     k.target_steps = k.step + nsteps[i];
-    k.corr_fixed = c->nn[s].on ? 1 : corr_fixed[i];   // the plugin sets tim_pcn = 1 (nonnewtonian.f90:70)
+    // the variable-viscosity plugins set tim_pcn = 1 (nonnewtonian.f90:70, LES.f90:76)
+    k.corr_fixed = (c->nn[s].on || c->les[s].on) ? 1 : corr_fixed[i];
     k.diag_every = diag_every;
     k.host_driven = 0;
     p.slots.push_back(s);
@@ -850,6 +937,25 @@ int opf_slot_coriolis(opf_ctx* c, int slot, int enable, double Ek0, double phi) 
   });
 }
 
+int opf_slot_les(opf_ctx* c, int slot, int enable, int model, double C_s, int filter, double A_plus) {
+  return guard([&] {
+    check_slot(c, slot);
+    if (enable && model != 1 && model != 2) throw OpfError(OPF_ERR_ARG, "load_params_model: unknown model");
+    if (enable && (filter < 1 || filter > (model == 1 ? 3 : 2))) throw OpfError(OPF_ERR_ARG, "bad LES filter");
+    if (enable && !(C_s > 0.0)) throw OpfError(OPF_ERR_ARG, "C_s must be > 0");
+    opf_ctx::LES& q = c->les[slot];
+    c->n_les += (enable ? 1 : 0) - q.on;
+    q = opf_ctx::LES{};
+    if (enable) q = opf_ctx::LES{1, model, filter, C_s, A_plus};
+    if (enable && c->setup[slot]) les_geometry(c, slot);
+    if (!enable && c->setup[slot]) {   // vel_nu back to 1 and the matrices with it
+      SlotParams p{};
+      HIPCHK(hipMemcpy(&p, c->B.prm + slot, sizeof(p), hipMemcpyDeviceToHost));
+      if (p.nu != 1.0) slot_setup(c, slot, p.Re, p.alpha, p.dt, p.implicit, p.const_flux);
+    }
+  });
+}
+
 int opf_slot_nonnewtonian(opf_ctx* c, int slot, int enable, double rho, double radius, double mu_0,
                           double mu_inf, double lam_CY, double n_CY, double a_CY) {
   return guard([&] {
@@ -1008,6 +1114,7 @@ int opf_vel_nonlinear(opf_ctx* c, int slot) {
       launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
       if (c->nn[slot].on) launch_nonnewtonian(c, slot);   // var_null(2)
+      if (c->les[slot].on) launch_les(c, slot);
     });
   });
 }

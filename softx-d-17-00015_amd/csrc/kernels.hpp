@@ -114,6 +114,12 @@ __global__ void k_nn_curlcurl(Geo G, Bufs B, int slot, double2* __restrict__ out
 __global__ void k_nn_visc(NnPhys P, size_t np, double lam, double d1, double d2, double a);
 __global__ void k_nn_force(NnPhys P, size_t np, double nu_r, double Re);
 __global__ void k_nn_addN(Geo G, Bufs B, int slot, int comp, const double2* __restrict__ src);
+__global__ void k_les_prep(Geo G, Bufs B, int slot, int model, double Cs, double Aplus, double* D,
+                           unsigned long long* numax);
+__global__ void k_les_visc(NnPhys P, size_t np, int N, int ZT, int model, double Cs, double dz, const double* D,
+                           unsigned long long* numax_slot);
+__global__ void k_les_nur(Bufs B, int slot, const unsigned long long* numax);
+__global__ void k_les_force(NnPhys P, size_t np, Bufs B, int slot);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64

@@ -682,6 +682,136 @@ __global__ void k_nn_force(NnPhys P, size_t np, double nu_r, double Re) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// LES eddy viscosity (the plugin of LES.f90:100-395).  D holds per slot
+// [Delta(N), Delta_r(N), L_theta(N), coef(N)]: the Smagorinsky filter width,
+// the radial spacing, the azimuthal length of Vreman's filter and, written
+// by k_les_prep, the per-plane model coefficient.
+// ---------------------------------------------------------------------------
+// u_tau, Re_tau, y+ (get_nu, LES.f90:170-177) and the per-plane coefficient:
+// Smagorinsky vanDr (C_s Delta)^2 (:206-213), Vreman's damping
+// d1 + (1 - d1) vanDr with d1 = 0.03/C_s, or 1 without damping (:247-256);
+// clears the slot's nu_max.  One wave.
+__global__ void k_les_prep(Geo G, Bufs B, int slot, int model, double Cs, double Aplus, double* D,
+                           unsigned long long* numax) {
+  const int N = G.N;
+  const MeshOff mo = MeshOff::make(N);
+  const double* me = G.mesh;
+  const double2* uz = B.U[B.ctl[slot].ucur] + (size_t)slot * 3 * G.coll + 2 * G.coll;
+  const double* w = me + mo.dr1 + (KL + 1);   // dr1(:,1)
+  double d1 = 0.0;
+  for (int i = 0; i <= KL; ++i) d1 = d1 + uz[(size_t)(N - 1 - KL + i) * G.Hp].x * w[i];
+  const double Re = B.prm[slot].Re;
+  const double utau = sqrt(fabs((d1 - 2.0) / Re));
+  const double Retau = Re * utau;
+  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+    const double yp = Retau * (1.0 - me[mo.r + n]);
+    const double q = yp / Aplus;
+    const double vanDr = 1.0 - exp(-(q * q));
+    double c;
+    if (model == 1) {
+      const double cd = Cs * D[n];
+      c = vanDr * (cd * cd);
+    } else if (Aplus <= 0.0) {
+      c = 1.0;
+    } else {
+      const double a = 0.03 / Cs;
+      c = a + (1.0 - a) * vanDr;
+    }
+    D[3 * N + n] = c;
+  }
+  if (threadIdx.x == 0) numax[slot] = 0ull;
+}
+
+// The eddy viscosity per grid point from the gradient tensor A (row-major in
+// P.S), which is then replaced by S = (A + A^T)/2: Smagorinsky nu = coef
+// sr, sr = (2 S:S)^(1/2); Vreman nu = coef C_s (P/AA)^(1/2) with
+// B = sum_m Delta_m^2 A_mi A_mj (:218-257).  The slot's nu_max by one
+// atomic per workgroup on the bit pattern (nu >= 0).
+__global__ void k_les_visc(NnPhys P, size_t np, int N, int ZT, int model, double Cs, double dz, const double* D,
+                           unsigned long long* numax_slot) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double nu = 0.0;
+  if (i < np) {
+    const int n = (int)(i / (size_t)ZT);
+    double a[9];
+    for (int q = 0; q < 9; ++q) a[q] = P.S[q][i];
+    if (model == 2) {
+      double AA = 0.0;
+      for (int q = 0; q < 9; ++q) AA = AA + a[q] * a[q];
+      const double dl[3] = {D[N + n], D[2 * N + n], dz};
+      double Bm[9];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double b = 0.0;
+          for (int m = 0; m < 3; ++m) b = b + (dl[m] * dl[m]) * a[m * 3 + r] * a[m * 3 + c];
+          Bm[r * 3 + c] = b;
+        }
+      double Pv = Bm[0] * Bm[4] - Bm[1] * Bm[1] + Bm[0] * Bm[8] - Bm[2] * Bm[2] + Bm[4] * Bm[8] - Bm[5] * Bm[5];
+      Pv = AA < 1e-12 ? 0.0 : Pv;
+      Pv = fmax(Pv, 0.0);
+      AA = fmax(AA, 1e-12);
+      nu = Cs * sqrt(Pv / AA) * D[3 * N + n];
+    }
+    const double s12 = 0.5 * (a[1] + a[3]);
+    const double s13 = 0.5 * (a[2] + a[6]);
+    const double s23 = 0.5 * (a[5] + a[7]);
+    if (model == 1) {
+      double sr = a[0] * a[0] + a[4] * a[4] + a[8] * a[8] + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23);
+      sr = sqrt(2.0 * sr);
+      nu = D[3 * N + n] * sr;
+    }
+    P.S[1][i] = s12; P.S[3][i] = s12;
+    P.S[2][i] = s13; P.S[6][i] = s13;
+    P.S[5][i] = s23; P.S[7][i] = s23;
+    P.nu[i] = nu;
+  }
+  for (int off = 32; off >= 1; off >>= 1) nu = fmax(nu, __shfl_xor(nu, off));
+  __shared__ double wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = nu;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wm[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) v = fmax(v, wm[q]);
+    atomicMax(numax_slot, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// update_nu_r (LES.f90:264-273) before a predictor: vel_nu = 1.1 (1 + Re
+// nu_max) when it has left [0.93, 1.07] of that value; the change is left in
+// `refactor` for the gated vel_matrices launches that follow.
+__global__ void k_les_nur(Bufs B, int slot, const unsigned long long* numax) {
+  SlotCtl* c = &B.ctl[slot];
+  SlotParams* prm = &B.prm[slot];
+  if (!(c->active && c->mode == MODE_PRED)) {
+    c->refactor = 0;
+    return;
+  }
+  const double nm = __longlong_as_double((long long)numax[slot]);
+  double d1 = 1.0 + prm->Re * nm;
+  d1 = 1.10 * d1;
+  const bool ch = prm->nu < 0.93 * d1 || prm->nu > 1.07 * d1;
+  if (ch) prm->nu = d1;
+  c->refactor = ch ? 1 : 0;
+}
+
+// The LES forcing per grid point (update_nonlin_terms, LES.f90:303-311):
+// q_i = (1/Re) nu' (-curlcurl(u)_i) + nu (-4) delta_i3 + sum_j grad(nu)_j S_ji 2,
+// nu' = Re nu + 1 - vel_nu.
+__global__ void k_les_force(NnPhys P, size_t np, Bufs B, int slot) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double Re = B.prm[slot].Re, nur = B.prm[slot].nu;
+  const double nu = P.nu[i];
+  const double nu_ = Re * nu + (1.0 - nur);
+  for (int c = 0; c < 3; ++c) {
+    double q = (1.0 / Re) * nu_ * (-P.q[c][i]);
+    if (c == 2) q = q + nu * (-4.0);
+    for (int j = 0; j < 3; ++j) q = q + P.g[j][i] * P.S[j * 3 + c][i] * 2.0;
+    P.q[c][i] = q;
+  }
+}
+
 // N += N': one component added to the raw nonlinear terms of the slot's
 // current evaluation.
 __global__ void k_nn_addN(Geo G, Bufs B, int slot, int comp, const double2* __restrict__ src) {

@@ -44,6 +44,7 @@ PROTOTYPES = {
     "opf_steporbit": (C.c_int, [vp, C.c_int, ip, ip, dp, dp, dp]),
     "opf_slot_coriolis": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
     "opf_slot_nonnewtonian": (C.c_int, [vp, C.c_int, C.c_int] + [C.c_double] * 7),
+    "opf_slot_les": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double]),
     "opf_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "opf_run_slots": (C.c_int, [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int]),
     "opf_slot_status": (C.c_int, [vp, C.c_int, ip, dp, ip, ip, dp, dp]),

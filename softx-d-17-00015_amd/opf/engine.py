@@ -320,6 +320,13 @@ class Context:
         check(self.lib.opf_slot_nonnewtonian(self.h, slot, int(on), float(rho or 0.0), float(radius), float(mu_0),
                                              float(mu_inf), float(lam_CY), float(n_CY), float(a_CY)))
 
+    def les(self, slot, model=None, C_s=0.025, filter=1, A_plus=25.0):
+        """The LES plugin (LES.f90): model 1 Smagorinsky or 2 Vreman with the
+        params_model.in values C_s, filter, A_plus; model=None removes it."""
+        on = model is not None
+        check(self.lib.opf_slot_les(self.h, slot, int(on), int(model or 1), float(C_s), int(filter),
+                                    float(A_plus)))
+
     def timestep(self, slot):
         """tim_dt, tim_corr_dt, tim_cfl_dt, tim_cfl_dir (io_write_timestep)."""
         dt, cd, cf, di = C.c_double(), C.c_double(), C.c_double(), C.c_int()

@@ -382,3 +382,54 @@ def test_nonnewtonian_with_unit_viscosity_adds_nothing():
     prm = dict(NN_PARAMS, mu_inf=NN_PARAMS["mu_0"])
     out = _nn_term(ctx, u, **prm)
     assert max(np.abs(a).max() for a in out) <= 1e-13
+
+
+def _les_term(ctx, u, **prm):
+    """The LES util(2) term alone, at a predictor evaluation (tim_it = 0)."""
+    ctx.set_state(*u)
+    ctx.it = 0
+    ctx.Nr, ctx.Nt, ctx.Nz = ctx.zeros(), ctx.zeros(), ctx.zeros()
+    ctx.add_les(**prm)
+    return ctx.Nr, ctx.Nt, ctx.Nz
+
+
+def test_les_vreman_vanishes_on_laminar_flow():
+    """Vreman's model is zero for a one-directional shear: on Hagen-Poiseuille
+    flow only A_13 = dU/dr is non-zero, so P = 0 and nu_t = 0; update_nu_r
+    then sets vel_nu = 1.1 (1 + 0) and the force is the reference-viscosity
+    correction (1/Re)(1 - vel_nu)(-Lap U) = 0.4/Re on u_z of the (0, 0) mode."""
+    ctx = O.Ctx(24, 3, 3, Re=2400.0, alpha=1.25)
+    z = ctx.zeros()
+    nr, nt, nz = _les_term(ctx, (z, z.copy(), z.copy()), model=2, C_s=0.03, filter=1, A_plus=0.0)
+    assert np.abs(ctx.les_nu).max() == 0.0
+    assert ctx.nu == 1.1
+    want = 0.4 / 2400.0
+    assert np.abs(nz[:, 0].real - want).max() <= 1e-12 * want
+    assert np.abs(nz[:, 1:]).max() <= 1e-14 * want and np.abs(nz[:, 0].imag).max() <= 1e-14 * want
+    assert np.abs(nr).max() <= 1e-14 * want and np.abs(nt).max() <= 1e-14 * want
+
+
+@pytest.mark.parametrize("filt", [1, 2, 3])
+def test_les_smagorinsky_viscosity_on_laminar_flow(filt):
+    """Smagorinsky on Hagen-Poiseuille flow: strain rate 2r, u_tau = sqrt(2/Re)
+    from the wall shear, y+ = Re u_tau (1 - r), so nu_t(r) = (1 -
+    exp(-(y+/A+)^2)) (C_s Delta(r))^2 2r with the filter widths of
+    LES.f90:199-206 (Delta = (Delta_r R dtheta dz)^(1/3), with r, or the rms);
+    the force is axisymmetric and axial."""
+    Re, Cs, Ap = 2400.0, 0.1, 25.0
+    ctx = O.Ctx(24, 3, 3, Re=Re, alpha=1.25)
+    z = ctx.zeros()
+    nr, nt, nz = _les_term(ctx, (z, z.copy(), z.copy()), model=1, C_s=Cs, filter=filt, A_plus=Ap)
+    r = ctx.mesh.r
+    Rdth = 2 * np.pi / (3 * 3) * 1.5
+    dz = 2 * np.pi / (1.25 * 9) * 1.5
+    Dr = np.concatenate([[r[1] / 2], (r[2:] - r[:-2]) / 2, [1 - r[-2]]])
+    Delta = {1: np.cbrt(Dr * Rdth * dz), 2: np.cbrt(Dr * r * Rdth * dz),
+             3: np.sqrt((Dr ** 2 + (r * Rdth) ** 2 + dz ** 2) / 3)}[filt]
+    yp = Re * np.sqrt(2 / Re) * (1 - r)
+    nu = (1 - np.exp(-(yp / Ap) ** 2)) * (Cs * Delta) ** 2 * 2 * r
+    assert np.abs(ctx.les_nuc[:, 0].real - nu).max() <= 1e-13 * nu.max()
+    assert np.abs(ctx.les_nu.max() - nu.max()) <= 1e-14 * nu.max()
+    assert abs(ctx.nu - 1.1 * (1 + Re * ctx.les_nu.max())) <= 1e-15 * ctx.nu
+    s = np.abs(nz[:, 0]).max()
+    assert np.abs(nz[:, 1:]).max() <= 1e-13 * s and np.abs(nr).max() <= 1e-13 * s and np.abs(nt).max() <= 1e-13 * s
