@@ -311,6 +311,51 @@ module opf_c
          integer(c_int) :: rc
       end function
 
+      ! forcing plugins at var_null(2) (Coriolis.f90, nonnewtonian.f90, LES.f90)
+      function opf_slot_coriolis(ctx, slot, enable, Ek0, phi) bind(C, name="opf_slot_coriolis") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, enable
+         real(c_double), value :: Ek0, phi
+         integer(c_int) :: rc
+      end function
+
+      function opf_slot_nonnewtonian(ctx, slot, enable, rho, radius, mu_0, mu_inf, lam_CY, n_CY, a_CY) &
+            bind(C, name="opf_slot_nonnewtonian") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, enable
+         real(c_double), value :: rho, radius, mu_0, mu_inf, lam_CY, n_CY, a_CY
+         integer(c_int) :: rc
+      end function
+
+      function opf_slot_les(ctx, slot, enable, model, C_s, filter, A_plus) bind(C, name="opf_slot_les") &
+            result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, enable, model, filter
+         real(c_double), value :: C_s, A_plus
+         integer(c_int) :: rc
+      end function
+
+      ! steporbit (newton.f90:721-775) of several slots, var_coll_shift
+      function opf_steporbit(ctx, nslots, slots, ndts, T, sth, sz) bind(C, name="opf_steporbit") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nslots
+         integer(c_int), intent(in) :: slots(*), ndts(*)
+         real(c_double), intent(in) :: T(*), sth(*), sz(*)
+         integer(c_int) :: rc
+      end function
+
+      function opf_vel_shift(ctx, slot, dth, dz) bind(C, name="opf_vel_shift") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), value :: dth, dz
+         integer(c_int) :: rc
+      end function
+
       ! state files: io_save_state / io_load_state (io.f90:98-392)
       function opf_state_file_info(path, N, K, M, Mp, t, Re, alpha, dt) bind(C, name="opf_state_file_info") &
             result(rc)
