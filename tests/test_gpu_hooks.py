@@ -168,3 +168,68 @@ def test_les_plugin_matches_oracle(model, filt, Ap):
             assert rel_err(u[c], ref) <= tol, (n, c)
     assert o.nu > 1.0                   # vel_nu moved off 1 and the matrices followed
     ctx.les(0, None)
+
+
+def _seeded(N, K, M, Mp, alpha, seed, E0):
+    import opf_oracle as O
+    from opf import ic
+    me = O.build_mesh(N)
+    return ic.seeded_perturbation(me.r, me.intrdr, K, M, alpha, seed, E0, Mp=Mp)
+
+
+@pytest.mark.parametrize("plugin", ["nonnewtonian", "les"])
+def test_viscosity_plugins_with_mp2_match_oracle(plugin):
+    """The variable-viscosity plugins at Mp = 2 (the azimuthal wavenumbers
+    m Mp in the gradient tensor, Vreman's and Smagorinsky's R dtheta):
+    device against oracle, 1 step to 1e-12 and 8 steps to 1e-10."""
+    import opf
+    import opf_oracle as O
+    N, K, M, Mp, Re, alpha, dt = 16, 4, 4, 2, 2000.0, 1.1, 0.01
+    u = _seeded(N, K, M, Mp, alpha, 7, 1e-3)
+    prm = (dict(rho=1000.0, radius=0.01, mu_0=0.056, mu_inf=0.0056, lam_CY=3e-3, n_CY=0.6, a_CY=2.0)
+           if plugin == "nonnewtonian" else dict(model=2, C_s=0.1, filter=2, A_plus=25.0))
+    ctx = opf.Context(N, K, M, Mp=Mp)
+    ctx.setup(0, Re, alpha, dt)
+    getattr(ctx, plugin)(0, **prm)
+    ctx.put_state(0, *u)
+    o = O.Ctx(N, K, M, Mp=Mp, Re=Re, alpha=alpha, dt=dt)
+    setattr(o, plugin, prm)
+    o.set_state(*u)
+    for n, tol in ((1, 1e-12), (8, 1e-10)):
+        print(plugin, "Mp=2", n, flush=True)
+        ctx.run(n - o.step)
+        o.run(n, ncorr_fixed=1)
+        got = ctx.get_state(0)
+        for c, ref in enumerate((o.ur, o.ut, o.uz)):
+            assert rel_err(got[c], ref) <= tol, (n, c)
+
+
+def test_les_with_adaptive_time_step_matches_oracle():
+    """LES and the adaptive time step on one slot: before each predictor the
+    device first refactors for a new vel_nu (update_nu_r inside var_null(2))
+    and then, if tim_new_tstep changes tim_dt, again (main.f90:23-29); 12
+    steps against the oracle to 1e-10, with the same time series."""
+    import opf
+    import opf_oracle as O
+    g = load_golden("mini")
+    prm = dict(model=1, C_s=0.1, filter=1, A_plus=25.0)
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    ctx = opf.Context(N, K, M)
+    ctx.adaptive(0)
+    ctx.setup(0, float(g["Re"]), float(g["alpha"]), 0.01)
+    ctx.les(0, **prm)
+    ctx.put_state(0, *g["ic"])
+    ctx.run(12, diag_every=1)
+    o = O.Ctx(N, K, M, Re=float(g["Re"]), alpha=float(g["alpha"]), dt=0.01)
+    o.les = prm
+    o.set_state(*g["ic"])
+    ser = o.run(12, ncorr_fixed=1, adaptive=True)
+    u = ctx.get_state(0)
+    for c, ref in enumerate((o.ur, o.ut, o.uz)):
+        assert rel_err(u[c], ref) <= 1e-10, c
+    d = ctx.diag(0)
+    t_ref = np.array([r["t"] for r in ser]) if "t" in ser[0] else None
+    assert abs(ctx.timestep(0)["dt"] - o.dt) <= 1e-12 * o.dt
+    assert o.dt != 0.01 and o.nu > 1.0
+    if t_ref is not None:
+        assert np.max(np.abs(d[:, 1] - t_ref)) <= 1e-12
