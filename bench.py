@@ -54,7 +54,7 @@ def alg_bytes(N, K, M):
                 step_pred=9 * C + F, step_corr=15 * C + F, curl=6 * C)
 
 
-def survey_bytes_per_step(N, K, M, ncorr):
+def alg_bytes_per_step(N, K, M, ncorr):
     """B_alg = (21 + 27 n_corr) C + (1 + n_corr) F, the two-phase compulsory
     traffic model (factors once per +-k pair, influence vectors per harmonic)."""
     H = (2 * K - 1) * M - (K - 1)
@@ -313,7 +313,7 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "copies": 5}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design, not a code path)"}))
         sys.exit(2)
     import opf
@@ -370,7 +370,7 @@ def main():
     dom = max(kt, key=lambda k: kt[k]["ms_total"])
     achieved = ab[dom] / (kt[dom]["us_avg"] * 1e-6) / 1e9
     ms_step = 1000.0 * secs / args.steps
-    B_alg = survey_bytes_per_step(N, K, M, ncorr_mean)
+    B_alg = alg_bytes_per_step(N, K, M, ncorr_mean)
     step_gbs = B_alg / (secs / args.steps) / 1e9
 
     batched = None
