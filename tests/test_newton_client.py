@@ -189,3 +189,33 @@ def test_logeig_quadrants():
     want[-1] = -np.pi
     assert np.allclose(si, want / 2.0, atol=1e-15)
     assert np.allclose(sr, np.log(np.hypot(wr, wi)) / 2.0, atol=1e-15)
+
+
+def test_getrhs_wires_shooting_segments():
+    """getrhs (newton.f90:374-393) with ms = 2 and an identity time step:
+    segment p's image lands in segment p+1 (cyclically), the constraint rows
+    are zero, so f = (x_1 - x_0, x_0 - x_1) in the velocity entries."""
+    on, u, _ = _orbit_problem(ms=2)
+    n = on.n
+    x = on.x0.copy()
+    x[n + 8:] = 0.5 * x[n + 8:]                 # make the two segments differ
+    f = on.getrhs(x)
+    assert np.all(f[:8] == 0.0) and np.all(f[n:n + 8] == 0.0)
+    assert np.array_equal(f[n + 8:], x[8:n] - x[n + 8:])
+    assert np.array_equal(f[8:n], x[n + 8:] - x[8:n])
+
+
+def test_multj_constraint_rows_for_fixed_period_and_frozen_shift():
+    """multJ's special cases (newton.f90:448-451): with T_fixed and a frozen
+    theta shift (mirror or shift-reflect symmetry) the rows 1 and 3 of each
+    segment return the update's own entries."""
+    from opf.newton import NewtonHook
+    on, u, _ = _orbit_problem()
+    on.T_fixed, on.freeze_theta, on.epsJ = True, True, 1e-7
+    on.tim_dt = 0.2
+    nh = NewtonHook(on.x0)
+    nh.new_fx = on.getrhs(nh.new_x)
+    dx = np.random.default_rng(1).standard_normal(on.n)
+    y = on.multJ(nh, dx)
+    assert y[0] == dx[0] and y[2] == dx[2]
+    assert y[1] == on.dotprod(-1, on.getshiftdir(2, on.x0), dx)
