@@ -20,11 +20,14 @@ Third-party algorithms the reference calls:
   * LAPACK dgetrf/dgetri (mesh weights, 4x4 influence matrix): restated as
     unblocked DGETF2 + DGETRI (DTRTI2 + column sweep + column swaps).
 
-Parity pins (tests/test_oracle_*.py): golden vectors produced by the compiled
-reference (oracle/_ref, tests/golden/make_golden.py); FD weights exact on
-polynomials; DFT brute force; banded LU vs dense solve; laminar closed forms
-(E = pi^2/(3 alpha) normalisation, u_tau = sqrt(2/Re)); the axisymmetric swirl
-decay rate.
+Parity pins (tests/test_oracle_*.py, tests/test_state_io.py): golden vectors
+produced by the compiled reference (oracle/_ref, tests/golden/make_golden.py),
+including steporbit's call sequence; FD weights exact on polynomials; DFT brute
+force; banded LU vs dense solve; laminar closed forms (E = pi^2/(3 alpha)
+normalisation, u_tau = sqrt(2/Re)); the axisymmetric swirl decay rate;
+var_coll_shift as a translation of the grid by whole points; io_interp exact
+on polynomials of the field's parity across the axis; the plugins' forces in
+closed form on Hagen-Poiseuille flow (Coriolis also about the pipe axis).
 """
 from __future__ import annotations
 
