@@ -369,6 +369,114 @@ class DeviceStepper:
         return self.ctx.get_state(s)[:3]
 
 
+class BatchedStepper:
+    """Several independent searches (one thread each) on one context: every
+    steporbit request waits until each live search has posted one, then all
+    of them advance in one opf_steporbit launch sequence over their slots.
+    The searches' own arithmetic is untouched, so each follows exactly the
+    iterates it would follow alone; only the device work is batched.
+
+    ``stepper(i)`` is the Stepper of search i (slots ``slots[i]``).  Run each
+    search's ``solve`` in its own thread (``run_searches`` does this)."""
+
+    def __init__(self, ctx, slots):
+        import threading
+        self.ctx = ctx
+        self.slots = [list(s) for s in slots]
+        self.cv = threading.Condition()
+        self.live = len(self.slots)
+        self.pending = {}
+        self.results = {}
+        self.error = None
+        self.launches = 0
+
+    def stepper(self, i):
+        outer = self
+
+        class _One:
+            def steporbit(self, fields, ndts, T, sth, sz, dt1):
+                return outer._request(i, (fields, ndts, T, sth, sz, dt1))
+
+            def imposesym(self, fields):
+                with outer.cv:   # outside the batched phase: done alone, serialised
+                    s = outer.slots[i][0]
+                    outer.ctx.put_state(s, *fields)
+                    outer.ctx.imposesym(s)
+                    return outer.ctx.get_state(s)[:3]
+        return _One()
+
+    def finished(self, i):
+        with self.cv:
+            self.live -= 1
+            self._maybe_flush()
+            self.cv.notify_all()
+
+    def _request(self, i, req):
+        with self.cv:
+            self.pending[i] = req
+            self._maybe_flush()
+            while i not in self.results and self.error is None:
+                self.cv.wait()
+            if self.error is not None:
+                raise RuntimeError("batched steporbit failed") from self.error
+            return self.results.pop(i)
+
+    def _maybe_flush(self):
+        if not self.pending or len(self.pending) < self.live:
+            return
+        ctx = self.ctx
+        try:
+            sl, nd, Ts, sths, szs, owners = [], [], [], [], [], []
+            for i, (fields, ndts, T, sth, sz, dt1) in sorted(self.pending.items()):
+                for j, f in enumerate(fields):
+                    s = self.slots[i][j]
+                    if ndts[j] == 1 and ctx.timestep(s)["dt"] != dt1:
+                        ctx.vel_matrices(s, dt1)
+                    ctx.put_state(s, *f)
+                    sl.append(s)
+                    nd.append(ndts[j])
+                    Ts.append(T[j])
+                    sths.append(sth[j])
+                    szs.append(sz[j])
+                    owners.append(i)
+            ctx.steporbit(sl, nd, Ts, sths, szs)
+            self.launches += 1
+            out = {i: [] for i in self.pending}
+            for s, i in zip(sl, owners):
+                out[i].append(ctx.get_state(s)[:3])
+            self.results.update(out)
+        except Exception as e:   # every waiting search sees the failure
+            self.error = e
+        self.pending = {}
+        self.cv.notify_all()
+
+
+def run_searches(batch, searches, **solve_args):
+    """Run OrbitNewton searches (each built on ``batch.stepper(i)``) in
+    threads with their device integrations batched; returns each search's
+    (info, NewtonHook)."""
+    import threading
+    res = [None] * len(searches)
+
+    def work(i):
+        try:
+            res[i] = searches[i].solve(**solve_args)
+        except Exception as e:   # noqa: BLE001  (reported to the caller below)
+            res[i] = e
+        finally:
+            batch.finished(i)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(searches))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for r in res:
+        if isinstance(r, Exception):
+            raise r
+    return res
+
+
 class OrbitNewton:
     """newton.f90's search for a relative periodic orbit on ms shooting
     segments.  x0 fields: (ur, ut, uz) per segment (or one state, segments

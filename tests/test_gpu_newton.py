@@ -218,3 +218,46 @@ def test_floquet_eigenpairs_satisfy_the_device_map():
     # discretisation error at N = 16, dt = 0.01
     j11 = 3.8317059702075123
     assert np.abs(r["exponents"].real + j11 * j11 / Re).max() <= 1e-4 * j11 * j11 / Re
+
+
+def test_independent_searches_share_launches():
+    """Three orbit searches from different states run as threads over one
+    context (BatchedStepper): each reaches bit for bit the iterates it reaches
+    alone, while their integrations advance together: one launch sequence per
+    round of requests instead of one per request."""
+    import opf
+    from opf.newton import BatchedStepper, DeviceStepper, OrbitNewton, run_searches
+    g = load_golden("orbit_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    alpha = float(g["alpha"])
+    me = O.build_mesh(N)
+    starts = [[a * s for a in g["ic"]] for s in (1.0, 0.7, 1.3)]
+    args = dict(m=6, nits=2, rel_err=1e-10, delta=-1.0, epsJ=1e-7)
+    many = _ctx(opf, g, capacity=3)
+    batch = BatchedStepper(many, [[0], [1], [2]])
+    searches = [OrbitNewton(batch.stepper(i), N, K, M, 1, alpha, me.intrdr, st, 0.0, 0.0, 0.1, [10], r=me.r)
+                for i, st in enumerate(starts)]
+    print("batched searches", flush=True)
+    res = run_searches(batch, searches, **args)
+    calls = []
+    for i, st in enumerate(starts):
+        one = _ctx(opf, g)
+        counter = DeviceStepper(one, [0])
+        n_calls = [0]
+        orig = counter.steporbit
+
+        def counted(*a, _o=orig, _n=n_calls):
+            _n[0] += 1
+            return _o(*a)
+
+        counter.steporbit = counted
+        alone = OrbitNewton(counter, N, K, M, 1, alpha, me.intrdr, st, 0.0, 0.0, 0.1, [10], r=me.r)
+        info, nh = alone.solve(**args)
+        assert info == res[i][0]
+        assert np.array_equal(nh.new_x, res[i][1].new_x), i
+        assert [r["tol"] for r in alone.history] == [r["tol"] for r in searches[i].history]
+        calls.append(n_calls[0])
+    # lock step: one launch sequence per round of requests, as many rounds as
+    # the busiest search needs
+    assert batch.launches == max(calls) < sum(calls)
+    print("launches batched", batch.launches, "alone", sum(calls), flush=True)
