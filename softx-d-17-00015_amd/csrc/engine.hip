@@ -322,20 +322,24 @@ void ensure_temps(opf_ctx* c) {
   c->tC = c->alloc<double2>(c->G.coll);
 }
 
-// Single-field transforms on device buffers (tra_spec2phys / tra_phys2spec
-// of one coll field, transform.fftw3.f90:65-147).
+// Single-field transforms on device buffers for the plugins (tra_spec2phys /
+// tra_phys2spec of one coll field, transform.fftw3.f90:65-147); physical
+// fields in the plugins' layout [n][z][theta].
+int lines_per_block(const opf_ctx* c) { return std::max(1, FFT_TILE / padded_len(c->T)); }
 void coll2phys_dev(opf_ctx* c, const double2* src, double* dst) {
   const FftTiles& t = c->tiles;
+  const int BT = lines_per_block(c);
   launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
          c->B, t, src, c->tY);
-  launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
-         (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, (const double2*)c->tY, dst);
+  launch(c, "c2r_lines", k_c2r_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, 1), dim3(FFT_THREADS),
+         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, (const double2*)c->tY, dst);
 }
 void phys2coll_dev(opf_ctx* c, const double* src, double2* dst) {
   const FftTiles& t = c->tiles;
+  const int BT = lines_per_block(c);
   HIPCHK(hipMemsetAsync(dst, 0, sizeof(double2) * c->G.coll, c->stream));
-  launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
-         (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, src, c->tY);
+  launch(c, "r2c_lines", k_r2c_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, 1), dim3(FFT_THREADS),
+         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, src, c->tY);
   launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
          c->B, t, (const double2*)c->tY, dst);
 }
@@ -359,14 +363,14 @@ NnPhys visc_fields(opf_ctx* c) {
 void visc_gradient(opf_ctx* c, int slot, const NnPhys& P) {
   double2* c3 = c->nnC;
   for (int row = 0; row < 3; ++row) {          // A(row, j), velocity-gradient rows
-    launch(c, "nn_grad", k_nn_grad, dim3((c->H + 63) / 64), dim3(64), 0, c->G, c->B, slot, row, c3);
+    launch(c, "nn_grad", k_nn_grad, dim3((c->H + 63) / 64, c->N), dim3(64), 0, c->G, c->B, slot, row, c3);
     for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.S[row * 3 + j]);
   }
 }
 void visc_derivatives(opf_ctx* c, int slot, const NnPhys& P) {
   double2* c3 = c->nnC;
   double2* nuc = c->nnC + 3 * c->G.coll;
-  const dim3 gh((c->H + 63) / 64), b64(64);
+  const dim3 gh((c->H + 63) / 64, c->N), b64(64);
   phys2coll_dev(c, P.nu, nuc);
   launch(c, "nn_gradp", k_nn_gradp, gh, b64, 0, c->G, c->B, slot, (const double2*)nuc, c3);
   for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.g[j]);

@@ -581,6 +581,83 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_c2r(Geo G, Bufs B, FftTil
 }
 
 // r2c along theta (scaled by 1/(3K 3M) first, transform.fftw3.f90:114-117).
+// The theta transforms of the variable-viscosity plugins: the z-lines of one
+// radial plane packed two per complex transform (as k_theta_c2r does), BT
+// transforms per workgroup, on physical fields in the plugins' own layout
+// [n][z][theta] (theta fastest, so every line is one contiguous run).
+// grid: x = blocks of 2 BT z-lines, y = N.
+__global__ void __launch_bounds__(FFT_THREADS) k_c2r_lines(Geo G, Bufs B, FftTiles T, int BT, const double2* Y,
+                                                           double* phys) {
+  const int n = blockIdx.y;
+  const int M = G.M, TT = G.T, Z = G.Z;
+  const int Tp = padded_len(TT);
+  const int z0 = blockIdx.x * 2 * BT;
+  const int nt = min(BT, (Z - z0 + 1) / 2);
+  double2* buf = smem;
+  double2* twl = smem + (size_t)BT * Tp;
+  load_tw(twl, B.twT, TT);
+  for (int i = threadIdx.x; i < nt * TT; i += FFT_THREADS) {
+    const int b = i / TT, mm = i - b * TT;
+    const int z = z0 + 2 * b;
+    const bool two = z + 1 < Z;
+    const double2* A = Y + ((size_t)n * Z + z) * M;
+    const double2* Bp = A + M;
+    double2 v = make_double2(0.0, 0.0);
+    if (mm == 0) v = make_double2(A[0].x, two ? Bp[0].x : 0.0);
+    else if (mm < M) {
+      const double2 a = A[mm], c = two ? Bp[mm] : make_double2(0.0, 0.0);
+      v = make_double2(a.x - c.y, a.y + c.x);
+    } else if (mm > TT - M) {
+      const int q = TT - mm;
+      const double2 a = A[q], c = two ? Bp[q] : make_double2(0.0, 0.0);
+      v = make_double2(a.x + c.y, -a.y + c.x);
+    }
+    buf[(size_t)b * Tp + pidx(mm)] = v;
+  }
+  __syncthreads();
+  fft_lds(buf, nt, T.pt, 1.0, twl, B.twT);
+  for (int i = threadIdx.x; i < nt * TT; i += FFT_THREADS) {
+    const int b = i / TT, th = i - b * TT;
+    const int z = z0 + 2 * b;
+    const double2 v = buf[(size_t)b * Tp + pidx(th)];
+    double* out = phys + ((size_t)n * Z + z) * TT + th;
+    out[0] = v.x;
+    if (z + 1 < Z) out[TT] = v.y;
+  }
+}
+
+// The inverse of k_c2r_lines with the scaling of tra_phys2spec (1/(3K 3M),
+// transform.fftw3.f90:114), into the z-transform intermediates Y.
+__global__ void __launch_bounds__(FFT_THREADS) k_r2c_lines(Geo G, Bufs B, FftTiles T, int BT, const double* phys,
+                                                           double2* Y) {
+  const int n = blockIdx.y;
+  const int M = G.M, TT = G.T, Z = G.Z;
+  const int Tp = padded_len(TT);
+  const int z0 = blockIdx.x * 2 * BT;
+  const int nt = min(BT, (Z - z0 + 1) / 2);
+  const double scale = 1.0 / (double)(Z * TT);
+  double2* buf = smem;
+  double2* twl = smem + (size_t)BT * Tp;
+  load_tw(twl, B.twT, TT);
+  for (int i = threadIdx.x; i < nt * TT; i += FFT_THREADS) {
+    const int b = i / TT, th = i - b * TT;
+    const int z = z0 + 2 * b;
+    const double* in = phys + ((size_t)n * Z + z) * TT + th;
+    buf[(size_t)b * Tp + pidx(th)] = make_double2(scale * in[0], z + 1 < Z ? scale * in[TT] : 0.0);
+  }
+  __syncthreads();
+  fft_lds(buf, nt, T.pt, -1.0, twl, B.twT);
+  for (int i = threadIdx.x; i < nt * M; i += FFT_THREADS) {
+    const int b = i / M, mm = i - b * M;
+    const int z = z0 + 2 * b;
+    const double2* L = buf + (size_t)b * Tp;
+    const double2 X = L[pidx(mm)], Xc = L[pidx((TT - mm) % TT)];
+    double2* A = Y + ((size_t)n * Z + z) * M;
+    A[mm] = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
+    if (z + 1 < Z) A[M + mm] = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
+  }
+}
+
 __global__ void __launch_bounds__(FFT_THREADS) k_theta_r2c(Geo G, Bufs B, FftTiles T, const double* phys, double2* Y) {
   const int n = blockIdx.y;
   const int z0 = blockIdx.x * 2;

@@ -569,8 +569,10 @@ __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated) {
 // nonnewtonian.f90:108-147): row 0 the radial derivatives by
 // var_coll_meshmult with dr(1), row 1 the theta terms (1/r)(d/dtheta u -
 // e_theta x u), row 2 the z derivatives; out holds the three components.
+// One thread per (h, n): grid x = harmonics / 64, y = N.
 __global__ void k_nn_grad(Geo G, Bufs B, int slot, int row, double2* __restrict__ out) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
   if (h >= G.H) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
@@ -579,74 +581,89 @@ __global__ void k_nn_grad(Geo G, Bufs B, int slot, int row, double2* __restrict_
   const double* me = G.mesh;
   const size_t hs = G.Hp;
   const int N = G.N;
-  auto at = [&](int comp, int n) {   // u, with vel_U added to the (0, 0) mode of u_z
-    double2 v = u[(size_t)comp * G.coll + (size_t)n * hs + h];
-    if (comp == 2 && h == 0) v.x = v.x + me[mo.Ubase + n];
+  auto at = [&](int comp, int j) {   // u, with vel_U added to the (0, 0) mode of u_z
+    double2 v = u[(size_t)comp * G.coll + (size_t)j * hs + h];
+    if (comp == 2 && h == 0) v.x = v.x + me[mo.Ubase + j];
     return v;
   };
-  for (int n = 0; n < N; ++n) {
-    const size_t o = (size_t)n * hs + h;
-    if (row == 0) {
-      const int j0 = n - KL < 0 ? 0 : n - KL;
-      const int j1 = n + KL > N - 1 ? N - 1 : n + KL;
-      for (int comp = 0; comp < 3; ++comp) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int j = j0; j <= j1; ++j) {
-          const double a = me[mo.dr1b + (KL + n - j) * N + j];
-          const double2 v = at(comp, j);
-          acc.x = acc.x + a * v.x;
-          acc.y = acc.y + a * v.y;
-        }
-        out[(size_t)comp * G.coll + o] = acc;
+  const size_t o = (size_t)n * hs + h;
+  if (row == 0) {
+    const int j0 = n - KL < 0 ? 0 : n - KL;
+    const int j1 = n + KL > N - 1 ? N - 1 : n + KL;
+    for (int comp = 0; comp < 3; ++comp) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int j = j0; j <= j1; ++j) {
+        const double a = me[mo.dr1b + (KL + n - j) * N + j];
+        const double2 v = at(comp, j);
+        acc.x = acc.x + a * v.x;
+        acc.y = acc.y + a * v.y;
       }
-    } else if (row == 1) {
-      const double ri = me[mo.rinv + n];
-      const double mm = (double)(m * G.Mp);
-      const double2 vr = at(0, n), vt = at(1, n), vz = at(2, n);
-      out[o] = make_double2(ri * (-(vr.y * mm) - vt.x), ri * (vr.x * mm - vt.y));
-      out[G.coll + o] = make_double2(ri * (-(vt.y * mm) + vr.x), ri * (vt.x * mm + vr.y));
-      out[2 * G.coll + o] = make_double2(ri * (-(vz.y * mm)), ri * (vz.x * mm));
-    } else {
-      const double d1 = B.prm[slot].alpha * (double)k;
-      for (int comp = 0; comp < 3; ++comp) {
-        const double2 v = at(comp, n);
-        out[(size_t)comp * G.coll + o] = make_double2(-(v.y * d1), v.x * d1);
-      }
+      out[(size_t)comp * G.coll + o] = acc;
+    }
+  } else if (row == 1) {
+    const double ri = me[mo.rinv + n];
+    const double mm = (double)(m * G.Mp);
+    const double2 vr = at(0, n), vt = at(1, n), vz = at(2, n);
+    out[o] = make_double2(ri * (-(vr.y * mm) - vt.x), ri * (vr.x * mm - vt.y));
+    out[G.coll + o] = make_double2(ri * (-(vt.y * mm) + vr.x), ri * (vt.x * mm + vr.y));
+    out[2 * G.coll + o] = make_double2(ri * (-(vz.y * mm)), ri * (vz.x * mm));
+  } else {
+    const double d1 = B.prm[slot].alpha * (double)k;
+    for (int comp = 0; comp < 3; ++comp) {
+      const double2 v = at(comp, n);
+      out[(size_t)comp * G.coll + o] = make_double2(-(v.y * d1), v.x * d1);
     }
   }
 }
 
-// var_coll_grad (variables.f90:442-459) of a scalar coll field p into out.
+// var_coll_grad (variables.f90:442-459) of a scalar coll field p into out;
+// one thread per (h, n).
 __global__ void k_nn_gradp(Geo G, Bufs B, int slot, const double2* __restrict__ p, double2* __restrict__ out) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
   if (h >= G.H) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
   const MeshOff mo = MeshOff::make(G.N);
   const double* me = G.mesh;
   const double ak = B.prm[slot].alpha * (double)k;
-  for (int n = 0; n < G.N; ++n) {
-    const size_t o = (size_t)n * G.Hp + h;
-    out[o] = band_row(me + mo.dr1b, G.N, n, p + h, G.Hp, make_double2(0.0, 0.0));
-    const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
-    const double2 v = p[o];
-    out[G.coll + o] = make_double2(-(v.y * mr), v.x * mr);
-    out[2 * G.coll + o] = make_double2(-(v.y * ak), v.x * ak);
-  }
+  const size_t o = (size_t)n * G.Hp + h;
+  out[o] = band_row(me + mo.dr1b, G.N, n, p + h, G.Hp, make_double2(0.0, 0.0));
+  const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
+  const double2 v = p[o];
+  out[G.coll + o] = make_double2(-(v.y * mr), v.x * mr);
+  out[2 * G.coll + o] = make_double2(-(v.y * ak), v.x * ak);
 }
 
-// curl(curl(u + U z^)) of the current iterate: the curl of the slot's
-// vorticity om = curl u, plus curl curl (U z^) = -Lap(U) z^ = 4 z^ on the
-// (0, 0) mode (U = 1 - r^2).
+// curl(curl(u + U z^)) of the current iterate: the curl (var_coll_curl,
+// variables.f90:408-436) of the slot's vorticity om = curl u, plus curl curl
+// (U z^) = -Lap(U) z^ = 4 z^ on the (0, 0) mode (U = 1 - r^2); one thread
+// per (h, n).
 __global__ void k_nn_curlcurl(Geo G, Bufs B, int slot, double2* __restrict__ out) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
   if (h >= G.H) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
-  const double2* om = B.om + (size_t)slot * 3 * G.coll;
-  curl_lane(G, h, k, m, B.prm[slot].alpha, om, om + G.coll, om + 2 * G.coll, out);
-  if (h == 0)
-    for (int n = 0; n < G.N; ++n) out[2 * G.coll + (size_t)n * G.Hp].x += 4.0;
+  const int N = G.N;
+  const size_t hs = G.Hp;
+  const MeshOff mo = MeshOff::make(N);
+  const double* me = G.mesh;
+  const double ak = B.prm[slot].alpha * (double)k;
+  const double2* wr = B.om + (size_t)slot * 3 * G.coll;
+  const double2* wt = wr + G.coll;
+  const double2* wz = wr + 2 * G.coll;
+  const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
+  const size_t o = (size_t)n * hs + h;
+  const double2 vr = wr[o], vt = wt[o], vz = wz[o];
+  out[o] = make_double2(-(vz.y * mr) + vt.y * ak, vz.x * mr - vt.x * ak);
+  const double2 d1 = band_row(me + mo.dr1b, N, n, wz + h, hs, make_double2(0.0, 0.0));
+  out[G.coll + o] = make_double2(-(vr.y * ak) - d1.x, vr.x * ak - d1.y);
+  const double2 d2 = band_row(me + mo.dr1b, N, n, wt + h, hs, make_double2(0.0, 0.0));
+  const double ri = me[mo.rinv + n];
+  double2 qz = make_double2((vt.x * ri + d2.x) + vr.y * mr, (vt.y * ri + d2.y) - vr.x * mr);
+  if (h == 0) qz.x += 4.0;
+  out[2 * G.coll + o] = qz;
 }
 
 // get_nu's algebra per grid point (nonnewtonian.f90:148-172): S = (A + A^T)/2
