@@ -107,6 +107,7 @@ struct opf_ctx {
   unsigned long long* lesMax = nullptr;   // per slot nu_max (bit pattern)
   double* nnP = nullptr;       // non-Newtonian physical fields (16 of N*Z*T), first use
   double2* nnC = nullptr;      // non-Newtonian coll temporaries (4 of coll), first use
+  double2* nnY = nullptr;      // their z-transform intermediates (3 fields), first use
   // granular transform temporaries (allocated on first use)
   double2* tY = nullptr;
   double* tP = nullptr;
@@ -322,26 +323,30 @@ void ensure_temps(opf_ctx* c) {
   c->tC = c->alloc<double2>(c->G.coll);
 }
 
-// Single-field transforms on device buffers for the plugins (tra_spec2phys /
-// tra_phys2spec of one coll field, transform.fftw3.f90:65-147); physical
-// fields in the plugins' layout [n][z][theta].
+// Transforms of nf coll fields (stride coll) to / from physical fields
+// (stride N*Z*T) on device buffers for the plugins (tra_spec2phys /
+// tra_phys2spec, transform.fftw3.f90:65-147), one launch per stage with the
+// field as grid z; physical fields in the plugins' layout [n][z][theta].
 int lines_per_block(const opf_ctx* c) { return std::max(1, FFT_TILE / padded_len(c->T)); }
-void coll2phys_dev(opf_ctx* c, const double2* src, double* dst) {
+void coll2phys_dev(opf_ctx* c, const double2* src, double* dst, int nf) {
   const FftTiles& t = c->tiles;
   const int BT = lines_per_block(c);
-  launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-         c->B, t, src, c->tY);
-  launch(c, "c2r_lines", k_c2r_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, 1), dim3(FFT_THREADS),
-         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, (const double2*)c->tY, dst);
+  const size_t fy = (size_t)c->N * c->Z * c->M, fp = (size_t)c->N * c->Z * c->T;
+  launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, nf), dim3(FFT_THREADS), c->smem_z, c->G,
+         c->B, t, src, c->nnY, (size_t)c->G.coll, fy);
+  launch(c, "c2r_lines", k_c2r_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, nf), dim3(FFT_THREADS),
+         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, (const double2*)c->nnY, dst,
+         fy, fp);
 }
-void phys2coll_dev(opf_ctx* c, const double* src, double2* dst) {
+void phys2coll_dev(opf_ctx* c, const double* src, double2* dst, int nf) {
   const FftTiles& t = c->tiles;
   const int BT = lines_per_block(c);
-  HIPCHK(hipMemsetAsync(dst, 0, sizeof(double2) * c->G.coll, c->stream));
-  launch(c, "r2c_lines", k_r2c_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, 1), dim3(FFT_THREADS),
-         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, src, c->tY);
-  launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-         c->B, t, (const double2*)c->tY, dst);
+  const size_t fy = (size_t)c->N * c->Z * c->M, fp = (size_t)c->N * c->Z * c->T;
+  HIPCHK(hipMemsetAsync(dst, 0, sizeof(double2) * c->G.coll * nf, c->stream));
+  launch(c, "r2c_lines", k_r2c_lines, dim3((c->Z + 2 * BT - 1) / (2 * BT), c->N, nf), dim3(FFT_THREADS),
+         ((size_t)BT * padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, BT, src, c->nnY, fp, fy);
+  launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, nf), dim3(FFT_THREADS), c->smem_z, c->G,
+         c->B, t, (const double2*)c->nnY, dst, fy, (size_t)c->G.coll);
 }
 
 // The variable-viscosity plugins share their transforms: the gradient
@@ -353,6 +358,7 @@ NnPhys visc_fields(opf_ctx* c) {
   const size_t np = (size_t)c->N * c->Z * c->T;
   if (!c->nnP) c->nnP = c->alloc<double>(16 * np);
   if (!c->nnC) c->nnC = c->alloc<double2>(4 * c->G.coll);
+  if (!c->nnY) c->nnY = c->alloc<double2>(3 * (size_t)c->N * c->Z * c->M);
   NnPhys P{};
   for (int i = 0; i < 9; ++i) P.S[i] = c->nnP + (size_t)i * np;
   P.nu = c->nnP + 9 * np;
@@ -364,25 +370,25 @@ void visc_gradient(opf_ctx* c, int slot, const NnPhys& P) {
   double2* c3 = c->nnC;
   for (int row = 0; row < 3; ++row) {          // A(row, j), velocity-gradient rows
     launch(c, "nn_grad", k_nn_grad, dim3((c->H + 63) / 64, c->N), dim3(64), 0, c->G, c->B, slot, row, c3);
-    for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.S[row * 3 + j]);
+    coll2phys_dev(c, c3, P.S[row * 3], 3);
   }
 }
 void visc_derivatives(opf_ctx* c, int slot, const NnPhys& P) {
   double2* c3 = c->nnC;
   double2* nuc = c->nnC + 3 * c->G.coll;
   const dim3 gh((c->H + 63) / 64, c->N), b64(64);
-  phys2coll_dev(c, P.nu, nuc);
+  phys2coll_dev(c, P.nu, nuc, 1);
   launch(c, "nn_gradp", k_nn_gradp, gh, b64, 0, c->G, c->B, slot, (const double2*)nuc, c3);
-  for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.g[j]);
+  coll2phys_dev(c, c3, P.g[0], 3);
   launch(c, "nn_curlcurl", k_nn_curlcurl, gh, b64, 0, c->G, c->B, slot, c3);
-  for (int j = 0; j < 3; ++j) coll2phys_dev(c, c3 + (size_t)j * c->G.coll, P.q[j]);
+  coll2phys_dev(c, c3, P.q[0], 3);
 }
 void visc_add(opf_ctx* c, int slot, const NnPhys& P) {
-  for (int j = 0; j < 3; ++j) {
-    phys2coll_dev(c, P.q[j], c->tC);
+  double2* c3 = c->nnC;
+  phys2coll_dev(c, P.q[0], c3, 3);
+  for (int j = 0; j < 3; ++j)
     launch(c, "nn_addN", k_nn_addN, dim3((c->H + 63) / 64, c->N, 1), dim3(64), 0, c->G, c->B, slot, j,
-           (const double2*)c->tC);
-  }
+           (const double2*)(c3 + (size_t)j * c->G.coll));
 }
 
 // util(2) of the non-Newtonian plugin for one slot: get_nu and
@@ -1277,7 +1283,7 @@ int opf_get_phys(opf_ctx* c, int slot, int which, double* phys) {
                                    : c->B.om + (size_t)slot * 3 * c->G.coll + (size_t)(which - 3) * c->G.coll;
     const FftTiles& t = c->tiles;
     launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-           c->B, t, src, c->tY);
+           c->B, t, src, c->tY, (size_t)0, (size_t)0);
     launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS),
            (padded_len(c->T) + c->T) * sizeof(double2), c->G, c->B, t, (const double2*)c->tY, c->tP);
     HIPCHK(hipMemcpyAsync(phys, c->tP, sizeof(double) * c->N * c->Z * c->T, hipMemcpyDeviceToHost, c->stream));
@@ -1313,7 +1319,7 @@ int opf_tra_coll2phys(opf_ctx* c, const double* cre, const double* cim, double* 
     coll_to_dev(c, cre, cim, c->tC);
     const FftTiles& t = c->tiles;
     launch(c, "zfwd1", k_zfwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-           c->B, t, (const double2*)c->tC, c->tY);
+           c->B, t, (const double2*)c->tC, c->tY, (size_t)0, (size_t)0);
     launch(c, "c2r", k_theta_c2r, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), (padded_len(c->T) + c->T) * sizeof(double2), c->G,
            c->B, t, (const double2*)c->tY, c->tP);
     HIPCHK(hipMemcpyAsync(phys, c->tP, sizeof(double) * c->N * c->Z * c->T, hipMemcpyDeviceToHost, c->stream));
@@ -1331,7 +1337,7 @@ int opf_tra_phys2coll(opf_ctx* c, const double* phys, double* cre, double* cim) 
     launch(c, "r2c", k_theta_r2c, dim3((c->Z + 1) / 2, c->N, 1), dim3(FFT_THREADS), (padded_len(c->T) + c->T) * sizeof(double2), c->G,
            c->B, t, (const double*)c->tP, c->tY);
     launch(c, "zbwd1", k_zbwd_one, dim3((c->M + t.tm - 1) / t.tm, c->N, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-           c->B, t, (const double2*)c->tY, c->tC);
+           c->B, t, (const double2*)c->tY, c->tC, (size_t)0, (size_t)0);
     dev_to_coll(c, c->tC, cre, cim);
   });
 }

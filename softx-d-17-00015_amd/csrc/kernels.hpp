@@ -120,8 +120,10 @@ __global__ void k_les_visc(NnPhys P, size_t np, int N, int ZT, int model, double
                            unsigned long long* numax_slot);
 __global__ void k_les_nur(Bufs B, int slot, const unsigned long long* numax);
 __global__ void k_les_force(NnPhys P, size_t np, Bufs B, int slot);
-__global__ void k_c2r_lines(Geo G, Bufs B, FftTiles T, int BT, const double2* Y, double* phys);
-__global__ void k_r2c_lines(Geo G, Bufs B, FftTiles T, int BT, const double* phys, double2* Y);
+__global__ void k_c2r_lines(Geo G, Bufs B, FftTiles T, int BT, const double2* Y, double* phys, size_t fy,
+                            size_t fp);
+__global__ void k_r2c_lines(Geo G, Bufs B, FftTiles T, int BT, const double* phys, double2* Y, size_t fp,
+                            size_t fy);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
@@ -137,9 +139,9 @@ struct FftKerns {
 };
 FftKerns fft_kernels(const FftPlan& pz, const FftPlan& pt);
 // materialise physical fields / transform arbitrary coll fields (granular API)
-__global__ void k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y);
+__global__ void k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y, size_t fa, size_t fy);
 __global__ void k_theta_c2r(Geo G, Bufs B, FftTiles T, const double2* Y, double* phys);
 __global__ void k_theta_r2c(Geo G, Bufs B, FftTiles T, const double* phys, double2* Y);
-__global__ void k_zbwd_one(Geo G, Bufs B, FftTiles T, const double2* Y, double2* a);
+__global__ void k_zbwd_one(Geo G, Bufs B, FftTiles T, const double2* Y, double2* a, size_t fy, size_t fa);
 
 }  // namespace opf

@@ -526,7 +526,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles 
 // Granular transforms of one coll field (tra_coll2phys / tra_phys2coll of the
 // drop-in interface): spec -> Y[n][z][m] -> phys[n][z][th] and back.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(FFT_THREADS) k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y) {
+__global__ void __launch_bounds__(FFT_THREADS) k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y,
+                                                          size_t fa, size_t fy) {
+  a += blockIdx.z * fa;   // field blockIdx.z of a batch at strides fa, fy
+  Y += blockIdx.z * fy;
   const int n = blockIdx.y;
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
@@ -587,7 +590,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_c2r(Geo G, Bufs B, FftTil
 // [n][z][theta] (theta fastest, so every line is one contiguous run).
 // grid: x = blocks of 2 BT z-lines, y = N.
 __global__ void __launch_bounds__(FFT_THREADS) k_c2r_lines(Geo G, Bufs B, FftTiles T, int BT, const double2* Y,
-                                                           double* phys) {
+                                                           double* phys, size_t fy, size_t fp) {
+  Y += blockIdx.z * fy;   // field blockIdx.z of a batch at strides fy, fp
+  phys += blockIdx.z * fp;
   const int n = blockIdx.y;
   const int M = G.M, TT = G.T, Z = G.Z;
   const int Tp = padded_len(TT);
@@ -629,7 +634,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_lines(Geo G, Bufs B, FftTil
 // The inverse of k_c2r_lines with the scaling of tra_phys2spec (1/(3K 3M),
 // transform.fftw3.f90:114), into the z-transform intermediates Y.
 __global__ void __launch_bounds__(FFT_THREADS) k_r2c_lines(Geo G, Bufs B, FftTiles T, int BT, const double* phys,
-                                                           double2* Y) {
+                                                           double2* Y, size_t fp, size_t fy) {
+  phys += blockIdx.z * fp;   // field blockIdx.z of a batch at strides fp, fy
+  Y += blockIdx.z * fy;
   const int n = blockIdx.y;
   const int M = G.M, TT = G.T, Z = G.Z;
   const int Tp = padded_len(TT);
@@ -680,7 +687,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_r2c(Geo G, Bufs B, FftTil
   }
 }
 
-__global__ void __launch_bounds__(FFT_THREADS) k_zbwd_one(Geo G, Bufs B, FftTiles T, const double2* Y, double2* a) {
+__global__ void __launch_bounds__(FFT_THREADS) k_zbwd_one(Geo G, Bufs B, FftTiles T, const double2* Y, double2* a,
+                                                          size_t fy, size_t fa) {
+  Y += blockIdx.z * fy;   // field blockIdx.z of a batch at strides fy, fa
+  a += blockIdx.z * fa;
   const int n = blockIdx.y;
   const int m0 = blockIdx.x * T.tm;
   const int tm = min(T.tm, G.M - m0);
