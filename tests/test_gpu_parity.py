@@ -271,3 +271,40 @@ def test_pairs_per_workgroup_agree(opf, name, monkeypatch):
     rs = g["series"]
     assert np.max(np.abs(d[:, 2] - rs[:, 2]) / rs[:, 2]) <= TOL_DIAG
     assert np.array_equal(d[:, 8].astype(int), rs[:, 8].astype(int))
+
+
+@pytest.mark.parametrize("dt,expect", [(0.1, None), (0.2, 2), (0.4, 3)])
+def test_corrector_decisions_at_large_dt(dt, expect):
+    """tim_check_cgce's branches in the device loop at (16, 6, 4): dt = 0.1
+    needs 4-5 correctors a step (states and corrector counts against the
+    oracle over 20 steps); dt = 0.2 fails at step 0 with 'too many its'
+    (status 2) and dt = 0.4 with 'increasing error' (status 3), as the
+    oracle's tim_check_cgce does."""
+    import opf
+    import opf_oracle as O
+    from opf._lib import OpfError
+    g = load_golden("mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    ctx = opf.Context(N, K, M)
+    ctx.setup(0, float(g["Re"]), float(g["alpha"]), dt)
+    ctx.put_state(0, *g["ic"])
+    o = O.Ctx(N, K, M, Re=float(g["Re"]), alpha=float(g["alpha"]), dt=dt)
+    o.set_state(*g["ic"])
+    print("large dt", dt, flush=True)
+    if expect is None:
+        ctx.run(20, diag_every=1)
+        ser = o.run(20)
+        d = ctx.diag(0)
+        assert np.array_equal(d[:, 8].astype(int), np.array([s["ncorr"] for s in ser]))
+        assert max(s["ncorr"] for s in ser) >= 4
+        u = ctx.get_state(0)
+        for c, ref in enumerate((o.ur, o.ut, o.uz)):
+            assert rel_err(u[c], ref) <= 1e-10, c
+    else:
+        with pytest.raises(OpfError):
+            ctx.run(5)
+        st = ctx.status(0)
+        assert (st["status"], st["step"]) == (expect, 0)
+        with pytest.raises(RuntimeError, match="too many its" if expect == 2 else "increasing error"):
+            o.run(5)
+        assert o.step == 0

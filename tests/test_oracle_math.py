@@ -433,3 +433,37 @@ def test_les_smagorinsky_viscosity_on_laminar_flow(filt):
     assert abs(ctx.nu - 1.1 * (1 + Re * ctx.les_nu.max())) <= 1e-15 * ctx.nu
     s = np.abs(nz[:, 0]).max()
     assert np.abs(nz[:, 1:]).max() <= 1e-13 * s and np.abs(nr).max() <= 1e-13 * s and np.abs(nt).max() <= 1e-13 * s
+
+
+def _cgce(ctx, seq, dt=0.01, step=0):
+    """Feed tim_check_cgce a sequence of corrector errors from it = 1."""
+    ctx.dt, ctx.step, ctx.it = dt, step, 1
+    out = []
+    for e in seq:
+        ctx.dterr = e
+        st = ctx.check_cgce()
+        out.append((st, ctx.it))
+        if ctx.it == 0:
+            break
+    return out
+
+
+def test_check_cgce_decisions():
+    """tim_check_cgce (timestep.f90:214-253) on synthetic corrector errors,
+    d_dterr = 1e-5: convergence sets tim_corr_dt = dt sqrt(tol/err1); a
+    decreasing error above tol iterates; eleven iterations fail ('too many
+    its'); a growing error fails above 2 tol and otherwise stops with
+    tim_corr_dt = dt/(10 d_courant); dt < 1e-9 after step 30 fails."""
+    c = O.Ctx(9, 2, 1)
+    assert _cgce(c, [1e-6]) == [("ok", 0)]
+    assert abs(c.corr_dt - 0.01 * math.sqrt(10.0)) <= 1e-15
+    assert _cgce(c, [1e-4, 5e-5, 1e-6]) == [("ok", 2), ("ok", 3), ("ok", 0)]
+    assert abs(c.corr_dt - 0.01 * math.sqrt(0.1)) <= 1e-15          # from the first error only
+    seq = [1e-3 / (1 + i) for i in range(12)]
+    out = _cgce(c, seq)
+    assert out[-1] == ("too many its", 0) and len(out) == 11
+    assert _cgce(c, [1e-4, 2e-4]) == [("ok", 2), ("increasing error", 0)]
+    assert _cgce(c, [1.5e-5, 1.8e-5]) == [("ok", 2), ("ok", 0)]
+    assert abs(c.corr_dt - 0.01 / 5.0) <= 1e-17
+    assert _cgce(c, [1e-6], dt=1e-10, step=31) == [("dt->0", 0)]
+    assert _cgce(c, [1e-6], dt=1e-10, step=30) == [("ok", 0)]
