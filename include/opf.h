@@ -236,7 +236,7 @@ int opf_get_phys(opf_ctx* ctx, int slot, int which, double* phys);
 int opf_add_nonlinear(opf_ctx* ctx, int slot, const double* nr_re, const double* nr_im, const double* nt_re,
                       const double* nt_im, const double* nz_re, const double* nz_im);
 
-/* tra_coll2phys = var_coll2spec + tra_spec2phys/* tra_coll2phys = var_coll2spec + tra_spec2phys (variables.f90:184-197,
+/* tra_coll2phys = var_coll2spec + tra_spec2phys (variables.f90:184-197,
  * transform.fftw3.f90:65-102) and tra_phys2coll = tra_phys2spec +
  * var_spec2coll (transform.fftw3.f90:108-147, variables.f90:281-294), on
  * host arrays of one field. */
