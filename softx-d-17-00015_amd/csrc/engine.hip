@@ -241,6 +241,11 @@ void launch(opf_ctx* c, const char* name, Kern k, dim3 grid, dim3 block, size_t 
 }
 
 dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); }
+// blocks of k_zbwd_t: its (m tile, n, field) tiles rounded up to 8 XCDs
+unsigned zbwd_blocks(const opf_ctx* c) {
+  const int gx = (c->M + c->tiles.tm - 1) / c->tiles.tm;
+  return (unsigned)(8 * ((gx * 3 * c->N + 7) / 8));
+}
 
 // One evaluation (vel_transform + vel_nonlinear without the base-flow
 // terms) for slots [s0, s0+ns).
@@ -250,7 +255,7 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
          c->G, c->B, t, s0);
   launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
          c->G, c->B, t, s0);
-  launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, ns), dim3(FFT_THREADS), c->smem_z,
+  launch(c, "zbwd", c->fk.zbwd, dim3(zbwd_blocks(c), 1, ns), dim3(FFT_THREADS), c->smem_z,
          c->G, c->B, t, s0);
 }
 
@@ -1121,7 +1126,7 @@ int opf_vel_nonlinear(opf_ctx* c, int slot) {
       const FftTiles& t = c->tiles;
       launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, 1), dim3(FFT_THREADS), c->smem_t, c->G,
              c->B, t, slot);
-      launch(c, "zbwd", c->fk.zbwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 3, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+      launch(c, "zbwd", c->fk.zbwd, dim3(zbwd_blocks(c), 1, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
       if (c->nn[slot].on) launch_nonnewtonian(c, slot);   // var_null(2)
       if (c->les[slot].on) launch_les(c, slot);

@@ -506,8 +506,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles 
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
   constexpr int LZ = plan_len<R0, R1, R2>();
-  const int n = blockIdx.y / 3, f = blockIdx.y % 3;
-  const int m0 = blockIdx.x * T.tm;
+  // XCD-aware tiles: launch order b goes to XCD b % 8, so tile t = (b % 8)
+  // per + b / 8 keeps runs of neighbouring tiles (which share cache lines of
+  // Yb) on one XCD's L2.  grid x = 8 per >= tiles; spare blocks return.
+  const int gx = (G.M + T.tm - 1) / T.tm, ntile = gx * 3 * G.N;
+  const int per = (ntile + 7) / 8;
+  const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (tile >= ntile) return;
+  const int ty = tile / gx, tx = tile - ty * gx;
+  const int n = ty / 3, f = ty % 3;
+  const int m0 = tx * T.tm;
   const int tm = min(T.tm, G.M - m0);
   const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
   const double2* Y = B.Yb + (((size_t)slot * G.N + n) * Z) * 3 * M + (size_t)f * M;
