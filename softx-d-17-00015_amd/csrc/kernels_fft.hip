@@ -430,7 +430,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   const bool cor = B.prm[slot].coriolis != 0;
   const double cor_d1 = B.prm[slot].cor_d1, cor_s = B.prm[slot].cor_sphi, cor_c = B.prm[slot].cor_cphi;
   double mr = 0.0, mt = 0.0, mz = 0.0;
-  for (int i = threadIdx.x; i < tz * TT; i += FFT_THREADS) {
+  double nzv[TLD];   // N_z of this thread's items, written packed after a barrier
+#pragma unroll
+  for (int u = 0; u < TLD; ++u) {
+    const int i = threadIdx.x + u * FFT_THREADS;
+    if (i >= tz * TT) continue;
     const int line = i / TT, th = i - line * TT;
     double2* L0 = buf + (size_t)line * Tp + pidx(th);
     const double2 w0 = L0[0], w1 = L0[(size_t)tz * Tp], w2 = L0[(size_t)2 * tz * Tp];
@@ -450,7 +454,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
       Nz += cor_d1 * (Or * ut - Ot * ur);
     }
     L0[0] = make_double2(scale * Nr, scale * Nt);
-    L0[(size_t)tz * Tp] = make_double2(scale * Nz, 0.0);
+    nzv[u] = scale * Nz;
   }
   // maxima are exact and order free: non-negative doubles order like their
   // bit patterns, so one integer atomic max per workgroup and component.
@@ -477,21 +481,34 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     }
   }
   __syncthreads();
-  run_fft<R0, R1, R2>(buf, 2 * tz, T.pt, -1.0, twl, B.twT);
+  // N_z of lines 2q and 2q + 1 as the real and imaginary parts of transform
+  // tz + q (two real transforms in one complex one; an odd last line pairs
+  // with zero)
+  const int nzt = (tz + 1) / 2;
+#pragma unroll
+  for (int u = 0; u < TLD; ++u) {
+    const int i = threadIdx.x + u * FFT_THREADS;
+    if (i >= tz * TT) continue;
+    const int line = i / TT, th = i - line * TT;
+    double* d = (double*)(buf + ((size_t)tz + (line >> 1)) * Tp + pidx(th));
+    d[line & 1] = nzv[u];
+    if ((line & 1) == 0 && line + 1 == tz) d[1] = 0.0;
+  }
+  __syncthreads();
+  run_fft<R0, R1, R2>(buf, tz + nzt, T.pt, -1.0, twl, B.twT);
   // unpack the real transforms: A(m) = (X(m) + conj X(-m))/2, B(m) = (X(m) - conj X(-m))/2i
   double2* Yb = B.Yb + (((size_t)slot * G.N + n) * G.Z + z0) * 3 * M;
   for (int i = threadIdx.x; i < tz * 3 * M; i += FFT_THREADS) {
     const int line = i / (3 * M), rem = i - line * 3 * M;
     const int f = rem / M, mm = rem - f * M;
     double2 v;
-    if (f == 2) {
-      v = buf[((size_t)tz + line) * Tp + pidx(mm)];
-    } else {
-      const double2* L = buf + (size_t)line * Tp;
-      const double2 X = L[pidx(mm)], Xc = L[pidx((TT - mm) % TT)];
-      if (f == 0) v = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
-      else v = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
-    }
+    // f = 0, 1: N_r, N_t of the line's own transform; f = 2: N_z, the real
+    // (even line) or imaginary (odd line) part of transform tz + line/2
+    const double2* L = buf + (f == 2 ? (size_t)tz + (line >> 1) : (size_t)line) * Tp;
+    const double2 X = L[pidx(mm)], Xc = L[pidx((TT - mm) % TT)];
+    const bool re = f == 0 || (f == 2 && (line & 1) == 0);
+    if (re) v = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
+    else v = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
     Yb[(size_t)line * 3 * M + f * M + mm] = v;
   }
 }
