@@ -56,10 +56,16 @@ int opf_version(void);
  * `capacity` states. */
 int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r);
 int opf_ctx_destroy(opf_ctx* ctx);
-/* H = number of harmonics; bytes = device memory held by the context. */
+/* H = number of harmonics; bytes = device memory held by the context now
+ * (including buffers allocated on first use of opf_vel_shift, opf_steporbit
+ * and the variable-viscosity plugins). */
 int opf_ctx_dims(const opf_ctx* ctx, int* N, int* K, int* M, int* Mp, int* H, int* capacity,
                  size_t* bytes);
-/* Device bytes a context of this shape would hold (admission control). */
+/* Device bytes a context of this shape holds after opf_ctx_create
+ * (admission control).  Allocated later, on first use: the shift tables
+ * ((2K + M) * 16 bytes per slot), and for opf_slot_nonnewtonian /
+ * opf_slot_les 16 physical fields of N*3K*3M doubles, 4 spectral fields and
+ * 3 fields of N*3K*M complex values per context, and 4N + 1 words per slot. */
 size_t opf_ctx_bytes_needed(int N, int K, int M, int capacity);
 
 /* mes_D of meshs.f90:48-58: r(1:N), intrdr(1:N), dr0(1:5,0:4), dr1(1:5,0:4)
