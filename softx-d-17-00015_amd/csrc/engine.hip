@@ -189,23 +189,37 @@ size_t step_smem(int N, int gp, int bufd2) {
 // Pairs per workgroup of the fused step kernel and its factor ring.  A
 // chunk block must hold one window of the widest U sweep (9 columns of 5
 // double2) plus its pad; 48 double2 per Helmholtz block gives 15-column
-// chunks for the common band widths.  Four pairs per workgroup (N <= 64)
+// chunks for the common band widths.  Four pairs per workgroup (N <= 96)
 // put 48 of the solver wave's 64 lanes to work in the Helmholtz solves
 // instead of 24 and halve the solve instructions per harmonic; they are the
 // default when the slots give at least three such workgroups per CU (51 KiB
-// of LDS each at N = 64), otherwise two pairs (26 KiB, six per CU).
-StepTiles choose_step_tiles(int N, int P, int cap, int ncu, int gp_req) {
+// of LDS each at N = 64), otherwise two pairs (26 KiB, six per CU).  Above
+// N = 80 four pairs need more than 64 KiB of LDS (69 KiB at N = 96, two
+// workgroups per CU), which the kernel requests explicitly; at
+// (96,256,128) they take the step kernel from 1.55 to 1.35 ms.
+StepTiles choose_step_tiles(int N, int P, int cap, int ncu, int gp_req, size_t lds_max) {
   StepTiles t{};
   int gp = gp_req;
-  if (gp <= 0) gp = (N <= 64 && (long long)((P + 3) / 4) * cap >= 3LL * ncu) ? 4 : 2;
+  if (gp <= 0)
+    gp = (N <= 96 && step_smem(N, 4, step_bufd2(4)) <= lds_max && (long long)((P + 3) / 4) * cap >= 3LL * ncu)
+             ? 4 : 2;
   if (gp != 1 && gp != 2 && gp != 4) throw OpfError(OPF_ERR_ARG, "step pairs per workgroup must be 1, 2 or 4");
-  if (gp == 4 && N > 64) throw OpfError(OPF_ERR_ARG, "four pairs per workgroup need N <= 64");
+  if (gp == 4 && N > 96) throw OpfError(OPF_ERR_ARG, "four pairs per workgroup need N <= 96");
   t.gp = gp;
   t.bufd2 = step_bufd2(gp);
   t.ngroups = (P + gp - 1) / gp;
   t.smem = step_smem(N, gp, t.bufd2);
-  if (t.smem > 64 * 1024) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit in LDS");
+  if (t.smem > lds_max) throw OpfError(OPF_ERR_ARG, "step kernel tiles do not fit in LDS");
   return t;
+}
+
+// the instantiation of the fused step kernel for the context's pairs per
+// workgroup and N (items per thread cover N * 2 * pairs)
+using StepKern = void (*)(Geo, Bufs, StepTiles, int);
+StepKern step_kernel(const opf_ctx* c) {
+  if (c->st.gp == 4) return c->N <= 64 ? k_step_g4 : k_step_g4x;
+  if (c->st.gp == 1) return k_step_g1;
+  return c->N <= 64 ? k_step_g2 : k_step_g2x;
 }
 
 void check_slot(opf_ctx* c, int slot) {
@@ -261,8 +275,8 @@ void launch_eval(opf_ctx* c, int s0, int ns) {
 
 void launch_step(opf_ctx* c, int s0, int ns) {
   const int per = (c->st.ngroups + 7) / 8;   // XCD-aware order inside the kernel
-  auto k = c->st.gp == 4 ? k_step_g4 : c->st.gp == 1 ? k_step_g1 : (c->N <= 64 ? k_step_g2 : k_step_g2x);
-  launch(c, "step", k, dim3(8 * per, 1, ns), dim3(step_threads(c->st.gp)), c->st.smem, c->G, c->B, c->st, s0);
+  launch(c, "step", step_kernel(c), dim3(8 * per, 1, ns), dim3(step_threads(c->st.gp)), c->st.smem, c->G, c->B,
+         c->st, s0);
   launch(c, "control", k_control, dim3(1, 1, ns), dim3(256), 0, c->G, c->B, s0);
 }
 
@@ -486,10 +500,14 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
   {
     const char* e1 = std::getenv("OPF_STEP_GP");
-    int dev = 0, ncu = 0;
+    int dev = 0, ncu = 0, lds = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    c->st = choose_step_tiles(N, c->P, cap, ncu, e1 ? std::atoi(e1) : 0);
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    c->st = choose_step_tiles(N, c->P, cap, ncu, e1 ? std::atoi(e1) : 0, (size_t)lds);
+    if (c->st.smem > 64 * 1024)   // dynamic LDS beyond 64 KiB is requested per kernel
+      HIPCHK(hipFuncSetAttribute((const void*)step_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)c->st.smem));
   }
   c->smem_z = ((size_t)c->tiles.tm * padded_len(c->Z) + c->Z) * sizeof(double2);       // data + twiddles
   c->smem_t = ((size_t)3 * c->tiles.tz * padded_len(c->T) + c->T) * sizeof(double2);

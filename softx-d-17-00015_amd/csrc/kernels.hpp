@@ -129,6 +129,7 @@ __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
 __global__ void k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 128
 __global__ void k_step_g4(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64, 256 threads
+__global__ void k_step_g4x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 96, 256 threads
 __global__ void k_control(Geo G, Bufs B, int slot0);
 
 // transforms: kernels specialised to the radix plans of (Z, T), chosen once

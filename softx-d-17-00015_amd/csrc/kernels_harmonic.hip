@@ -1062,13 +1062,21 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0) {
   step_body<2, 2, 128>(G, B, ST, slot0);
 }
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
+// 64 < N <= 128: at most four workgroups (two waves each) fit in a CU's LDS,
+// so two waves per SIMD; the register budget that allows keeps the four
+// items per thread out of scratch
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
 k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0) {
   step_body<2, 4, 128>(G, B, ST, slot0);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_step_g4(Geo G, Bufs B, StepTiles ST, int slot0) {
   step_body<4, 2, 256>(G, B, ST, slot0);
+}
+// 64 < N <= 96: about 69 KiB of LDS at N = 96, two workgroups per CU
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_step_g4x(Geo G, Bufs B, StepTiles ST, int slot0) {
+  step_body<4, 3, 256>(G, B, ST, slot0);
 }
 
 }  // namespace opf

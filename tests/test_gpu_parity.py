@@ -273,6 +273,33 @@ def test_pairs_per_workgroup_agree(opf, name, monkeypatch):
     assert np.array_equal(d[:, 8].astype(int), rs[:, 8].astype(int))
 
 
+@pytest.mark.parametrize("N", [80, 96])
+def test_four_pairs_beyond_64_points(opf, N, monkeypatch):
+    """Four pairs per workgroup for 64 < N <= 96 (at N = 96 the workgroup
+    asks for more than 64 KiB of LDS): the states and diagnostics after 20
+    steps equal those of one and two pairs per workgroup bit for bit."""
+    from opf import ic as icgen
+    K, M, alpha = 6, 4, 1.25
+    res = {}
+    for gp in (1, 2, 4):
+        monkeypatch.setenv("OPF_STEP_GP", str(gp))
+        print(f"N={N} pairs per workgroup={gp}", flush=True)
+        ctx = opf.Context(N, K, M)
+        assert ctx.step_pairs() == gp
+        ctx.setup(0, 2400.0, alpha, 0.01)
+        me = ctx.mesh()
+        ctx.put_state(0, *icgen.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, 5, 1e-3))
+        ctx.run(20, diag_every=1)
+        res[gp] = (ctx.get_state(0), ctx.diag(0))
+        ctx.close()
+    monkeypatch.delenv("OPF_STEP_GP")
+    assert len(res[2][1]) == 20
+    for gp in (1, 4):
+        for c in range(3):
+            assert np.array_equal(res[gp][0][c], res[2][0][c]), (gp, c)
+        assert np.array_equal(res[gp][1], res[2][1]), gp
+
+
 @pytest.mark.parametrize("dt,expect", [(0.1, None), (0.2, 2), (0.4, 3)])
 def test_corrector_decisions_at_large_dt(dt, expect):
     """tim_check_cgce's branches in the device loop at (16, 6, 4): dt = 0.1
