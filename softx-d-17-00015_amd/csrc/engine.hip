@@ -255,21 +255,22 @@ void launch(opf_ctx* c, const char* name, Kern k, dim3 grid, dim3 block, size_t 
 }
 
 dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); }
-// blocks of k_zbwd_t: its (m tile, n, field) tiles rounded up to 8 XCDs
-unsigned zbwd_blocks(const opf_ctx* c) {
+// blocks of k_zfwd_t / k_zbwd_t: their (m tile, n, field) tiles, 6 or 3
+// fields, rounded up to 8 XCDs
+unsigned ztile_blocks(const opf_ctx* c, int nfield) {
   const int gx = (c->M + c->tiles.tm - 1) / c->tiles.tm;
-  return (unsigned)(8 * ((gx * 3 * c->N + 7) / 8));
+  return (unsigned)(8 * ((gx * nfield * c->N + 7) / 8));
 }
 
 // One evaluation (vel_transform + vel_nonlinear without the base-flow
 // terms) for slots [s0, s0+ns).
 void launch_eval(opf_ctx* c, int s0, int ns) {
   const FftTiles& t = c->tiles;
-  launch(c, "zfwd", c->fk.zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, ns), dim3(FFT_THREADS), c->smem_z,
-         c->G, c->B, t, s0);
+  launch(c, "zfwd", c->fk.zfwd, dim3(ztile_blocks(c, 6), 1, ns), dim3(FFT_THREADS), c->smem_z, c->G, c->B, t,
+         s0);
   launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
          c->G, c->B, t, s0);
-  launch(c, "zbwd", c->fk.zbwd, dim3(zbwd_blocks(c), 1, ns), dim3(FFT_THREADS), c->smem_z,
+  launch(c, "zbwd", c->fk.zbwd, dim3(ztile_blocks(c, 3), 1, ns), dim3(FFT_THREADS), c->smem_z,
          c->G, c->B, t, s0);
 }
 
@@ -1132,8 +1133,8 @@ int opf_vel_transform(opf_ctx* c, int slot) {
   return guard([&] {
     with_active(c, slot, -1, [&] {
       const FftTiles& t = c->tiles;
-      launch(c, "zfwd", c->fk.zfwd, dim3((c->M + t.tm - 1) / t.tm, c->N * 6, 1), dim3(FFT_THREADS), c->smem_z, c->G,
-             c->B, t, slot);
+      launch(c, "zfwd", c->fk.zfwd, dim3(ztile_blocks(c, 6), 1, 1), dim3(FFT_THREADS), c->smem_z, c->G, c->B, t,
+             slot);
     });
   });
 }
@@ -1144,7 +1145,7 @@ int opf_vel_nonlinear(opf_ctx* c, int slot) {
       const FftTiles& t = c->tiles;
       launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, 1), dim3(FFT_THREADS), c->smem_t, c->G,
              c->B, t, slot);
-      launch(c, "zbwd", c->fk.zbwd, dim3(zbwd_blocks(c), 1, 1), dim3(FFT_THREADS), c->smem_z, c->G,
+      launch(c, "zbwd", c->fk.zbwd, dim3(ztile_blocks(c, 3), 1, 1), dim3(FFT_THREADS), c->smem_z, c->G,
              c->B, t, slot);
       if (c->nn[slot].on) launch_nonnewtonian(c, slot);   // var_null(2)
       if (c->les[slot].on) launch_les(c, slot);

@@ -323,7 +323,7 @@ extern __shared__ double2 smem[];
 
 // ---------------------------------------------------------------------------
 // z transform of u and curl u for all active slots.
-// grid: x = m tiles, y = N * 6 (plane, field), z = slot.
+// grid: x = 8 * per >= (m tile, plane, field) tiles, z = slot.
 // LDS: tm columns at stride Zp, then Z twiddles.
 // ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
@@ -333,14 +333,21 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
   if (!c->active) return;
   constexpr int LZ = plan_len<R0, R1, R2>();
   constexpr int TMC = LZ ? tile_m(LZ) : 0;
-  const int n = blockIdx.y / 6, f = blockIdx.y % 6;
-  const int m0 = blockIdx.x * T.tm;
-  const int tm = min(T.tm, G.M - m0);
-  const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
   // a new evaluation: clear the slot's per-plane velocity maxima (the theta
   // kernel that follows records them)
-  if (blockIdx.x == 0 && blockIdx.y == 0)
+  if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < 3 * G.N; i += FFT_THREADS) B.cflmax[(size_t)slot * 3 * G.N + i] = 0ull;
+  // XCD-aware tiles as in k_zbwd_t: runs of neighbouring m tiles, which
+  // share cache lines of Y and of the coll rows, stay on one XCD's L2
+  const int gx = (G.M + T.tm - 1) / T.tm, ntile = gx * 6 * G.N;
+  const int per = (ntile + 7) / 8;
+  const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (tile >= ntile) return;
+  const int ty = tile / gx, tx = tile - ty * gx;
+  const int n = ty / 6, f = ty % 6;
+  const int m0 = tx * T.tm;
+  const int tm = min(T.tm, G.M - m0);
+  const int Z = LZ ? LZ : G.Z, K = LZ ? LZ / 3 : G.K, M = G.M;
   const double2* src = (f < 3 ? B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll
                               : B.om + (size_t)slot * 3 * G.coll + (size_t)(f - 3) * G.coll) +
                        (size_t)n * G.Hp;
