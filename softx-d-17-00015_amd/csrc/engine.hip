@@ -174,6 +174,7 @@ FftTiles choose_tiles(int Z, int T, int M) {
   if (Z > FFT_TILE || 3 * T > FFT_TILE) throw OpfError(OPF_ERR_ARG, "transform length too large for the FFT tiles");
   t.tm = std::max(1, std::min(M, tile_m(Z)));
   t.tz = std::max(1, std::min(Z, FFT_TILE / (3 * T)));
+  if (t.tz == 1 && Z >= 2 && 2 * T <= THETA_ITEMS) t.tz = 2;   // long theta lines
   return t;
 }
 

@@ -82,6 +82,11 @@ constexpr int FC_TAIL = 16;        // double2 after the factor ring: the solvers
 
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_TILE = 2048;   // complex values per transform workgroup (all its lines)
+// items (line, theta point) one theta workgroup holds in registers: FFT_TILE / 3
+// rounded up to whole rounds of FFT_THREADS (768).  Lines of 3M > FFT_TILE / 2
+// would leave one line per workgroup under FFT_TILE; they take two lines
+// when 2 * 3M fits in these items (up to 3M = 384).
+constexpr int THETA_ITEMS = ((FFT_TILE / 3 + FFT_THREADS - 1) / FFT_THREADS) * FFT_THREADS;
 
 // LDS stride of one transform of length L in the transform kernels: one pad
 // slot per 16 elements, rounded up to an odd count.

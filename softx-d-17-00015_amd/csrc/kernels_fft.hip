@@ -133,17 +133,17 @@ __device__ __forceinline__ void bfly(double2 (&v)[R], double sgn, const double2*
 __host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
 // One Stockham pass of radix R over B transforms of length L stored at
-// buf[b*Lp + pidx(i)] (B*L <= FFT_TILE).  Ns = product of the radices of the
+// buf[b*Lp + pidx(i)] (B*L <= CAP).  Ns = product of the radices of the
 // previous passes (the twiddles of the first pass are all 1 and skipped);
 // tw: the length-L twiddle table in LDS, twg: the same table in global
 // memory, read with uniform indices for the butterflies' internal twiddles.
 // Outer twiddles are fetched four at a time to bound register use.
 // LC, NSC: the length and Ns at compile time (0: the run-time L_, Ns_), so
 // that the index divisions become multiplications by constants.
-template <int R, int LC, int NSC>
+template <int R, int LC, int NSC, int CAP = FFT_TILE>
 __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, double sgn, const double2* tw,
                                          const double2* twg) {
-  constexpr int MB = (FFT_TILE + R * FFT_THREADS - 1) / (R * FFT_THREADS);
+  constexpr int MB = (CAP + R * FFT_THREADS - 1) / (R * FFT_THREADS);
   const int L = LC ? LC : L_;
   const int Ns = LC ? NSC : Ns_;
   const int Lp = padded_len(L);
@@ -197,19 +197,20 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
 }
 
 // Full transform: sgn = +1 (to physical space) or -1, unnormalised (FFTW).
+template <int CAP = FFT_TILE>
 __device__ __forceinline__ void fft_lds(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
   int Ns = 1;
   for (int p = 0; p < pl.nrad; ++p) {
     switch (pl.rad[p]) {
-      case 16: fft_pass<16, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 12: fft_pass<12, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 9: fft_pass<9, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 8: fft_pass<8, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 6: fft_pass<6, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 4: fft_pass<4, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 3: fft_pass<3, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      default: fft_pass<2, 0, 0>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 16: fft_pass<16, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 12: fft_pass<12, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 9: fft_pass<9, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 8: fft_pass<8, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 6: fft_pass<6, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 4: fft_pass<4, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 3: fft_pass<3, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      default: fft_pass<2, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
     }
     Ns *= pl.rad[p];
   }
@@ -223,17 +224,18 @@ constexpr int plan_len() {
 
 // Transform with the radix sequence fixed at compile time (R0 = 0: the
 // plan's own sequence).  Each pass then keeps only its own registers live,
-// and lengths and strides are constants.
-template <int R0, int R1, int R2>
+// and lengths and strides are constants.  CAP: the most values (B * L) a
+// call transforms.
+template <int R0, int R1, int R2, int CAP = FFT_TILE>
 __device__ __forceinline__ void run_fft(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
   if constexpr (R0 == 0) {
-    fft_lds(buf, B, pl, sgn, tw, twg);
+    fft_lds<CAP>(buf, B, pl, sgn, tw, twg);
   } else {
     constexpr int L = plan_len<R0, R1, R2>();
-    fft_pass<R0, L, 1>(buf, L, B, 1, sgn, tw, twg);
-    if constexpr (R1 > 1) fft_pass<R1, L, R0>(buf, L, B, R0, sgn, tw, twg);
-    if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1>(buf, L, B, R0 * R1, sgn, tw, twg);
+    fft_pass<R0, L, 1, CAP>(buf, L, B, 1, sgn, tw, twg);
+    if constexpr (R1 > 1) fft_pass<R1, L, R0, CAP>(buf, L, B, R0, sgn, tw, twg);
+    if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1, CAP>(buf, L, B, R0 * R1, sgn, tw, twg);
   }
 }
 
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
 // the two forward transforms of every line are the first 2*tz transforms;
 // then T twiddles.
 // ---------------------------------------------------------------------------
-constexpr int TLD = (FFT_TILE / 3 + FFT_THREADS - 1) / FFT_THREADS;
+constexpr int TLD = THETA_ITEMS / FFT_THREADS;
 template <int R0, int R1, int R2>
 __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles T, int slot0) {
   const int slot = slot0 + blockIdx.z;
@@ -391,7 +393,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   // pack (u_r,u_t), (u_z,w_r), (w_t,w_z) as a + ib into full Hermitian
   // spectra: item (line, pair, m) fills bins m and T-m; c2r reads only the
   // real part of the m = 0 bin (transform.fftw3.f90:97-98)
-  const int total = tz * 3 * M;   // <= FFT_TILE / 3 <= TLD * FFT_THREADS
+  const int total = tz * 3 * M;   // = tz * T <= THETA_ITEMS = TLD * FFT_THREADS
   double2 A[TLD], Bv[TLD];
   int it[TLD];
 #pragma unroll
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     buf[((size_t)pr * tz + line) * Tp + pidx(mm)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  run_fft<R0, R1, R2>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
+  run_fft<R0, R1, R2, 3 * THETA_ITEMS>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
   // products (velocity.f90:325-329), scaled as tra_phys2spec (:114-117);
   // the plane's max |u_r|, |u_t|, |u_z + 1 - r^2| for vel_maxtstep
   // (velocity.f90:464-475)
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     if ((line & 1) == 0 && line + 1 == tz) d[1] = 0.0;
   }
   __syncthreads();
-  run_fft<R0, R1, R2>(buf, tz + nzt, T.pt, -1.0, twl, B.twT);
+  run_fft<R0, R1, R2, 3 * THETA_ITEMS>(buf, tz + nzt, T.pt, -1.0, twl, B.twT);
   // unpack the real transforms: A(m) = (X(m) + conj X(-m))/2, B(m) = (X(m) - conj X(-m))/2i
   double2* Yb = B.Yb + (((size_t)slot * G.N + n) * G.Z + z0) * 3 * M;
   for (int i = threadIdx.x; i < tz * 3 * M; i += FFT_THREADS) {

@@ -144,6 +144,31 @@ def test_nonlinear_and_stages_mini(opf):
     assert abs(d - g["series"][0, 9]) <= 1e-8 * g["series"][0, 9]
 
 
+@pytest.mark.parametrize("K", [6, 3])
+def test_nonlinear_long_theta_lines(opf, K):
+    """vel_nonlinear with 3M = 384 theta points, where the theta kernel
+    takes two lines per workgroup at its largest register tile (the theta
+    length of the (96,256,128) size), against the oracle's products and
+    transforms on the same state.  K = 3 gives an odd number of z lines, so
+    the last workgroup holds one line."""
+    from opf import ic as icgen
+    N, M, Re, alpha = 12, 128, 3000.0, 1.25
+    ctx = opf.Context(N, K, M)
+    ctx.setup(0, Re, alpha, 0.01)
+    me = ctx.mesh()
+    u = icgen.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, 7, 1e-2)
+    ctx.put_state(0, *u)
+    ctx.transform(0)
+    ctx.nonlinear(0)
+    got = ctx.get_nonlinear(0)
+    oc = O.Ctx(N, K, M, Re=Re, alpha=alpha, dt=0.01)
+    oc.set_state(*u)
+    oc.transform()
+    oc.nonlinear()
+    for c, ref in enumerate((oc.Nr, oc.Nt, oc.Nz)):
+        assert rel_err(got[c], ref) <= TOL_STEP, c
+
+
 @pytest.mark.parametrize("name", ["mini", "c0"])
 def test_one_step_golden(opf, name):
     g = load_golden(name)
