@@ -169,6 +169,28 @@ def test_nonlinear_long_theta_lines(opf, K):
         assert rel_err(got[c], ref) <= TOL_STEP, c
 
 
+def test_nonlinear_long_z_lines(opf):
+    """vel_nonlinear with 3K = 768 z points (the z length of the
+    (96,256,128) size), where a z-transform workgroup holds two m-columns,
+    against the oracle."""
+    from opf import ic as icgen
+    N, K, M, Re, alpha = 12, 256, 4, 3000.0, 0.4
+    ctx = opf.Context(N, K, M)
+    ctx.setup(0, Re, alpha, 0.01)
+    me = ctx.mesh()
+    u = icgen.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, 9, 1e-2)
+    ctx.put_state(0, *u)
+    ctx.transform(0)
+    ctx.nonlinear(0)
+    got = ctx.get_nonlinear(0)
+    oc = O.Ctx(N, K, M, Re=Re, alpha=alpha, dt=0.01)
+    oc.set_state(*u)
+    oc.transform()
+    oc.nonlinear()
+    for c, ref in enumerate((oc.Nr, oc.Nt, oc.Nz)):
+        assert rel_err(got[c], ref) <= TOL_STEP, c
+
+
 @pytest.mark.parametrize("name", ["mini", "c0"])
 def test_one_step_golden(opf, name):
     g = load_golden(name)
