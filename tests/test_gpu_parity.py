@@ -1,6 +1,6 @@
 """HIP path against the oracle and the reference's golden vectors.
 
-Bars (stated here): one-step spectral coefficients
+Bars (the target tolerances, stated here): one-step spectral coefficients
 <= 1e-12 normwise relative per field; energy and friction velocity series
 <= 1e-10 relative; identical corrector counts; factors and transforms
 <= 1e-13 relative.  Every call goes through the C ABI (libopf.so).
