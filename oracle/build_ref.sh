@@ -43,12 +43,14 @@ done
 cpp -traditional-cpp -P -I"$TMP/program" parameters.f90 > "$TMP/pp/parameters.f90"
 cd "$TMP/pp"
 FFLAGS="-O2"
+# the module arrays of the largest sizes exceed 2 GiB of static data
+if [ $((N * K * M)) -gt 1000000 ]; then FFLAGS="$FFLAGS -mcmodel=medium"; fi
 $FC $FFLAGS -c "$REF/cheby.f" -o cheby.o
 for f in parameters mpi meshs variables transform.fftw3 timestep velocity; do
   $FC $FFLAGS -c $f.f90 2>/dev/null
 done
 $FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
-$FC -o "$OUT/opf_ref_$NAME" ref_driver.o parameters.o mpi.o cheby.o meshs.o \
+$FC $FFLAGS -o "$OUT/opf_ref_$NAME" ref_driver.o parameters.o mpi.o cheby.o meshs.o \
     variables.o transform.fftw3.o timestep.o velocity.o \
     -L/opt/conda/lib -lmkl_rt -Wl,-rpath,/opt/conda/lib
 echo "built $OUT/opf_ref_$NAME (N=$N K=$K M=$M Mp=$MP)" >&2

@@ -25,8 +25,11 @@
 !                                  and after each corrector)
 !   out_prefix_state_XXXXXX.bin    state after the listed steps
 !   out_prefix_timing.txt          wall seconds for the time loop
-!   nsteps < 0: no time loop; out_prefix_sym_I.bin holds var_imposesym(I)
+!   nsteps = -1: no time loop; out_prefix_sym_I.bin holds var_imposesym(I)
 !   (variables.f90:538-618) of the input state for I = 0..3.
+!   nsteps = -2: no time loop; out_prefix_mesh.bin holds mes_D of
+!   meshs.f90:48-58 as mes_precompute left it: r(:,1), intrdr, dr0, dr1,
+!   dr(1)%M, dr(2)%M, radLap%M (column-major float64, in that order).
 !   OPF_ORBIT="ndts T sth sz" in the environment: instead of the loop, the
 !   call sequence of steporbit (newton.f90:721-775): for ndts /= 1,
 !   tim_dt = T/ndts and vel_matrices; ndts steps of imposesym, transform,
@@ -72,6 +75,10 @@
    adaptive = dt_in < 0d0
    tim_dt = dabs(dt_in)
    call read_state(trim(infile))
+   if(nsteps == -2) then
+      call dump_mesh()
+      stop
+   end if
    if(nsteps < 0) then
       call dump_symmetries()
       stop
@@ -230,6 +237,15 @@
          call write_coll3(trim(prefix)//'_sym_'//ci//'.bin', a, b, c)
       end do
    end subroutine dump_symmetries
+
+   subroutine dump_mesh()
+      integer :: u
+      open(newunit=u, file=trim(prefix)//'_mesh.bin', access='stream', form='unformatted',  &
+         status='replace')
+      write(u) mes_D%r(:,1), mes_D%intrdr, mes_D%dr0, mes_D%dr1
+      write(u) mes_D%dr(1)%M, mes_D%dr(2)%M, mes_D%radLap%M
+      close(u)
+   end subroutine dump_mesh
 
    subroutine dump_stage(tag, a, b, c)
       character(*), intent(in) :: tag

@@ -728,12 +728,17 @@ void run_plan(opf_ctx* c, const std::vector<int>& slots, const std::vector<int>&
   const RunPlan p = run_begin(c, slots, nsteps, corr_fixed, diag_every, true);
   if (p.ns == 0) return;
   // Most steps take one corrector; enqueue 2 iterations per step, then top
-  // up until every slot has reached its target.
-  long chunk = 2L * p.max_steps;
-  for (int round = 0; round < 64 && chunk > 0; ++round) {
+  // up until every slot has reached its target.  A step takes at most 11
+  // iterations (a predictor and tim_check_cgce's 10 correctors), or fails.
+  const long budget = 11L * p.max_steps + 2;
+  long enqueued = 0;
+  long chunk = std::min(budget, 2L * p.max_steps);
+  int remaining = 1;
+  while (chunk > 0) {
     run_enqueue(c, p, chunk);
-    const int remaining = run_remaining(c, p);
-    chunk = remaining > 0 ? std::max(2L, 2L * remaining) : 0;
+    enqueued += chunk;
+    remaining = run_remaining(c, p);
+    chunk = remaining > 0 ? std::min(budget - enqueued, std::max(2L, 2L * remaining)) : 0;
   }
   auto ctl = read_ctl(c);
   for (int s : slots)
@@ -742,6 +747,11 @@ void run_plan(opf_ctx* c, const std::vector<int>& slots, const std::vector<int>&
       std::snprintf(msg, sizeof msg, "tim_check_cgce failure %d in slot %d at step %d", ctl[s].status, s, ctl[s].step);
       throw OpfError(OPF_ERR_CORRECTOR, msg);
     }
+  if (remaining > 0) {   // cannot happen while k_control keeps its corrector limit; never leave slots running
+    for (int s : p.slots) ctl[s].active = 0;
+    write_ctl(c, ctl);
+    throw OpfError(OPF_ERR_CORRECTOR, "time loop: steps left after the iteration budget");
+  }
 }
 
 void run_slots(opf_ctx* c, const std::vector<int>& slots, int nsteps, int corr_fixed, int diag_every) {
@@ -1591,7 +1601,27 @@ extern "C" int opf_steporbit(opf_ctx* c, int nslots, const int* slots, const int
       dth.push_back(ndts[i] != 1 ? -sth[i] : 0.0);
       dz.push_back(ndts[i] != 1 ? -sz[i] : 0.0);
     }
+    // multJ's trajectory direction (ndts = 1) leaves tim_t and tim_step as
+    // they were (newton.f90:759-762 advances them only for ndts /= 1)
+    std::vector<std::pair<int, double>> keep_t;
+    std::vector<int> keep_step;
+    {
+      const auto ctl = read_ctl(c);
+      for (int i = 0; i < nslots; ++i)
+        if (ndts[i] == 1) {
+          keep_t.push_back({slots[i], ctl[slots[i]].t});
+          keep_step.push_back(ctl[slots[i]].step);
+        }
+    }
     run_plan(c, sel, steps, fixed, 0);
     orbit_finish(c, sel, dth, dz);
+    if (!keep_t.empty()) {
+      auto ctl = read_ctl(c);
+      for (size_t i = 0; i < keep_t.size(); ++i) {
+        ctl[keep_t[i].first].t = keep_t[i].second;
+        ctl[keep_t[i].first].step = keep_step[i];
+      }
+      write_ctl(c, ctl);
+    }
   });
 }

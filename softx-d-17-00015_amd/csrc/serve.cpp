@@ -103,7 +103,10 @@ struct opf_server {
     planes(c, r->state.data(), p);
     int rc = opf_slot_setup(c.ctx, slot, r->Re, r->alpha, r->dt, 0.51, 1);
     if (rc == OPF_OK) rc = opf_put_state(c.ctx, slot, p[0], p[1], p[2], p[3], p[4], p[5], 0.0);
-    r->t_admit = now();
+    {
+      std::lock_guard<std::mutex> g(mu);
+      r->t_admit = now();
+    }
     if (rc != OPF_OK) retire(c, slot, r, rc);
   }
 

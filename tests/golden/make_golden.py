@@ -9,7 +9,7 @@ dumps (mini only) and its per-step series (t, E, Ek0, Em0, Pr0, Uc, Ufr,
 ncorr, dterr); adaptive runs (dt < 0) add the reference's per-step timestep
 quantities.  Fixtures are data only: inputs and the reference's outputs.
 
-usage: python tests/golden/make_golden.py [mini c0 s48 c1 ...]
+usage: python tests/golden/make_golden.py [mini c0 s48 c1 c2 mesh ...]
 """
 from __future__ import annotations
 
@@ -34,6 +34,7 @@ WORKLOADS = {
     "c0": (32, 24, 12, 2400.0, 1.25, 0.01, 1e-3, 1, 100, (1, 100), True),
     "s48": (48, 48, 24, 3000.0, 1.25, 0.01, 1e-3, 7, 50, (1,), False),
     "c1": (64, 64, 48, 5300.0, 0.5, 0.005, 1e-2, 1, 1000, (1,), False),
+    "c2": (96, 256, 128, 5300.0, 0.4, 0.0025, 1e-2, 1, 20, (1, 20), False),
     # adaptive time step (dt < 0: |dt| is the initial tim_dt, main.f90:24-29)
     "adapt_mini": (16, 6, 4, 2400.0, 1.25, -0.01, 1e-3, 1, 120, (1, 120), True),
     "adapt_c0": (32, 24, 12, 2400.0, 1.25, -0.01, 1e-3, 1, 200, (200,), True),
@@ -154,6 +155,43 @@ def make_orbit(name):
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
 
 
+# mes_D (meshs.f90:48-58) of the reference's mes_precompute for the radial
+# resolutions it is built at: executable name per N
+MESH_EXES = {16: "mini", 32: "c0", 48: "s48", 64: "c1", 96: "c2"}
+KL = 4
+
+
+def make_mesh(name):
+    """The reference's radial mesh, finite-difference bands and quadrature
+    weights, dumped by oracle/ref_driver.f90 (nsteps = -2)."""
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for N, exe_name in MESH_EXES.items():
+            exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{exe_name}")
+            if not os.path.exists(exe):
+                raise SystemExit(f"{exe} missing: run oracle/build_ref.sh")
+            K, M = WORKLOADS[{"s48": "s48"}.get(exe_name, exe_name)][1:3]
+            H = (2 * K - 1) * M - (K - 1)
+            icf = os.path.join(td, f"z{N}.bin")
+            np.zeros(6 * N * H).astype("<f8").tofile(icf)
+            pre = os.path.join(td, f"m{N}")
+            env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
+            subprocess.run([exe, icf, pre, "1.0", "1.0", "0.01", "-2"], check=True, cwd=td, env=env,
+                           stdout=subprocess.DEVNULL, preexec_fn=_unlimited_stack)
+            v = np.fromfile(pre + "_mesh.bin", dtype="<f8")
+            sizes = [("r", N), ("intrdr", N), ("dr0", (KL + 1) ** 2), ("dr1", (KL + 1) ** 2),
+                     ("dr1band", (2 * KL + 1) * N), ("dr2band", (2 * KL + 1) * N),
+                     ("radlap", (2 * KL + 1) * N)]
+            assert v.size == sum(n for _, n in sizes), (N, v.size)
+            o = 0
+            for key, n in sizes:
+                out[f"{key}_{N}"] = v[o:o + n].copy()
+                o += n
+    path = os.path.join(HERE, f"golden_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
 def _unlimited_stack():
     # The reference keeps large automatic coll temporaries (e.g. velocity.f90:417)
     # on the stack; its users run it with an unlimited stack.
@@ -214,8 +252,10 @@ def make(name):
 
 if __name__ == "__main__":
     for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini",
-                                "orbit_mini", "orbit_sr_mini", "newton_alg", "arnoldi_mat"]):
-        if nm == "newton_alg":
+                                "orbit_mini", "orbit_sr_mini", "newton_alg", "arnoldi_mat", "mesh", "c2"]):
+        if nm == "mesh":
+            make_mesh(nm)
+        elif nm == "newton_alg":
             make_newton(nm)
         elif nm == "arnoldi_mat":
             make_arnoldi(nm)

@@ -38,6 +38,12 @@ def test_steporbit_matches_reference(name, symm):
         ref = g[f"orbit_{i}"]
         for c in range(3):
             assert rel_err(u[c], ref[c]) <= 1e-11, (i, c)
+        # tim_t and tim_step advance only for ndts /= 1 (newton.f90:759-762)
+        st = ctx.status(0)
+        if int(ndts) == 1:
+            assert st["step"] == 0 and st["t"] == 0.0, st
+        else:
+            assert st["step"] == int(ndts) and abs(st["t"] - T) <= 1e-12 * T, st
 
 
 def test_steporbit_batched_equals_alone():

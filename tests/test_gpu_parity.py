@@ -243,6 +243,9 @@ def test_one_step_s48_sampled(opf):
 
 
 def test_c1_series(opf):
+    """(64,64,48): the sampled state after step 1, then the reference's whole
+    1000-step run: energy and friction velocity of every step <= 1e-10
+    relative, corrector counts equal."""
     g = load_golden("c1")
     ctx = _setup(opf, g)
     from opf import ic as icgen
@@ -255,9 +258,54 @@ def test_c1_series(opf):
     sel = g["sel"]
     for c in range(3):
         assert np.abs(u[c][:, sel] - g["state_1"][c]).max() / g["state_1_norm"][c] <= TOL_STEP
-    ctx.run(99, diag_every=1)
+    rs = g["series"]
+    assert rs.shape[0] == 1000
+    print("c1: 999 more steps", flush=True)
+    recs = {int(r[0]): r for r in ctx.diag(0)}
+    done = 1
+    while done < rs.shape[0]:             # the records ring holds 256 steps: read every 200
+        n = min(200, rs.shape[0] - done)
+        ctx.run(n, diag_every=1)
+        done += n
+        recs.update({int(r[0]): r for r in ctx.diag(0)})
+    d = np.array([recs[k] for k in sorted(recs)])
+    assert d.shape[0] == rs.shape[0]
+    assert np.array_equal(d[:, 0].astype(int), rs[:, 0].astype(int))
+    assert np.max(np.abs(d[:, 2] - rs[:, 2]) / rs[:, 2]) <= TOL_DIAG
+    assert np.max(np.abs(d[:, 7] - rs[:, 7]) / rs[:, 7]) <= TOL_DIAG
+    assert np.array_equal(d[:, 8].astype(int), rs[:, 8].astype(int))
+
+
+def test_c2_reference_run(opf):
+    """(96,256,128), alpha 0.4, Re 5300, dt 0.0025: the reference's own run
+    (oracle/_ref built at this size) from the same seeded state.  Sampled
+    spectral coefficients after step 1 <= 1e-12 and after step 20 <= 1e-10
+    normwise relative per field; energy and friction velocity of all 20
+    steps <= 1e-10 relative; equal corrector counts."""
+    g = load_golden("c2")
+    ctx = _setup(opf, g)
+    from opf import ic as icgen
+    me = ctx.mesh()
+    ur, ut, uz = icgen.seeded_perturbation(me["r"], me["intrdr"], ctx.K, ctx.M, float(g["alpha"]),
+                                           int(g["seed"]), float(g["E0"]))
+    sel = g["sel"]
+    for c, f in enumerate((ur, ut, uz)):
+        assert np.array_equal(f[:, sel], g["ic_sel"][c])
+    ctx.put_state(0, ur, ut, uz)
+    print("c2: step 1", flush=True)
+    ctx.run(1, diag_every=1)
+    u = ctx.get_state(0)
+    for c in range(3):
+        assert np.abs(u[c][:, sel] - g["state_1"][c]).max() / g["state_1_norm"][c] <= TOL_STEP, c
+    rs = g["series"]
+    print(f"c2: {rs.shape[0] - 1} more steps", flush=True)
+    ctx.run(rs.shape[0] - 1, diag_every=1)
+    u = ctx.get_state(0)
+    last = int(g["nsteps"])
+    for c in range(3):
+        assert np.abs(u[c][:, sel] - g[f"state_{last}"][c]).max() / g[f"state_{last}_norm"][c] <= TOL_DIAG, c
     d = ctx.diag(0)
-    rs = g["series"][:100]
+    assert d.shape[0] == rs.shape[0]
     assert np.max(np.abs(d[:, 2] - rs[:, 2]) / rs[:, 2]) <= TOL_DIAG
     assert np.max(np.abs(d[:, 7] - rs[:, 7]) / rs[:, 7]) <= TOL_DIAG
     assert np.array_equal(d[:, 8].astype(int), rs[:, 8].astype(int))

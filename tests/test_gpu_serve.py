@@ -82,3 +82,88 @@ def test_server_rejects_bad_requests():
     with pytest.raises(opf.serve.ServerError):
         srv.submit(0, 2400.0, 1.25, -0.01, 10, st)
     srv.close()
+
+
+# The serving mix: sizes (32,24,12), (48,48,24), (64,64,48) in ratio 5:3:2,
+# Poisson arrivals, Re ~ U[2000, 5300], alpha in {0.5, 1.25}, one corrector
+# per step.  Horizons are cut from T ~ U[5, 50] time units to U[0.1, 1.0]
+# at dt = 0.01 (10-100 steps) so that every request can also be run alone
+# within the test's time.
+MIX_CLASSES = [(32, 24, 12), (48, 48, 24), (64, 64, 48)]
+
+
+def _mix_requests(n, seed=77):
+    rng = np.random.default_rng(seed)
+    ratio = np.array([5.0, 3.0, 2.0])
+    out, t = [], 0.0
+    for i in range(n):
+        cls = int(rng.choice(3, p=ratio / ratio.sum()))
+        Re = float(rng.uniform(2000.0, 5300.0))
+        alpha = float(rng.choice([0.5, 1.25]))
+        nsteps = int(round(float(rng.uniform(0.1, 1.0)) / 0.01))
+        E0 = float(10.0 ** rng.uniform(-5.0, -3.0))
+        t += float(rng.exponential(1.0 / 40.0))
+        out.append(dict(cls=cls, Re=Re, alpha=alpha, dt=0.01, nsteps=nsteps, cf=1, E0=E0, seed=500 + i, at=t))
+    return out
+
+
+def test_serving_mix_alone_equals_in_mix_and_reference():
+    """Requests of the three serving sizes arrive as a Poisson stream while
+    others run.  Every result is bit for bit the same integration run alone
+    in a plain context.  Two requests carry the reference's own cases and
+    are integrated one step with the corrector to convergence: the
+    (48,48,24) one meets golden_s48 and the (64,64,48) one golden_c1 after
+    step 1 (sampled coefficients <= 1e-12 normwise relative)."""
+    import opf
+    reqs = _mix_requests(24)
+    meshes = [opf.host_mesh(N) for (N, _, _) in MIX_CLASSES]
+    from opf import ic
+    states = []
+    for r in reqs:
+        N, K, M = MIX_CLASSES[r["cls"]]
+        me = meshes[r["cls"]]
+        states.append(ic.seeded_perturbation(me["r"], me["intrdr"], K, M, r["alpha"], r["seed"], r["E0"]))
+    goldens = {1: load_golden("s48"), 2: load_golden("c1")}
+    for k, (cls, g) in enumerate(goldens.items()):
+        j = 3 + 7 * k
+        N, K, M = MIX_CLASSES[cls]
+        me = meshes[cls]
+        reqs[j].update(cls=cls, Re=float(g["Re"]), alpha=float(g["alpha"]), dt=float(g["dt"]), nsteps=1, cf=0,
+                       golden=cls)
+        states[j] = ic.seeded_perturbation(me["r"], me["intrdr"], K, M, float(g["alpha"]), int(g["seed"]),
+                                           float(g["E0"]))
+    srv = opf.Server(MIX_CLASSES, capacity=[6, 4, 2], chunk=10)
+    t0 = time.time()
+    ids = []
+    for r, st in zip(reqs, states):
+        wait = r["at"] - (time.time() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(r["cls"], r["Re"], r["alpha"], r["dt"], r["nsteps"], st, corr_fixed=r["cf"]))
+    print("serving mix: submitted", len(ids), "requests", flush=True)
+    deadline = time.time() + 120
+    while not all(srv.poll(i)["done"] for i in ids):
+        assert time.time() < deadline, srv.stats()
+        time.sleep(0.01)
+    results = []
+    for rid, r in zip(ids, reqs):
+        p = srv.poll(rid)
+        assert p["status"] == 0, (rid, p)
+        results.append(srv.result(rid, r["cls"]))
+    srv.close()
+    print("serving mix: checking each request alone", flush=True)
+    for r, st, u in zip(reqs, states, results):
+        N, K, M = MIX_CLASSES[r["cls"]]
+        ctx = opf.Context(N, K, M)
+        ctx.setup(0, r["Re"], r["alpha"], r["dt"])
+        ctx.put_state(0, *st[:3])
+        ctx.run(r["nsteps"], corr_fixed=r["cf"])
+        v = ctx.get_state(0)
+        ctx.close()
+        for c in range(3):
+            assert np.array_equal(u[c], v[c]), (r, c)
+        if "golden" in r:
+            g = goldens[r["golden"]]
+            sel = g["sel"]
+            for c in range(3):
+                assert np.abs(u[c][:, sel] - g["state_1"][c]).max() / g["state_1_norm"][c] <= 1e-12, (r, c)
