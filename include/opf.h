@@ -231,6 +231,11 @@ int opf_get_nonlinear(opf_ctx* ctx, int slot, double* nr_re, double* nr_im, doub
  * 2 u_z) of the slot's current state: E, Ek(0:K-1), Em(0:M-1) (NULL
  * skips).  io_write_energy sums E over the three components. */
 int opf_var_norm(opf_ctx* ctx, int slot, int comp, double* E, double* Ek, double* Em);
+/* var_coll_norm (variables.f90:718-751) of any field a in the coll layout
+ * (host arrays re, im), with d_alpha of the slot's parameters: E,
+ * Ek(0:K-1), Em(0:M-1) (NULL skips).  Runs on the device. */
+int opf_var_coll_norm(opf_ctx* ctx, int slot, const double* a_re, const double* a_im, double* E, double* Ek,
+                      double* Em);
 /* io_write_friction (io.f90:627-639): Ub, Uc, u_tau of the current state. */
 int opf_friction(opf_ctx* ctx, int slot, double* Ub, double* Uc, double* Ufr);
 /* vel_transform's physical fields (velocity.f90:304-317) of the current

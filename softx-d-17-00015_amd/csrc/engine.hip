@@ -113,6 +113,8 @@ struct opf_ctx {
   double* tP = nullptr;
   double2* tC = nullptr;
   double2* shiftTab = nullptr;   // var_coll_shift tables, (2K + M) per slot (first use)
+  double* normBuf = nullptr;     // opf_var_coll_norm: field, partials and sums (first use)
+  int device = 0;                // HIP device of the context
   Timing tm;
 
   template <typename T> T* alloc(size_t n) {
@@ -504,6 +506,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
     const char* e1 = std::getenv("OPF_STEP_GP");
     int dev = 0, ncu = 0, lds = 0;
     HIPCHK(hipGetDevice(&dev));
+    c->device = dev;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     c->st = choose_step_tiles(N, c->P, cap, ncu, e1 ? std::atoi(e1) : 0, (size_t)lds);
@@ -1269,6 +1272,43 @@ int opf_var_norm(opf_ctx* c, int slot, int comp, double* E, double* Ek, double* 
     if (E) *E = e;
     if (Ek) std::copy(ek.begin(), ek.end(), Ek);
     if (Em) std::copy(em.begin(), em.end(), Em);
+  });
+}
+
+// var_coll_norm's two passes (k_norm_harm, k_norm_sum) on device arrays:
+// the field (re, im in the coll layout), partials b[H], out[1 + K + M].
+static void launch_norm(opf_ctx* c, int slot, const double* re, const double* im, double* b, double* out,
+                 hipStream_t stream) {
+  hipLaunchKernelGGL(k_norm_harm, dim3((c->H + 255) / 256), dim3(256), 0, stream, c->G,
+                     (const SlotParams*)(c->B.prm + slot), re, im, b);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_norm_sum, dim3(1), dim3(256), 0, stream, c->G, (const double*)b, out);
+  HIPCHK(hipGetLastError());
+}
+
+int opf_var_coll_norm(opf_ctx* c, int slot, const double* re, const double* im, double* E, double* Ek,
+                      double* Em) {
+  // var_coll_norm (variables.f90:718-751) of any coll field, with d_alpha of
+  // the slot's parameters, on the device
+  return guard([&] {
+    check_slot(c, slot);
+    if (!re || !im) throw OpfError(OPF_ERR_ARG, "null field");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (alpha unknown)");
+    const size_t nh = (size_t)c->N * c->H, nout = 1 + (size_t)c->K + c->M;
+    if (!c->normBuf) c->normBuf = c->alloc<double>(2 * nh + c->H + nout);
+    double* dre = c->normBuf;
+    double* dim = dre + nh;
+    double* b = dim + nh;
+    double* out = b + c->H;
+    HIPCHK(hipMemcpyAsync(dre, re, nh * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dim, im, nh * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_norm(c, slot, dre, dim, b, out, c->stream);
+    std::vector<double> h(nout);
+    HIPCHK(hipMemcpyAsync(h.data(), out, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (E) *E = h[0];
+    if (Ek) std::copy(h.begin() + 1, h.begin() + 1 + c->K, Ek);
+    if (Em) std::copy(h.begin() + 1 + c->K, h.end(), Em);
   });
 }
 

@@ -130,6 +130,9 @@ __global__ void k_c2r_lines(Geo G, Bufs B, FftTiles T, int BT, const double2* Y,
 __global__ void k_r2c_lines(Geo G, Bufs B, FftTiles T, int BT, const double* phys, double2* Y, size_t fp,
                             size_t fy);
 __global__ void k_add_coll(Geo G, double2* dst, const double2* src);
+// var_coll_norm of a field in the coll layout: per-harmonic partials b[H], then [E, Ek, Em]
+__global__ void k_norm_harm(Geo G, const SlotParams* prm, const double* re, const double* im, double* b);
+__global__ void k_norm_sum(Geo G, const double* b, double* out);
 __global__ void k_step_g1(Geo G, Bufs B, StepTiles ST, int slot0);
 __global__ void k_step_g2(Geo G, Bufs B, StepTiles ST, int slot0);    // N <= 64
 __global__ void k_step_g2x(Geo G, Bufs B, StepTiles ST, int slot0);   // N <= 128

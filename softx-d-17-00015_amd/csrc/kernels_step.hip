@@ -364,6 +364,62 @@ __global__ void k_add_coll(Geo G, double2* dst, const double2* src) {
 }
 
 // ---------------------------------------------------------------------------
+// var_coll_norm (variables.f90:718-751) of a field in the reference's coll
+// layout (re, im: N x H, r fastest).  Pass 1, one thread per harmonic:
+// b(h) = sum_n f(n) intrdr(n) with f = w (re^2 + im^2), w = 4 pi^2 / alpha,
+// in n order, halved for (0,0).  Pass 2, one workgroup: Em(m) and Ek(|k|)
+// accumulate b in harmonic order (parallel.h:47-56), then E = sum(Em) in m
+// order.  out = [E, Ek(0:K-1), Em(0:M-1)].
+// ---------------------------------------------------------------------------
+__global__ void k_norm_harm(Geo G, const SlotParams* __restrict__ prm, const double* __restrict__ re,
+                            const double* __restrict__ im, double* __restrict__ b) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= G.H) return;
+  const double w = 4.0 * D_PI * D_PI / prm->alpha;
+  const double* intrdr = G.mesh + MeshOff::make(G.N).intrdr;
+  const double* a = re + (size_t)h * G.N;
+  const double* c = im + (size_t)h * G.N;
+  double s = 0.0;
+  for (int n = 0; n < G.N; ++n) {
+    const double f = w * (a[n] * a[n] + c[n] * c[n]);
+    s = s + f * intrdr[n];
+  }
+  b[h] = h == 0 ? 0.5 * s : s;
+}
+
+__global__ void k_norm_sum(Geo G, const double* __restrict__ b, double* __restrict__ out) {
+  const int t = threadIdx.x;
+  const int K = G.K, M = G.M, W = 2 * G.K - 1;
+  double* Ek = out + 1;
+  double* Em = out + 1 + K;
+  for (int m = t; m < M; m += blockDim.x) {   // the harmonics of m, k ascending
+    double s = 0.0;
+    if (m == 0) {
+      for (int k = 0; k < K; ++k) s = s + b[k];
+    } else {
+      const double* bm = b + K + (size_t)(m - 1) * W;
+      for (int i = 0; i < W; ++i) s = s + bm[i];
+    }
+    Em[m] = s;
+  }
+  for (int q = t; q < K; q += blockDim.x) {   // |k| = q: (q, 0), then (-q, m), (q, m) for m >= 1
+    double s = b[q];
+    for (int m = 1; m < M; ++m) {
+      const double* bm = b + K + (size_t)(m - 1) * W + (K - 1);
+      if (q > 0) s = s + bm[-q];
+      s = s + bm[q];
+    }
+    Ek[q] = s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double e = 0.0;
+    for (int m = 0; m < M; ++m) e = e + Em[m];
+    out[0] = e;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // vel_adjPPE(0) influence vectors (velocity.f90:191-218) per pair, and
 // vel_adjustFlux(0) (velocity.f90:160-168) on pair 0.
 // Uinf slots: 0 U1 (LDp), 1 U2 (LDm), 2 U3 (LDz), 3 U4, 4 U5, 5 p (U6 = p*alpha*k).

@@ -68,6 +68,7 @@ PROTOTYPES = {
     "opf_vel_matrices": (C.c_int, [vp, C.c_int, C.c_double]),
     "opf_step_config": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
+    "opf_var_coll_norm": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
     "opf_friction": (C.c_int, [vp, C.c_int, dp, dp, dp]),
     "opf_get_phys": (C.c_int, [vp, C.c_int, C.c_int, dp]),
     "opf_add_nonlinear": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),

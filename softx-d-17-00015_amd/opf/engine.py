@@ -239,6 +239,16 @@ class Context:
         check(self.lib.opf_var_norm(self.h, slot, comp, C.byref(E), _dptr(Ek), _dptr(Em)))
         return E.value, Ek, Em
 
+    def coll_norm(self, field, slot=0):
+        """var_coll_norm (variables.f90:718-751) of any field (N x H complex),
+        with the slot's alpha, on the device: (E, Ek[0:K], Em[0:M])."""
+        re, im = _split(field, self.N, self.H)
+        E = C.c_double()
+        Ek = np.zeros(self.K)
+        Em = np.zeros(self.M)
+        check(self.lib.opf_var_coll_norm(self.h, slot, _dptr(re), _dptr(im), C.byref(E), _dptr(Ek), _dptr(Em)))
+        return E.value, Ek, Em
+
     def friction(self, slot=0):
         """io_write_friction (io.f90:627-639): (Ub, Uc, u_tau)."""
         v = [C.c_double() for _ in range(3)]

@@ -1,22 +1,28 @@
 """Newton-Krylov-hookstep search for (relative) periodic orbits over the
-device time step: the client of newton.f90 with the solver of
-NewtonHook.f90 and GMRESm.f90.
+device time step: the client of newton.f90.
 
-The solver (``NewtonHook``) is generic: it finds a root of f(x) = 0 with
-GMRES(m) on the Jacobian action and Viswanath's hookstep in a trust region,
-exactly as newtonhook / gmresm / hookstep do (NewtonHook.f90:48-179,
-GMRESm.f90:27-153).  ``OrbitNewton`` is newton.f90's problem: the Newton
-vector holds, per shooting segment, the theta shift, z shift and period
-(scaled by scaleT) and the scaled velocity field (rtz2vec), f(x) = F(x) - x
-with F = steporbit (newton.f90:374-393, 721-775), and the Jacobian action by
-finite differences plus the constraints of no update along the shift
-directions and the trajectory (multJ, newton.f90:399-456).
+``HookstepNewton`` is a generic root finder for f(x) = 0: restarted GMRES
+(``Gmres``, Saad & Schultz 1986) on a Jacobian action, Viswanath's hookstep
+(``hookstep``) inside a trust region, and Dennis & Schnabel's region
+updates.  Its thresholds and the order of its decisions are those of the
+reference's solver (NewtonHook.f90:48-179, GMRESm.f90:27-153), so a search
+follows the reference's iterates; the parameters are in ``HookParams`` and
+``TrustRegionParams``.
+
+``OrbitNewton`` is newton.f90's problem: the Newton vector holds, per
+shooting segment, the theta shift, z shift and period (scaled by scaleT) and
+the scaled velocity field (rtz2vec), f(x) = F(x) - x with F the integration
+over one period followed by the shift back (newton.f90:374-393, 721-775),
+and the Jacobian action by finite differences with constraint rows that
+forbid updates along the shift directions and the trajectory
+(newton.f90:399-456).
 
 The time integrations run on the device through ``opf_steporbit``: the
 segments of multiple shooting advance together in one context (one slot
-each), so getrhs costs one batched integration rather than ms sequential
-ones.  Any object with the ``Stepper`` interface can stand in for the device
-(the tests use the oracle's CPU time step to check the device path).
+each), so a residual costs one batched integration rather than ms
+sequential ones.  Any object with the ``Stepper`` interface can stand in for
+the device (the tests use the oracle's CPU time step to check the device
+path).
 """
 from __future__ import annotations
 
@@ -31,199 +37,274 @@ D_PI = 3.1415926535897931
 
 
 # ---------------------------------------------------------------------------
-# NewtonHook.f90 / GMRESm.f90
+# Krylov solver, hookstep and trust-region Newton iteration
 # ---------------------------------------------------------------------------
+class HookParams:
+    """Parameters of the hookstep's search for the Levenberg-Marquardt
+    multiplier (NewtonHook.f90 / GMRESm.f90:121-153 use these values): the
+    search starts at max(sigma_min^2 * start_rel, start_abs) and grows the
+    multiplier by the factor `growth` until the step fits the region."""
+    start_rel = 1e-6
+    start_abs = 1e-99
+    growth = 1.1
+
+
+class TrustRegionParams:
+    """The trust-region rules of newtonhook (NewtonHook.f90:120-160), in
+    Dennis & Schnabel's terms: rho = actual / predicted reduction of |f|.
+    rho < accept_ratio: keep the trial as a fallback and shrink the region
+    to shrink_ok * |s|; the fallback's own region is |s| when
+    rho > poor_ratio, else halve * |s|.  |f| grew: shrink to halve * |s|.
+    A good step shorter than full_fraction of the region doubles the region
+    (expand * |s|, capped) and is taken; a good step that filled the region
+    is tried again with the region expanded while the region is below
+    full_fraction of its cap."""
+    accept_ratio = 0.75
+    poor_ratio = 0.1
+    shrink_ok = 0.7
+    halve = 0.5
+    full_fraction = 0.9
+    expand = 2.0
+
+
+class Status:
+    """Exit codes shared with newtonhook's info and gmresm's info."""
+    CONVERGED = 0
+    KRYLOV_STALLED = 1      # GMRES: the residual stopped decreasing
+    MAX_ITERATIONS = 2
+    REGION_COLLAPSED = 3    # Newton: trust region below its minimum
+
+
 def hookstep(h, j, beta, delta):
-    """hookstep (GMRESm.f90:121-153): the y minimising |beta e1 - H y| with
-    |y| <= delta (Viswanath 2008), from the SVD of the (j+1) x j Hessenberg
-    matrix; returns y and the predicted residual |beta e1 - H y|."""
-    a = h[:j + 1, :j]
-    u, s, vt = np.linalg.svd(a, full_matrices=True)
-    p = beta * u[0, :j]
-    mu = max(s[j - 1] * s[j - 1] * 1e-6, 1e-99)
-    qn = 1e99
-    q = np.zeros(j)
-    while qn > delta:
-        mu = mu * 1.1
-        q = p * s / (mu + s * s)
-        qn = math.sqrt(float(np.dot(q, q)))
-    y = q @ vt
-    r = -(a @ y)
-    r[0] = r[0] + beta
-    return y, math.sqrt(float(np.dot(r, r)))
+    """Viswanath's hookstep (Viswanath 2007, J. Fluid Mech. 580; also
+    arXiv:0809.1498): among the y with |y| <= delta, the one minimising
+    |beta e1 - Hbar y| for the (j+1) x j Hessenberg matrix Hbar = h[:j+1, :j].
+    With Hbar = U diag(sigma) V^T, y(mu) = V c with
+    c_i = g_i sigma_i / (sigma_i^2 + mu) and g = beta U^T e1; mu grows
+    geometrically from a small start until |c| <= delta.  Returns y and the
+    residual |beta e1 - Hbar y| it leaves (the predicted |f| after the step)."""
+    hb = h[:j + 1, :j]
+    u, sigma, vt = np.linalg.svd(hb, full_matrices=True)
+    g = beta * u[0, :j]
+    mu = max(sigma[j - 1] * sigma[j - 1] * HookParams.start_rel, HookParams.start_abs)
+    while True:
+        mu = mu * HookParams.growth
+        c = g * sigma / (mu + sigma * sigma)
+        if math.sqrt(float(np.dot(c, c))) <= delta:
+            break
+    y = c @ vt
+    resid = -(hb @ y)
+    resid[0] = resid[0] + beta
+    return y, math.sqrt(float(np.dot(resid, resid)))
 
 
 class Gmres:
-    """gmresm (GMRESm.f90:27-115): GMRES(m) with modified Gram-Schmidt and
-    least squares by LAPACK dgelsy on the Hessenberg matrix; keeps the Krylov
-    basis, H, j and beta between calls so that info = 2 recomputes the
-    hookstep for a new trust region without new products."""
+    """Restarted GMRES(m) (Saad & Schultz 1986, SIAM J. Sci. Stat. Comput.
+    7:856) with modified Gram-Schmidt and right preconditioning, solving
+    A x = b for the Newton step.  After each Arnoldi column the small least-
+    squares problem min |beta e1 - Hbar y| is solved by a rank-revealing QR
+    (LAPACK dgelsy, as gmresm does) to monitor the residual.  At the end of a
+    cycle the step is either that solution or, with a trust region
+    (radius > 0), the hookstep.  The Arnoldi basis of the last cycle is kept
+    so that `rescale` returns the hookstep for another radius without new
+    products of A (newtonhook's retries, GMRESm.f90:50-55).
+
+    Stopping (GMRESm.f90:76-104): |r| <= tol, the iteration budget spent, or
+    |r| no longer below (1 - stall) of the previous column's value."""
+
+    stall = 1e-14
 
     def __init__(self, m, n):
         self.m, self.n = m, n
-        self.h = np.zeros((m + 1, m))
-        self.v = np.zeros((m + 1, n))     # row i = Krylov vector i+1
-        self.j = 0
+        self.hess = np.zeros((m + 1, m))
+        self.basis = np.zeros((m + 1, n))     # row i: Krylov vector i
+        self.cols = 0
         self.beta = 0.0
 
-    def _combine(self, y, j):
+    def _expand(self, y, cols, precond):
         z = np.zeros(self.n)
-        for i in range(j):
-            z = z + y[i] * self.v[i]
-        return z
+        for i in range(cols):
+            z = z + y[i] * self.basis[i]
+        return precond(z)
 
-    def solve(self, x, b, matvec, psolve, dotprd, res, delta, its, info):
-        """Returns (x, res, delta, its, info) as gmresm leaves its inout
-        arguments: info 0 converged, 1 breakdown (residual grew), 2 max its."""
-        m = self.m
-        if info == 2:
-            y, delta = hookstep(self.h, self.j, self.beta, delta)
-            z = psolve(self._combine(y, self.j))
-            return z, res, delta, its, 0
-        tol = res
-        imx = its
+    def rescale(self, radius, precond):
+        """Hookstep of the last cycle's basis for a new radius: (step,
+        predicted residual)."""
+        y, pred = hookstep(self.hess, self.cols, self.beta, radius)
+        return self._expand(y, self.cols, precond), pred
+
+    def _residual_of_lsq(self, cols, beta):
+        hb = self.hess[:cols + 1, :cols]
+        e1 = np.zeros(cols + 1)
+        e1[0] = beta
+        y = scipy.linalg.lstsq(hb, e1, lapack_driver="gelsy")[0]
+        r = -(hb @ y)
+        r[0] = r[0] + beta
+        return y, math.sqrt(float(np.dot(r, r)))
+
+    def solve(self, b, apply, precond, dot, tol, budget, radius, x=None):
+        """Returns (x, |r|, predicted residual, iterations, status).  The
+        predicted residual is the hookstep's when radius > 0, else 0."""
+        x = np.zeros(self.n) if x is None else x
         its = 0
-        self.v[:] = 0.0
-        while True:   # restart
-            res_ = 1e99
-            stgn = 1.0 - 1e-14
-            beta = math.sqrt(dotprd(x, x))
-            w = np.zeros(self.n) if beta == 0.0 else matvec(x)
-            w = b - w
-            beta = math.sqrt(dotprd(w, w))
+        pred = 0.0
+        self.basis[:] = 0.0
+        while True:                                      # one cycle of m columns
+            r = b if math.sqrt(dot(x, x)) == 0.0 else b - apply(x)
+            beta = math.sqrt(dot(r, r))
             self.beta = beta
-            self.v[0] = w / beta
-            self.h[:] = 0.0
-            for j in range(1, m + 1):
-                self.j = j
+            self.basis[0] = r / beta
+            self.hess[:] = 0.0
+            last = math.inf
+            for cols in range(1, self.m + 1):
+                self.cols = cols
                 its += 1
-                z = psolve(self.v[j - 1].copy())
-                w = matvec(z)
-                for i in range(j):
-                    self.h[i, j - 1] = dotprd(w, self.v[i])
-                    w = w - self.h[i, j - 1] * self.v[i]
-                self.h[j, j - 1] = math.sqrt(dotprd(w, w))
-                self.v[j] = w / self.h[j, j - 1]
-                p = np.zeros(j + 1)
-                p[0] = beta
-                y = scipy.linalg.lstsq(self.h[:j + 1, :j], p, lapack_driver="gelsy")[0]
-                p = -(self.h[:j + 1, :j] @ y)
-                p[0] = p[0] + beta
-                res = math.sqrt(float(np.dot(p, p)))
-                done = res <= tol or its == imx or res > res_
-                if done or j == m:
-                    if delta > 0.0:
-                        y, delta = hookstep(self.h, j, beta, delta)
-                    z = psolve(self._combine(y, j))
-                    x = x + z
-                    if its == imx:
-                        info = 2
-                    if res > res_:
-                        info = 1
-                    if res <= tol:
-                        info = 0
-                    if done:
-                        return x, res, delta, its, info
-                    break   # j == m: restart from the new x
-                res_ = res * stgn
+                w = apply(precond(self.basis[cols - 1].copy()))
+                for i in range(cols):                    # modified Gram-Schmidt
+                    self.hess[i, cols - 1] = dot(w, self.basis[i])
+                    w = w - self.hess[i, cols - 1] * self.basis[i]
+                self.hess[cols, cols - 1] = math.sqrt(dot(w, w))
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    self.basis[cols] = w / self.hess[cols, cols - 1]
+                y, res = self._residual_of_lsq(cols, beta)
+                converged, spent, stalled = res <= tol, its == budget, res > last
+                if not (converged or spent or stalled or cols == self.m):
+                    last = res * (1.0 - self.stall)
+                    continue
+                if radius > 0.0:
+                    y, pred = hookstep(self.hess, cols, beta, radius)
+                x = x + self._expand(y, cols, precond)
+                if converged:
+                    return x, res, pred, its, Status.CONVERGED
+                if stalled:
+                    return x, res, pred, its, Status.KRYLOV_STALLED
+                if spent:
+                    return x, res, pred, its, Status.MAX_ITERATIONS
+                break                                    # full cycle: restart from x
 
 
-class NewtonHook:
-    """newtonhook (NewtonHook.f90:48-179) with the state of its module
-    (new_x, new_fx, new_tol, new_del, new_nits, new_gits)."""
+class _Trial:
+    """A point of the Newton iteration: x, f(x), |f(x)| and, for a kept
+    fallback, the trust radius to resume with."""
+    __slots__ = ("x", "fx", "res", "radius")
+
+    def __init__(self, x, fx, res, radius=0.0):
+        self.x, self.fx, self.res, self.radius = x, fx, res, radius
+
+
+class HookstepNewton:
+    """Newton's method with a GMRES-hookstep trust region (Viswanath 2007;
+    the region management of Dennis & Schnabel 1996, ch. 6) for f(x) = 0,
+    the solver newton.f90 calls (newtonhook, NewtonHook.f90:48-179).
+
+    Each iteration solves J s = f(x) for the step by GMRES(m) to a residual
+    krylov_rel * |f(x)|, constrained to |s| <= radius by the hookstep, and
+    tries x - s.  The region then shrinks or grows by TrustRegionParams;
+    shrinking re-solves the hookstep on the same Krylov basis.  Rejected
+    trials that still reduced |f| are kept as a fallback, taken when a later
+    trial from the same point is worse.
+
+    Public state after `solve` (and inside the callbacks): x, fx, res = |f(x)|,
+    radius, newton_its, krylov_its (GMRES iterations of the last solve), and
+    `log`, one line per trial and per decision."""
 
     def __init__(self, x0):
-        self.new_x = np.array(x0, dtype=np.float64)
-        self.new_fx = np.zeros_like(self.new_x)
-        self.new_tol = 0.0
-        self.new_del = 0.0
-        self.new_nits = 0
-        self.new_gits = 0
+        self.x = np.array(x0, dtype=np.float64)
+        self.fx = np.zeros_like(self.x)
+        self.res = 0.0
+        self.radius = 0.0
+        self.newton_its = 0
+        self.krylov_its = 0
         self.log = []
 
-    def _say(self, msg):
+    def _note(self, msg):
         self.log.append(msg)
 
-    def solve(self, f, df, dfp, sub, dp, m, gtol, tol, delta, mndl, mxdl, nits, info=0):
-        """f(x) -> f; df(x) -> Jacobian action about new_x; dfp(x) -> x
-        (preconditioner); sub() after every accepted iteration; dp(a, b).
-        Returns info: 0 converged, 2 max iterations, 3 trust region too small."""
-        n = self.new_x.size
-        gm = Gmres(m, n)
-        self.new_nits = 0
-        self.new_gits = 0
-        self.new_del = delta
-        mxdl_ = mxdl
-        ginfo = info
-        self.new_fx = f(self.new_x)
-        self.new_tol = math.sqrt(dp(self.new_fx, self.new_fx))
-        if delta < 0.0:
-            self.new_del = self.new_tol / 10.0
-            mxdl_ = 1e99
-        self._say(f"newton: nits={self.new_nits} res={self.new_tol:.6e}")
-        sub()
-        x_, fx_, tol_ = self.new_x.copy(), self.new_fx.copy(), self.new_tol
-        tol__ = 1e99
-        x__ = fx__ = None
-        del__ = 0.0
-        if self.new_tol < tol:
-            self._say("newton: input already converged")
-            return 0
+    def _take(self, p):
+        self.x, self.fx, self.res = p.x, p.fx, p.res
+
+    def solve(self, func, jac, precond, on_accept, dot, m, krylov_rel, tol, radius, radius_min, radius_max,
+              max_its):
+        """func(x) -> f(x); jac(v) -> J v about self.x; precond(v) -> v;
+        on_accept() after the start point and every accepted iterate;
+        dot(a, b).  radius = 0 disables the trust region; radius < 0 starts
+        it at |f(x0)|/10 with no cap.  Returns a Status code."""
+        tr = TrustRegionParams
+        norm = lambda v: math.sqrt(dot(v, v))  # noqa: E731
+        krylov = Gmres(m, self.x.size)
+        self.newton_its = 0
+        self.krylov_its = 0
+        self.radius = radius
+        cap = radius_max
+        self.fx = func(self.x)
+        self.res = norm(self.fx)
+        if radius < 0.0:
+            self.radius = self.res / 10.0
+            cap = 1e99
+        self._note(f"start: |f| = {self.res:.6e}")
+        on_accept()
+        base = _Trial(self.x.copy(), self.fx.copy(), self.res)
+        fallback = None
+        if self.res < tol:
+            self._note("start point within tolerance")
+            return Status.CONVERGED
+        reuse_basis = False
         while True:
-            if self.new_del < mndl:
-                self._say("newton: trust region too small")
-                return 3
-            s = np.zeros(n)
-            gres = gtol * self.new_tol
-            gdel = self.new_del
-            if ginfo != 2:
-                self.new_gits = m
-            s, gres, gdel, self.new_gits, ginfo = gm.solve(s, fx_, df, dfp, dp, gres, gdel, self.new_gits,
-                                                           ginfo)
-            ginfo = info
-            self.new_x = x_ - s
-            self.new_fx = f(self.new_x)
-            self.new_tol = math.sqrt(dp(self.new_fx, self.new_fx))
-            snrm = math.sqrt(dp(s, s))
-            ared = tol_ - self.new_tol
-            pred = tol_ - gdel
+            if self.radius < radius_min:
+                self._note(f"stop: trust radius {self.radius:.3e} below {radius_min:.3e}")
+                return Status.REGION_COLLAPSED
+            if reuse_basis:
+                step, pred_res = krylov.rescale(self.radius, precond)
+            else:
+                step, _, pred_res, self.krylov_its, _ = krylov.solve(
+                    base.fx, jac, precond, dot, krylov_rel * base.res, m, self.radius)
+            trial = _Trial(base.x - step, None, 0.0)
+            trial.fx = func(trial.x)
+            trial.res = norm(trial.fx)
+            self._take(trial)
+            snorm = norm(step)
+            actual = base.res - trial.res
+            predicted = base.res - pred_res
             with np.errstate(divide="ignore", invalid="ignore"):
-                ratio = float(np.float64(ared) / np.float64(pred))   # IEEE, as the Fortran division
-            self._say(f"newton: nits={self.new_nits} res={self.new_tol:.6e} gits={self.new_gits} "
-                      f"del={self.new_del:.6e} |s|={snrm:.6e} ared/pred={ratio:.4f}")
-            if self.new_tol > tol__:
-                self._say("newton: accepting previous step")
-                self.new_x, self.new_fx, self.new_tol, self.new_del = x__, fx__, tol__, del__
-            elif ared < 0.0:
-                self._say("newton: norm increased, try smaller step")
-                self.new_del = snrm * 0.5
-                ginfo = 2
-            elif ratio < 0.75:
-                self._say("newton: step ok, trying smaller step")
-                x__, fx__, tol__ = self.new_x, self.new_fx, self.new_tol
-                del__ = snrm if ratio > 0.1 else snrm * 0.5
-                self.new_del = snrm * 0.7
-                ginfo = 2
-            elif snrm < self.new_del * 0.9:
-                self._say("newton: step good, took full newton step")
-                self.new_del = min(mxdl_, snrm * 2.0)
-            elif self.new_del < mxdl_ * 0.9:
-                self._say("newton: step good, trying larger step")
-                x__, fx__, tol__, del__ = self.new_x, self.new_fx, self.new_tol, self.new_del
-                self.new_del = min(mxdl_, snrm * 2.0)
-                ginfo = 2
-            if ginfo == 2:
+                rho = float(np.float64(actual) / np.float64(predicted))
+            self._note(f"trial {self.newton_its}: |f| = {trial.res:.6e}, gmres its {self.krylov_its}, "
+                       f"radius {self.radius:.6e}, |s| = {snorm:.6e}, rho = {rho:.4f}")
+            reuse_basis = True
+            if fallback is not None and trial.res > fallback.res:
+                self._note("  worse than the kept fallback: take the fallback")
+                self._take(fallback)
+                self.radius = fallback.radius
+                reuse_basis = False
+            elif actual < 0.0:
+                self._note("  |f| grew: halve the region")
+                self.radius = snorm * tr.halve
+            elif rho < tr.accept_ratio:
+                self._note("  weak reduction: keep as fallback, shrink the region")
+                fallback = _Trial(trial.x, trial.fx, trial.res,
+                                  snorm if rho > tr.poor_ratio else snorm * tr.halve)
+                self.radius = snorm * tr.shrink_ok
+            elif snorm < self.radius * tr.full_fraction:
+                self._note("  good step inside the region: take it, expand the region")
+                self.radius = min(cap, snorm * tr.expand)
+                reuse_basis = False
+            elif self.radius < cap * tr.full_fraction:
+                self._note("  good step at the region's edge: keep as fallback, try a larger region")
+                fallback = _Trial(trial.x, trial.fx, trial.res, self.radius)
+                self.radius = min(cap, snorm * tr.expand)
+            else:
+                reuse_basis = False
+            if reuse_basis:
                 continue
-            self.new_nits += 1
-            sub()
-            x_, fx_, tol_ = self.new_x.copy(), self.new_fx.copy(), self.new_tol
-            tol__ = 1e99
-            if self.new_tol < tol:
-                self._say("newton: converged")
-                return 0
-            if self.new_nits == nits:
-                self._say("newton: reached max its")
-                return 2
+            self.newton_its += 1
+            on_accept()
+            base = _Trial(self.x.copy(), self.fx.copy(), self.res)
+            fallback = None
+            if self.res < tol:
+                self._note(f"converged after {self.newton_its} iterations")
+                return Status.CONVERGED
+            if self.newton_its == max_its:
+                self._note(f"stop: {max_its} iterations without convergence")
+                return Status.MAX_ITERATIONS
 
 
 # ---------------------------------------------------------------------------
@@ -454,7 +535,7 @@ class BatchedStepper:
 def run_searches(batch, searches, **solve_args):
     """Run OrbitNewton searches (each built on ``batch.stepper(i)``) in
     threads with their device integrations batched; returns each search's
-    (info, NewtonHook)."""
+    (status, HookstepNewton)."""
     import threading
     res = [None] * len(searches)
 
@@ -603,78 +684,86 @@ class OrbitNewton:
         f = [(-a.imag * ds[None, :]) + 1j * (a.real * ds[None, :]) for a in self.vec2rtz(x)]
         return self.rtz2vec(*f)
 
-    def getrhs(self, x):
-        """getrhs (newton.f90:374-393): F(x) - x over the segments, constraint rows 0."""
-        n, ms = self.n, self.ms
-        ys = self.steporbit_many([x[p * n:(p + 1) * n] for p in range(ms)], self.ndts)
-        y_ = np.zeros(n * ms)
-        for p in range(ms):
-            p1 = (p + 1) % ms
-            y_[p1 * n:(p1 + 1) * n] = ys[p]
-        y = y_ - x
-        for p in range(ms):
-            y[p * n:p * n + 8] = 0.0
-        return y
+    def _segments(self, x):
+        return [x[p * self.n:(p + 1) * self.n] for p in range(self.ms)]
 
-    def multJ(self, nh, dx):
-        """multJ (newton.f90:399-456): the Jacobian action by finite
-        differences and the constraint rows."""
-        n, ms = self.n, self.ms
-        eps = math.sqrt(self.dotprod_ms(1, dx, dx))
-        if eps == 0.0:
-            raise RuntimeError("multJ: eps=0 (1)")
-        eps = self.epsJ * math.sqrt(self.dotprod_ms(1, nh.new_x, nh.new_x)) / eps
-        if eps == 0.0:
-            raise RuntimeError("multJ: eps=0 (2)")
-        y = nh.new_x + eps * dx
-        s = self.getrhs(y)
-        y = (s - nh.new_fx) / eps
-        xs = [nh.new_x[p * n:(p + 1) * n] for p in range(ms)]
-        traj = self.steporbit_many(xs, [1] * ms)
-        for p in range(ms):
-            xp, dxp = xs[p], dx[p * n:(p + 1) * n]
-            y[p * n + 0] = self.dotprod(-1, self.getshiftdir(1, xp), dxp)
-            y[p * n + 1] = self.dotprod(-1, self.getshiftdir(2, xp), dxp)
-            st = (traj[p] - xp) / self.tim_dt
-            y[p * n + 2] = self.dotprod(-1, st, dxp)
-            if self.freeze_theta:
-                y[p * n + 0] = dxp[0]
-            if self.T_fixed:
-                y[p * n + 2] = dxp[2]
-        return y
+    def residual(self, x):
+        """f(x) = F(x) - x of multiple shooting (getrhs, newton.f90:374-393):
+        segment p is integrated over its period and compared with the start
+        of segment p + 1 (cyclically); the 8 leading entries of every segment
+        (shifts, period and spares) carry the constraints, zero here."""
+        n = self.n
+        images = self.steporbit_many(self._segments(x), self.ndts)
+        f = np.empty(n * self.ms)
+        for p, img in enumerate(images):
+            q = (p + 1) % self.ms
+            f[q * n:(q + 1) * n] = img - x[q * n:(q + 1) * n]
+        f.reshape(self.ms, n)[:, :8] = 0.0
+        return f
+
+    def _fd_increment(self, x, dx):
+        """The finite-difference increment h of the Jacobian action:
+        h |dx| = epsJ |x| (newton.f90:411-418)."""
+        ndx = math.sqrt(self.dotprod_ms(1, dx, dx))
+        if ndx == 0.0:
+            raise ValueError("Jacobian action requested for a zero vector")
+        h = self.epsJ * math.sqrt(self.dotprod_ms(1, x, x)) / ndx
+        if h == 0.0:
+            raise ValueError("Jacobian action about x = 0 has no finite-difference increment")
+        return h
+
+    def jacobian_action(self, solver, dx):
+        """J dx about the solver's current point (multJ, newton.f90:399-456):
+        the forward difference (f(x + h dx) - f(x)) / h in the velocity
+        entries; per segment, the constraint rows are the components of dx
+        along the theta-shift, z-shift and time directions of x (so that the
+        update does not move along the orbit's symmetry group), or dx's own
+        shift / period entry where that one is frozen."""
+        x, fx = solver.x, solver.fx
+        h = self._fd_increment(x, dx)
+        jv = (self.residual(x + h * dx) - fx) / h
+        xs = self._segments(x)
+        # the time direction: one time step of each segment
+        forward = self.steporbit_many(xs, [1] * self.ms)
+        rows = jv.reshape(self.ms, self.n)
+        for p, (xp, ahead) in enumerate(zip(xs, forward)):
+            dxp = dx[p * self.n:(p + 1) * self.n]
+            along_t = (ahead - xp) / self.tim_dt
+            rows[p, 0] = dxp[0] if self.freeze_theta else self.dotprod(-1, self.getshiftdir(1, xp), dxp)
+            rows[p, 1] = self.dotprod(-1, self.getshiftdir(2, xp), dxp)
+            rows[p, 2] = dxp[2] if self.T_fixed else self.dotprod(-1, along_t, dxp)
+        return jv
 
     # -- the search (newton.f90:46-121) -----------------------------------------
     def solve(self, m=100, nits=20, rel_err=1e-8, delta=-1.0, mndl=1e-12, mxdl=1e3, gtol=1e-3, epsJ=1e-7,
               on_iteration=None):
-        """newtonhook on this problem with the tolerances of params.in scaled
-        by |x0| as newton.f90:91-95 does.  Returns (info, NewtonHook)."""
+        """The hookstep Newton search on this problem, with the tolerances of
+        newton_params.in relative to |x0| as newton.f90:91-95 takes them.
+        Returns (status, HookstepNewton)."""
         self.epsJ = float(epsJ)
-        d = self.dotprod_ms(-1, self.x0, self.x0)
-        tol = rel_err * math.sqrt(d)
-        scale = math.sqrt(d)
-        nh = NewtonHook(self.x0)
+        scale = math.sqrt(self.dotprod_ms(-1, self.x0, self.x0))
+        solver = HookstepNewton(self.x0)
 
-        def sub():   # saveorbit (newton.f90:462-529): the newton.dat quantities
-            norm_x = math.sqrt(self.dotprod_ms(-1, nh.new_x, nh.new_x))
-            rec = dict(nits=nh.new_nits, gits=nh.new_gits, tol=nh.new_tol, delta=nh.new_del,
-                       rel=nh.new_tol / norm_x, norm_x=norm_x,
-                       normE_ratio=self.normE(nh.new_fx) / self.normE(nh.new_x),
-                       shifts=[tuple(nh.new_x[p * self.n:p * self.n + 3] * self.scaleT)
-                               for p in range(self.ms)])
-            self.history.append(rec)
+        def record():   # the quantities saveorbit writes to newton.dat (newton.f90:462-529)
+            norm_x = math.sqrt(self.dotprod_ms(-1, solver.x, solver.x))
+            self.history.append(dict(
+                nits=solver.newton_its, gits=solver.krylov_its, tol=solver.res, delta=solver.radius,
+                rel=solver.res / norm_x, norm_x=norm_x,
+                normE_ratio=self.normE(solver.fx) / self.normE(solver.x),
+                shifts=[self.shifts(solver.x, p) for p in range(self.ms)]))
             if on_iteration is not None:
-                on_iteration(self, nh)
+                on_iteration(self, solver)
 
-        info = nh.solve(self.getrhs, lambda v: self.multJ(nh, v), lambda v: v, sub,
-                        lambda a, b: self.dotprod_ms(1, a, b), m, gtol, tol,
-                        delta * scale if delta > 0 else delta, mndl * scale, mxdl * scale, nits)
-        self.result = nh
-        return info, nh
+        status = solver.solve(self.residual, lambda v: self.jacobian_action(solver, v), lambda v: v, record,
+                              lambda a, b: self.dotprod_ms(1, a, b), m, gtol, rel_err * scale,
+                              delta * scale if delta > 0 else delta, mndl * scale, mxdl * scale, nits)
+        self.result = solver
+        return status, solver
 
     # -- Floquet analysis (getEigen, newton.f90:850-968) -------------------------
     def multA(self, x, y0, epsA):
         """multA (newton.f90:814-842): the linearised map over all segments
-        about new_x by finite differences."""
+        about the orbit self.xn by finite differences."""
         n = self.n
         y = x.copy()
         for p in range(self.ms):
@@ -698,7 +787,7 @@ class OrbitNewton:
         vectors: (ur, ut, uz) per eigenvector column scaled to the orbit's
         energy as the saved states are)."""
         n = self.n
-        self.xn = np.array(self.result.new_x if x is None else x, dtype=np.float64)
+        self.xn = np.array(self.result.x if x is None else x, dtype=np.float64)
         rerr = min(rel_err, gtol)
         t = float(np.sum(self.xn[2::n][:self.ms])) * self.scaleT
         y0 = np.concatenate(self.steporbit_many([self.xn[p * n:(p + 1) * n] for p in range(self.ms)],

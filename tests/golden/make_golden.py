@@ -47,6 +47,7 @@ WORKLOADS = {
 # symmetry flag: oracle/build_ref.sh mini_sr 16 6 4 1 shiftrefl)
 EXE = {"adapt_mini": "mini", "adapt_c0": "c0", "sym_mini": "mini", "sr_mini": "mini_sr"}
 SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
+SAMPLE_COUNT = 400      # at most this many harmonics, evenly spaced, from the largest states
 
 # steporbit (newton.f90:721-775) runs of the reference's procedures from the
 # seeded state of a workload: name: (workload, executable, [(ndts, T, sth, sz)]);
@@ -228,7 +229,8 @@ def make(name):
         out["series"] = ser
         if dt < 0:   # step tim_dt tim_cfl_dt tim_cfl_dir tim_corr_dt tim_new_dt
             out["dtser"] = np.loadtxt(pre + "_dt.txt", ndmin=2)
-        sel = np.arange(0, H, SAMPLE_STRIDE) if not full else np.arange(H)
+        stride = max(SAMPLE_STRIDE, -(-H // SAMPLE_COUNT))
+        sel = np.arange(0, H, stride) if not full else np.arange(H)
         out["sel"] = sel
         if full:
             out["ic"] = np.stack([ur, ut, uz])

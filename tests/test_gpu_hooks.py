@@ -40,6 +40,36 @@ def test_norm_and_friction_match_reference(name):
     assert abs(Ufr - ref[7]) <= 1e-12 * abs(ref[7])
 
 
+@pytest.mark.parametrize("name", ["mini", "c0"])
+def test_coll_norm_of_any_field(name):
+    """var_coll_norm on host fields (opf_var_coll_norm): the three
+    components of the reference's initial state give the reference's step-0
+    E, Ek(0), Em(0) (1e-12) and equal the norms of the slot's own state; a
+    field that is no slot's state (u_r + 2 u_z) equals the oracle's
+    var_coll_norm to 1e-14 (the same operations in the same order)."""
+    import opf_oracle as O
+    g = load_golden(name)
+    ctx = _ctx(g)
+    E, Ek0, Em0 = 0.0, 0.0, 0.0
+    for comp in range(3):
+        e, ek, em = ctx.coll_norm(g["ic"][comp], 0)
+        e2, ek2, em2 = ctx.var_norm(0, comp)
+        assert abs(e - e2) <= 1e-14 * abs(e2)
+        assert np.allclose(ek, ek2, rtol=1e-14, atol=0) and np.allclose(em, em2, rtol=1e-14, atol=0)
+        E, Ek0, Em0 = E + e, Ek0 + ek[0], Em0 + em[0]
+    ref = g["series"][0]
+    assert abs(E - ref[2]) <= 1e-12 * ref[2]
+    assert abs(Ek0 - ref[3]) <= 1e-12 * abs(ref[3])
+    assert abs(Em0 - ref[4]) <= 1e-12 * abs(ref[4])
+    f = g["ic"][0] + 2.0 * g["ic"][2]
+    oc = O.Ctx(int(g["N"]), int(g["K"]), int(g["M"]), Re=float(g["Re"]), alpha=float(g["alpha"]),
+               dt=float(g["dt"]))
+    eo, eko, emo = oc.norm(f)
+    e, ek, em = ctx.coll_norm(f, 0)
+    assert abs(e - eo) <= 1e-14 * eo
+    assert np.abs(ek - eko).max() <= 1e-14 * eo and np.abs(em - emo).max() <= 1e-14 * eo
+
+
 def test_friction_without_constant_flux_and_diag_record():
     g = load_golden("mini")
     ctx = _ctx(g, const_flux=False)

@@ -120,7 +120,7 @@ def test_orbit_search_device_matches_oracle_stepper():
         on = OrbitNewton(stepper, N, K, M, 1, alpha, intrdr, g["ic"], 0.0, 0.0, 0.1, [10])
         print("orbit search with", type(stepper).__name__, flush=True)
         info, nh = on.solve(m=8, nits=2, rel_err=1e-10, delta=-1.0, epsJ=1e-7)
-        runs.append((info, on.history, nh.new_x))
+        runs.append((info, on.history, nh.x))
     (i0, h0, x0), (i1, h1, x1) = runs
     assert i0 == i1 == 2
     assert [r["gits"] for r in h0] == [r["gits"] for r in h1]
@@ -132,7 +132,7 @@ def test_orbit_search_device_matches_oracle_stepper():
 
 
 def test_multiple_shooting_segments_share_one_launch():
-    """ms = 2: getrhs integrates both segments in one batched call; the
+    """ms = 2: the residual integrates both segments in one batched call; the
     result equals the segments stepped one at a time."""
     import opf
     from opf.newton import DeviceStepper, OrbitNewton
@@ -142,7 +142,7 @@ def test_multiple_shooting_segments_share_one_launch():
     intrdr = O.build_mesh(N).intrdr
     both = _ctx(opf, g, capacity=2)
     on2 = OrbitNewton(DeviceStepper(both, [0, 1]), N, K, M, 1, alpha, intrdr, g["ic"], 0.1, 0.2, 0.1, [10, 10])
-    f2 = on2.getrhs(on2.x0)
+    f2 = on2.residual(on2.x0)
     single = _ctx(opf, g)
     n = on2.n
     for p in range(2):
@@ -260,7 +260,7 @@ def test_independent_searches_share_launches():
         alone = OrbitNewton(counter, N, K, M, 1, alpha, me.intrdr, st, 0.0, 0.0, 0.1, [10], r=me.r)
         info, nh = alone.solve(**args)
         assert info == res[i][0]
-        assert np.array_equal(nh.new_x, res[i][1].new_x), i
+        assert np.array_equal(nh.x, res[i][1].x), i
         assert [r["tol"] for r in alone.history] == [r["tol"] for r in searches[i].history]
         calls.append(n_calls[0])
     # lock step: one launch sequence per round of requests, as many rounds as

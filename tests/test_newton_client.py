@@ -31,21 +31,21 @@ def _system(n, amp):
 
 @pytest.mark.parametrize("case", range(4))
 def test_newtonhook_matches_reference(case):
-    from opf.newton import NewtonHook
+    from opf.newton import HookstepNewton
     g = load_golden("newton_alg")
     n, m, nits, gtol, tol, dl, mndl, mxdl, epsJ, amp = g["cases"][case]
     n, m, nits = int(n), int(m), int(nits)
     f, x0 = _system(n, amp)
-    nh = NewtonHook(x0)
+    nh = HookstepNewton(x0)
     rows = []
 
     def df(v):   # multJ's finite difference (newton.f90:411-418)
         eps = math.sqrt(float(v @ v))
-        eps = epsJ * math.sqrt(float(nh.new_x @ nh.new_x)) / eps
-        return (f(nh.new_x + eps * v) - nh.new_fx) / eps
+        eps = epsJ * math.sqrt(float(nh.x @ nh.x)) / eps
+        return (f(nh.x + eps * v) - nh.fx) / eps
 
     def sub():
-        rows.append([nh.new_nits, nh.new_gits, nh.new_tol, nh.new_del, *nh.new_x])
+        rows.append([nh.newton_its, nh.krylov_its, nh.res, nh.radius, *nh.x])
 
     info = nh.solve(f, df, lambda v: v, sub, lambda a, b: float(a @ b), m, gtol, tol, dl, mndl, mxdl, nits)
     ref = g[f"iters_{case}"]
@@ -83,7 +83,7 @@ def test_hookstep_respects_trust_region():
 
 
 def test_gmres_solves_linear_system():
-    """gmresm without a trust region on a well-conditioned system: the
+    """GMRES without a trust region on a well-conditioned system: the
     residual reached is the one reported and below the tolerance."""
     from opf.newton import Gmres
     rng = np.random.default_rng(4)
@@ -91,9 +91,9 @@ def test_gmres_solves_linear_system():
     A = np.eye(n) * 3 + rng.standard_normal((n, n)) * 0.3
     b = rng.standard_normal(n)
     gm = Gmres(40, n)
-    x, res, _, its, info = gm.solve(np.zeros(n), b, lambda v: A @ v, lambda v: v, lambda a, c: float(a @ c),
-                                    1e-10 * np.linalg.norm(b), 0.0, 40, 0)
-    assert info == 0 and its <= 40
+    x, res, _, its, status = gm.solve(b, lambda v: A @ v, lambda v: v, lambda a, c: float(a @ c),
+                                      1e-10 * np.linalg.norm(b), 40, 0.0)
+    assert status == 0 and its <= 40
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
 
 
@@ -191,31 +191,31 @@ def test_logeig_quadrants():
     assert np.allclose(sr, np.log(np.hypot(wr, wi)) / 2.0, atol=1e-15)
 
 
-def test_getrhs_wires_shooting_segments():
-    """getrhs (newton.f90:374-393) with ms = 2 and an identity time step:
+def test_residual_wires_shooting_segments():
+    """The residual (getrhs, newton.f90:374-393) with ms = 2 and an identity time step:
     segment p's image lands in segment p+1 (cyclically), the constraint rows
     are zero, so f = (x_1 - x_0, x_0 - x_1) in the velocity entries."""
     on, u, _ = _orbit_problem(ms=2)
     n = on.n
     x = on.x0.copy()
     x[n + 8:] = 0.5 * x[n + 8:]                 # make the two segments differ
-    f = on.getrhs(x)
+    f = on.residual(x)
     assert np.all(f[:8] == 0.0) and np.all(f[n:n + 8] == 0.0)
     assert np.array_equal(f[n + 8:], x[8:n] - x[n + 8:])
     assert np.array_equal(f[8:n], x[n + 8:] - x[8:n])
 
 
 def test_multj_constraint_rows_for_fixed_period_and_frozen_shift():
-    """multJ's special cases (newton.f90:448-451): with T_fixed and a frozen
+    """The Jacobian action's special cases (multJ, newton.f90:448-451): with T_fixed and a frozen
     theta shift (mirror or shift-reflect symmetry) the rows 1 and 3 of each
     segment return the update's own entries."""
-    from opf.newton import NewtonHook
+    from opf.newton import HookstepNewton
     on, u, _ = _orbit_problem()
     on.T_fixed, on.freeze_theta, on.epsJ = True, True, 1e-7
     on.tim_dt = 0.2
-    nh = NewtonHook(on.x0)
-    nh.new_fx = on.getrhs(nh.new_x)
+    nh = HookstepNewton(on.x0)
+    nh.fx = on.residual(nh.x)
     dx = np.random.default_rng(1).standard_normal(on.n)
-    y = on.multJ(nh, dx)
+    y = on.jacobian_action(nh, dx)
     assert y[0] == dx[0] and y[2] == dx[2]
     assert y[1] == on.dotprod(-1, on.getshiftdir(2, on.x0), dx)
