@@ -75,41 +75,94 @@ def make_ic(ctx, wl):
     return ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
 
 
+def host_cores():
+    """Host cores this process may use (the GPU box gives one GPU a share of
+    16), for the CPU baselines."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _ref_env():
+    return dict(os.environ, MKL_NUM_THREADS="1", MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
+
+
+def _unlimited_stack():
+    import resource
+    # the reference keeps large automatic coll temporaries on the stack
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def _ref_start(exe, icf, prefix, Re, alpha, dt, nsteps, ranks=1):
+    """Start one reference run (oracle/_ref: ref_driver.f90, or
+    ref_mpi_driver.f90 under mpiexec with `ranks` ranks); returns the process."""
+    cmd = [exe, icf, prefix, repr(Re), repr(alpha), repr(dt), str(nsteps)]
+    if ranks > 1:
+        cmd = ["/opt/conda/bin/mpiexec", "-n", str(ranks)] + cmd
+    return subprocess.Popen(cmd, cwd=os.path.dirname(prefix), env=_ref_env(), preexec_fn=_unlimited_stack,
+                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def _ref_seconds(proc, prefix, timeout=300):
+    """Wait for a reference run; the wall seconds of its time loop."""
+    if proc.wait(timeout=timeout) != 0:
+        raise RuntimeError(f"reference run failed with status {proc.returncode}")
+    with open(prefix + "_timing.txt") as f:
+        return float(f.read().split()[0])
+
+
+def _mpi_build(name, cores):
+    """The MPI build of the reference for this workload with the most ranks
+    that fit the host's cores: (executable, ranks) or None."""
+    best = None
+    for r in (4, 8, 16):
+        exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_mpi_{name}_{r}")
+        if r <= cores and os.path.exists(exe) and os.path.exists("/opt/conda/bin/mpiexec"):
+            best = (exe, r)
+    return best
+
+
 def cpu_baseline(name, wl, ic_fields, target_s):
-    """Reference Fortran (oracle/_ref, compiled from the reference sources
-    with MKL's LAPACK/FFTW3) on one host core, on a bounded number of steps
-    of the same workload; falls back to the numpy oracle."""
+    """The reference's own CPU path on this host's cores, on a bounded
+    sample of the same workload: the reference Fortran (oracle/_ref,
+    compiled from the reference sources with MKL's LAPACK/FFTW3) as MPI ranks
+    (its radial/harmonic decomposition, parallel.h:5-6) when an MPI build and
+    mpiexec are present, else one process on one core; median of 3 runs.
+    Falls back to the numpy oracle on one core."""
     N, K, M, Re, alpha, dt, E0, seed, _ = wl
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import opf_oracle as O
-    exe = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{name}")
-    per_step_guess = {"c1": 0.7, "c0": 0.025, "c2": 10.0}.get(name, 0.5)
-    nsteps = max(3, int(target_s / per_step_guess))
-    if os.path.exists(exe):
+    cores = host_cores()
+    mpi = _mpi_build(name, cores)
+    serial = os.path.join(REPO, "oracle", "_ref", f"opf_ref_{name}")
+    runs = [mpi] if mpi else []
+    if os.path.exists(serial):
+        runs.append((serial, 1))
+    for exe, ranks in runs:
         try:
             with tempfile.TemporaryDirectory() as td:
                 icf = os.path.join(td, "ic.bin")
                 O.write_coll3(icf, *ic_fields)
-                import resource
-
-                def lim():
-                    resource.setrlimit(resource.RLIMIT_STACK,
-                                       (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
-                env = dict(os.environ, MKL_NUM_THREADS="1", MKL_THREADING_LAYER="SEQUENTIAL",
-                           OMP_NUM_THREADS="1")
-                subprocess.run([exe, icf, os.path.join(td, "b"), repr(Re), repr(alpha), repr(dt),
-                                str(nsteps)], check=True, cwd=td, env=env, preexec_fn=lim,
-                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=300)
-                secs = float(open(os.path.join(td, "b_timing.txt")).read().split()[0])
-            return dict(value=nsteps / secs, unit="state-steps/s", cores=1, kind="reference",
-                        sample=f"{nsteps} steps of the same workload, reference Fortran "
-                               f"(amdflang -O2 + MKL sequential), 1 core, {secs:.1f} s")
+                pre = os.path.join(td, "b")
+                # a 2-step calibration run sizes the sample (about target_s / 3 per run)
+                cal = _ref_seconds(_ref_start(exe, icf, pre, Re, alpha, dt, 2, ranks), pre) / 2.0
+                nsteps = max(3, int(target_s / 3.0 / max(cal, 1e-6)))
+                secs = [_ref_seconds(_ref_start(exe, icf, pre, Re, alpha, dt, nsteps, ranks), pre)
+                        for _ in range(3)]
+            med = float(np.median(secs))
+            how = f"{ranks} MPI ranks (mpiexec, _Nr = {ranks})" if ranks > 1 else "1 process"
+            return dict(value=nsteps / med, unit="state-steps/s", cores=ranks, kind="reference",
+                        sample=f"{nsteps} steps of the same workload, reference Fortran (amdflang -O2 + "
+                               f"MKL sequential), {how}, median of 3 runs "
+                               f"({', '.join(f'{x:.2f}' for x in secs)} s); host cores available: {cores}",
+                        runs_s=secs)
         except Exception as e:                       # noqa: BLE001
-            print(f"cpu_baseline: reference run failed ({e}); timing the oracle port",
-                  file=sys.stderr)
+            print(f"cpu_baseline: reference run with {ranks} rank(s) failed ({e})", file=sys.stderr)
     oc = O.Ctx(N, K, M, Re=Re, alpha=alpha, dt=dt)
     oc.set_state(*ic_fields)
-    n = max(1, nsteps // 10)
+    n = 3
     t0 = time.perf_counter()
     oc.run(n)
     secs = time.perf_counter() - t0
@@ -117,15 +170,43 @@ def cpu_baseline(name, wl, ic_fields, target_s):
                 sample=f"{n} steps of the same workload, numpy oracle, 1 process, {secs:.1f} s")
 
 
+def cpu_concurrent(jobs, target_s):
+    """One serial reference process per core, all at once: jobs = list of
+    (executable, initial fields, Re, alpha, dt).  Each runs about target_s
+    seconds of steps (sized by a 2-step calibration run per executable).
+    Returns (summed state-steps/s, per-job (steps, seconds))."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import opf_oracle as O
+    with tempfile.TemporaryDirectory() as td:
+        per_step = {}
+        for exe, f, Re, alpha, dt in jobs:
+            if exe not in per_step:
+                icf = os.path.join(td, "cal.bin")
+                O.write_coll3(icf, *f)
+                pre = os.path.join(td, "cal")
+                per_step[exe] = _ref_seconds(_ref_start(exe, icf, pre, Re, alpha, dt, 2), pre) / 2.0
+        procs = []
+        for i, (exe, f, Re, alpha, dt) in enumerate(jobs):
+            icf = os.path.join(td, f"ic{i}.bin")
+            O.write_coll3(icf, *f)
+            n = max(3, int(target_s / max(per_step[exe], 1e-6)))
+            pre = os.path.join(td, f"j{i}")
+            procs.append((n, pre, _ref_start(exe, icf, pre, Re, alpha, dt, n)))
+        res = [(n, _ref_seconds(p, pre)) for n, pre, p in procs]
+    return sum(n / s for n, s in res), res
+
+
 # batched mode: many (32,24,12) states with their own Re, alpha and initial
 # energy, advanced together (every launch covers all slots)
 BATCH = dict(N=32, K=24, M=12, dt=0.01, seed=2024)
 
 
-def batched_leg(nstates, steps):
+def batched_leg(nstates, steps, cpu_s=0.0):
     """Batched throughput, the same states advanced one at a time (plain
     per-state loop), and the parity of slot 0, which holds the reference's
-    (32,24,12) case, against the reference's own energy series."""
+    (32,24,12) case, against the reference's own energy series.  With
+    cpu_s > 0, the CPU baseline of the ensemble: the reference on one state
+    per host core, all cores at once, summed."""
     import opf
     from opf import ic
     N, K, M, dt = BATCH["N"], BATCH["K"], BATCH["M"], BATCH["dt"]
@@ -145,6 +226,7 @@ def batched_leg(nstates, steps):
             golden = dict(ic=g["ic"], series=g["series"])
     t0 = time.perf_counter()
     me = None
+    first = []                                      # the first states, for the CPU baseline
     for s, (Re, alpha, E0, seed) in enumerate(params):
         ctx.setup(s, Re, alpha, dt)
         if me is None:
@@ -154,6 +236,8 @@ def batched_leg(nstates, steps):
         else:
             f = ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
         ctx.put_state(s, f[0], f[1], f[2])
+        if s < 16:
+            first.append((tuple(f[:3]), Re, alpha))
     setup_s = time.perf_counter() - t0
     ctx.run(2, diag_every=10)                       # warm-up (untimed)
     t0 = time.perf_counter()
@@ -184,6 +268,18 @@ def batched_leg(nstates, steps):
     one.run(n1, diag_every=10)
     alone = n1 / (time.perf_counter() - t0)
     one.close()
+    cpu = None
+    exe = os.path.join(REPO, "oracle", "_ref", "opf_ref_c0")
+    if cpu_s > 0 and os.path.exists(exe):
+        cores = min(host_cores(), len(first))
+        try:
+            tot, res = cpu_concurrent([(exe, f, Re, alpha, dt) for f, Re, alpha in first[:cores]], cpu_s)
+            cpu = dict(value=tot, unit="state-steps/s", cores=cores, kind="reference",
+                       sample=f"reference Fortran (serial build, MKL sequential), {cores} processes at once, "
+                              f"one ensemble state each (states 0..{cores - 1}), "
+                              f"{res[0][0]} steps each in about {cpu_s:.0f} s; summed state-steps/s")
+        except Exception as e:                       # noqa: BLE001
+            print(f"batched cpu baseline failed ({e})", file=sys.stderr)
     return {"metric": "batched state-steps/s", "value": value, "unit": "state-steps/s",
             "config": {"workload": f"{nstates} states at (N,K,M)=({N},{K},{M}), Re~U[1800,3000], "
                                    f"alpha in {{1.0,1.25,1.7}}, E0 log-uniform [1e-5,1e-2], dt={dt}, "
@@ -191,7 +287,8 @@ def batched_leg(nstates, steps):
                        "states": nstates, "steps": steps},
             "ms_per_step": 1000.0 * secs / steps, "setup_s": setup_s,
             "plain_per_state_loop": alone, "speedup_vs_plain_loop": value / alone,
-            "mean_last_ncorr": float(np.mean(ncorr)), "parity": parity}
+            "mean_last_ncorr": float(np.mean(ncorr)), "parity": parity, "cpu_baseline": cpu,
+            "speedup_vs_cpu": value / cpu["value"] if cpu else None}
 
 
 # serving: Poisson arrivals of integration requests of three sizes (5:3:2),
@@ -199,10 +296,16 @@ def batched_leg(nstates, steps):
 SERVE = dict(classes=[(32, 24, 12), (48, 48, 24), (64, 64, 48)], ratio=(5, 3, 2), dt=0.01, seed=77)
 
 
-def serving_leg(nreq, rate, budget_gb, chunk):
+SERVE_EXES = ("opf_ref_c0", "opf_ref_s48", "opf_ref_c1")   # reference builds of the three classes
+
+
+def serving_leg(nreq, rate, budget_gb, chunk, cpu_s=0.0):
     """p50/p99 request latency and throughput of the request server; parity
     of a served request against the reference, and of a served request
-    against the same integration run alone."""
+    against the same integration run alone.  With cpu_s > 0, the CPU
+    baseline of the mix: the reference on the first requests, one per host
+    core, all at once (summed state-steps/s), and the CPU latency of every
+    request of the mix run alone on one core."""
     import opf
     from opf import ic
     classes, ratio, dt = SERVE["classes"], np.array(SERVE["ratio"], float), SERVE["dt"]
@@ -263,6 +366,29 @@ def serving_leg(nreq, rate, budget_gb, chunk):
     v = one.get_state(0)
     parity["alone_vs_in_mix_bitwise"] = bool(all(np.array_equal(u[k], v[k]) for k in range(3)))
     one.close()
+    cpu = None
+    exes = [os.path.join(REPO, "oracle", "_ref", e) for e in SERVE_EXES]
+    if cpu_s > 0 and all(os.path.exists(e) for e in exes):
+        cores = min(host_cores(), len(reqs))
+        try:
+            jobs = [(exes[rq["cls"]], rq["state"][:3], rq["Re"], rq["alpha"], dt) for rq in reqs[:cores]]
+            tot, res = cpu_concurrent(jobs, cpu_s)
+            sec_step = {}
+            for rq, (n, sec) in zip(reqs[:cores], res):
+                sec_step.setdefault(rq["cls"], []).append(sec / n)
+            per = {c: float(np.mean(v)) for c, v in sec_step.items()}
+            lat_cpu = np.array([rq["nsteps"] * per[rq["cls"]] for rq in reqs if rq["cls"] in per])
+            cpu = dict(value=tot, unit="state-steps/s", cores=cores, kind="reference",
+                       sample=f"reference Fortran (serial builds of the three sizes, MKL sequential, corrector "
+                              f"to convergence), the first {cores} requests of the mix, one per core at once, "
+                              f"about {cpu_s:.0f} s each; summed state-steps/s",
+                       s_per_step={f"({classes[c][0]},{classes[c][1]},{classes[c][2]})": v
+                                   for c, v in sorted(per.items())},
+                       p50_s=float(np.percentile(lat_cpu, 50)), p99_s=float(np.percentile(lat_cpu, 99)),
+                       latency_note="each request's horizon at its class's seconds per step on one core, "
+                                    "no queueing")
+        except Exception as e:                       # noqa: BLE001
+            print(f"serving cpu baseline failed ({e})", file=sys.stderr)
     by_cls = {}
     for c, (N, K, M) in enumerate(classes):
         lc = lat[[k for k, rq in enumerate(reqs) if rq["cls"] == c]]
@@ -276,7 +402,7 @@ def serving_leg(nreq, rate, budget_gb, chunk):
                                    f"{[list(c) for c in classes]} in ratio 5:3:2, Re~U[2000,5300], "
                                    f"alpha in {{0.5,1.25}}, T~U[5,50] at dt={dt}, one corrector per step",
                        "slots": caps, "chunk_steps": chunk, "hbm_budget_gb": budget_gb},
-            "failed": failed, "by_class": by_cls, "parity": parity}
+            "failed": failed, "by_class": by_cls, "parity": parity, "cpu_baseline": cpu}
 
 
 def load_traffic(kernel, workload):
@@ -381,11 +507,13 @@ def main():
     batched = None
     ctx.close()
     if not args.no_batched:
-        batched = batched_leg(args.batch_states, args.batch_steps)
+        batched = batched_leg(args.batch_states, args.batch_steps,
+                              0.0 if args.no_cpu_baseline else args.cpu_seconds / 4)
 
     serving = None
     if not args.no_serving:
-        serving = serving_leg(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_chunk)
+        serving = serving_leg(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_chunk,
+                              0.0 if args.no_cpu_baseline else args.cpu_seconds / 4)
 
     cpu = None
     if not args.no_cpu_baseline:

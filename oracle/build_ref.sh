@@ -13,8 +13,13 @@
 # usage: oracle/build_ref.sh NAME N K M [Mp] [SYMM]
 #   SYMM: one of mirrorsym, shiftrefl, shiftrott sets that b_* flag of
 #   parameters.f90:26-28 (vel_imposesym then projects every step)
+# With NR and/or NS in the environment (NR*NS > 1) the core is built for
+# that many MPI ranks (_Nr, _Ns of parallel.h:5-6, a configured copy) against
+# MPICH's mpif.h and libraries in /opt/conda, driven by
+# oracle/ref_mpi_driver.f90; the executable is oracle/_ref/opf_ref_mpi_NAME.
 set -euo pipefail
 NAME=$1; N=$2; K=$3; M=$4; MP=${5:-1}; SYMM=${6:-}
+NR=${NR:-1}; NS=${NS:-1}
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
@@ -25,7 +30,8 @@ TMP=$(mktemp -d /tmp/opf_ref_build.XXXXXX)
 trap 'rm -rf "$TMP"' EXIT
 mkdir -p "$TMP/program" "$TMP/pp"
 # The sources include "../parallel.h" relative to their own directory.
-sed -e '/^#define _CPUTIME/d' "$REF/parallel.h" > "$TMP/parallel.h"
+sed -e '/^#define _CPUTIME/d' -e "s/^#define _Nr .*/#define _Nr $NR/" -e "s/^#define _Ns .*/#define _Ns $NS/" \
+    "$REF/parallel.h" > "$TMP/parallel.h"
 sed -e "s/\(i_N *= *\)[0-9][0-9]*/\1$N/" \
     -e "s/\(i_K *= *\)[0-9][0-9]*/\1$K/" \
     -e "s/\(i_M *= *\)[0-9][0-9]*/\1$M/" \
@@ -45,12 +51,17 @@ cd "$TMP/pp"
 FFLAGS="-O2"
 # the module arrays of the largest sizes exceed 2 GiB of static data
 if [ $((N * K * M)) -gt 1000000 ]; then FFLAGS="$FFLAGS -mcmodel=medium"; fi
+DRIVER=ref_driver; EXE="$OUT/opf_ref_$NAME"; LIBS="-lmkl_rt"
+if [ $((NR * NS)) -gt 1 ]; then
+  FFLAGS="$FFLAGS -I/opt/conda/include"
+  DRIVER=ref_mpi_driver; EXE="$OUT/opf_ref_mpi_$NAME"; LIBS="-lmkl_rt -lmpifort -lmpi"
+fi
 $FC $FFLAGS -c "$REF/cheby.f" -o cheby.o
 for f in parameters mpi meshs variables transform.fftw3 timestep velocity; do
   $FC $FFLAGS -c $f.f90 2>/dev/null
 done
-$FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
-$FC $FFLAGS -o "$OUT/opf_ref_$NAME" ref_driver.o parameters.o mpi.o cheby.o meshs.o \
+$FC $FFLAGS -c "$HERE/$DRIVER.f90" -o driver.o
+$FC $FFLAGS -o "$EXE" driver.o parameters.o mpi.o cheby.o meshs.o \
     variables.o transform.fftw3.o timestep.o velocity.o \
-    -L/opt/conda/lib -lmkl_rt -Wl,-rpath,/opt/conda/lib
-echo "built $OUT/opf_ref_$NAME (N=$N K=$K M=$M Mp=$MP)" >&2
+    -L/opt/conda/lib $LIBS -Wl,-rpath,/opt/conda/lib
+echo "built $EXE (N=$N K=$K M=$M Mp=$MP ranks=${NR}x${NS})" >&2
