@@ -55,6 +55,18 @@ int opf_version(void);
  * mesh.in) and K axial / M azimuthal modes, with device storage for
  * `capacity` states. */
 int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r);
+/* opf_ctx_create with fixed launch shapes (no reference counterpart; the
+ * autotuner's candidates): `pairs` (|k|, m) pairs per workgroup of the fused
+ * step kernel (1, 2 or 4), `tm` m columns per z-transform workgroup, `tz`
+ * z-lines per theta workgroup (each at most the default width; tz even,
+ * which keeps the pairing of z-lines in the theta transforms).  0 takes,
+ * in order, OPF_STEP_GP (pairs only), the row of the tuning table
+ * (tuning.tsv beside libopf.so, or the file OPF_TUNING names; OPF_TUNING=none
+ * skips it) for this (N, K, M) and capacity, then the built-in choice.
+ * opf_ctx_create is this call with 0, 0, 0.  Results do not depend on the
+ * shapes. */
+int opf_ctx_create_tiles(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r, int pairs,
+                         int tm, int tz);
 int opf_ctx_destroy(opf_ctx* ctx);
 /* H = number of harmonics; bytes = device memory held by the context now
  * (including buffers allocated on first use of opf_vel_shift, opf_steporbit
@@ -314,6 +326,23 @@ int opf_server_stats(opf_server* srv, int* queued, int* running, long long* stat
  * launch count since the last reset. */
 int opf_kernel_timing(opf_ctx* ctx, int enable);
 int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
+
+/* Launch modes of the device-resident time loop (no reference counterpart;
+ * for tests and benchmarks).  graph = n > 0: n iterations of the loop are
+ * captured as one HIP graph and replayed (not with the variable-viscosity
+ * plugins or kernel timing on, which launch eagerly; n <= 64); graph = 0:
+ * eager launches; compact = 1 (default):
+ * once at most half of a run's slots are still iterating, iterations launch
+ * over the list of those slots only.  A negative value leaves a mode as it
+ * is.  The counters (any pointer may be NULL) count graph replays, graph
+ * captures and compacted iterations since the context was created. */
+int opf_launch_mode(opf_ctx* ctx, int graph, int compact, long long* graph_launches,
+                    long long* graph_captures, long long* compact_iterations);
+
+/* Launch shapes in use (no reference counterpart): pairs per workgroup of the
+ * step kernel, m columns per z-transform workgroup, z-lines per theta
+ * workgroup. */
+int opf_tile_config(const opf_ctx* ctx, int* pairs, int* tm, int* tz);
 
 /* Launch shape of the fused step kernel chosen at opf_create: (|k|, m)
  * pairs per workgroup (1, 2 or 4; environment OPF_STEP_GP overrides the

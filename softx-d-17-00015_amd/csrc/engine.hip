@@ -9,13 +9,16 @@
 // (tim_check_cgce) runs in k_control on the device, so the host only enqueues
 // iterations and checks progress between chunks.
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -115,6 +118,22 @@ struct opf_ctx {
   double2* shiftTab = nullptr;   // var_coll_shift tables, (2K + M) per slot (first use)
   double* normBuf = nullptr;     // opf_var_coll_norm: field, partials and sums (first use)
   int device = 0;                // HIP device of the context
+  // launch modes of the time loop: one iteration captured as a HIP graph
+  // and replayed; iterations of a run's tail over the list of slots still
+  // active (compaction)
+  int use_graph = 0, use_compact = 1;   // use_graph: iterations per captured graph (0: eager)
+  int* smapDev = nullptr;        // compacted slot list on the device [cap]
+  std::vector<int> smapHost;     // its host copy (what the device holds)
+  struct GraphKey {
+    int s0 = -1, ns = 0, symm = 0, adapt = 0, iters = 0;
+    const int* smap = nullptr;
+    bool operator==(const GraphKey& o) const {
+      return s0 == o.s0 && ns == o.ns && symm == o.symm && adapt == o.adapt && iters == o.iters &&
+             smap == o.smap;
+    }
+  } gkey;
+  hipGraphExec_t gexec = nullptr;
+  long graph_launches = 0, graph_captures = 0, compact_iterations = 0;
   Timing tm;
 
   template <typename T> T* alloc(size_t n) {
@@ -177,6 +196,48 @@ FftTiles choose_tiles(int Z, int T, int M) {
   t.tm = std::max(1, std::min(M, tile_m(Z)));
   t.tz = std::max(1, std::min(Z, FFT_TILE / (3 * T)));
   if (t.tz == 1 && Z >= 2 && 2 * T <= THETA_ITEMS) t.tz = 2;   // long theta lines
+  return t;
+}
+
+// Tuned launch shapes.  The autotuner (opf/autotune.py) times the fused step
+// kernel's pairs per workgroup and the transforms' tile widths per (N, K, M,
+// capacity) on the device and writes the fastest into a table, tuning.tsv
+// beside libopf.so, or the file OPF_TUNING names (OPF_TUNING=none: no table).
+// Rows: N K M min_capacity pairs tm tz [us per step]; '#' starts a comment.
+// A context takes the row of its (N, K, M) with the largest min_capacity not
+// above its capacity.
+struct Tuned {
+  int gp = 0, tm = 0, tz = 0;
+};
+std::string tuning_path() {
+  if (const char* e = std::getenv("OPF_TUNING")) return std::string(e);
+  Dl_info info;
+  if (dladdr((void*)&tuning_path, &info) && info.dli_fname) {
+    const std::string lib(info.dli_fname);
+    const size_t k = lib.rfind('/');
+    return (k == std::string::npos ? std::string(".") : lib.substr(0, k)) + "/tuning.tsv";
+  }
+  return std::string();
+}
+Tuned lookup_tuning(int N, int K, int M, int cap) {
+  Tuned t;
+  const std::string path = tuning_path();
+  if (path.empty() || path == "none") return t;
+  std::ifstream f(path);
+  std::string line;
+  int best = -1;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream is(line);
+    int n, k, m, cmin, gp, tm, tz;
+    if (!(is >> n >> k >> m >> cmin >> gp >> tm >> tz)) continue;
+    if (n == N && k == K && m == M && cmin <= cap && cmin > best) {
+      best = cmin;
+      t.gp = gp;
+      t.tm = tm;
+      t.tz = tz;
+    }
+  }
   return t;
 }
 
@@ -300,6 +361,8 @@ void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
 void launch_nonnewtonian(opf_ctx* c, int slot);
 void launch_les(opf_ctx* c, int slot);
 
+// slots [s0, s0 + ns) of the launch, or, in a compacted launch, entries
+// s0 .. s0 + ns - 1 of the host copy of the slot list
 void launch_iteration(opf_ctx* c, int s0, int ns) {
   if (c->n_symm > 0) {   // vel_imposesym at the start of a step (main.f90:21), then the new curl
     launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, s0, 1);
@@ -307,7 +370,8 @@ void launch_iteration(opf_ctx* c, int s0, int ns) {
   }
   launch_eval(c, s0, ns);
   if (c->n_nn > 0 || c->n_les > 0)   // var_null(2) after vel_nonlinear (main.f90:23,38)
-    for (int s = s0; s < s0 + ns; ++s) {
+    for (int i = s0; i < s0 + ns; ++i) {
+      const int s = c->B.smap ? c->smapHost[i] : i;
       if (c->nn[s].on) launch_nonnewtonian(c, s);
       if (c->les[s].on) launch_les(c, s);
     }
@@ -490,7 +554,10 @@ std::vector<double2> twiddles(int L) {
   return tw;
 }
 
-void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
+// gp_req, tm_req, tz_req > 0 fix the launch shapes (the autotuner's
+// candidates); 0 takes OPF_STEP_GP, then the tuning table, then the defaults.
+void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r, int gp_req, int tm_req,
+            int tz_req) {
   if (N < 9 || N > MAXN) throw OpfError(OPF_ERR_ARG, "N must be in [9, 128]");
   if (K < 2 || M < 1 || Mp < 1 || cap < 1) throw OpfError(OPF_ERR_ARG, "bad K, M, Mp or capacity");
   if (M > 256) throw OpfError(OPF_ERR_ARG, "M must be <= 256");
@@ -502,6 +569,24 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   build_mesh(N, r, c->mesh);
   c->tiles = choose_tiles(c->Z, c->T, M);
   c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
+  const Tuned tuned = lookup_tuning(N, K, M, cap);
+  {
+    // narrower transform tiles than the defaults (the kernels hold at most
+    // the default width) from the caller or the table
+    const int tm = tm_req > 0 ? tm_req : tuned.tm;
+    const int tz = tz_req > 0 ? tz_req : tuned.tz;
+    if (tm > 0) {
+      if (tm > c->tiles.tm) throw OpfError(OPF_ERR_ARG, "z tile wider than the kernels hold");
+      c->tiles.tm = tm;
+    }
+    if (tz > 0 && tz != c->tiles.tz) {
+      // the theta kernel transforms N_z of z-lines 2q and 2q + 1 together; an
+      // even tile keeps those partners (and so the rounding) of the default
+      if (tz > c->tiles.tz) throw OpfError(OPF_ERR_ARG, "theta tile wider than the kernels hold");
+      if (tz % 2 != 0 || c->tiles.tz % 2 != 0) throw OpfError(OPF_ERR_ARG, "theta tile must be even");
+      c->tiles.tz = tz;
+    }
+  }
   {
     const char* e1 = std::getenv("OPF_STEP_GP");
     int dev = 0, ncu = 0, lds = 0;
@@ -509,7 +594,8 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
     c->device = dev;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    c->st = choose_step_tiles(N, c->P, cap, ncu, e1 ? std::atoi(e1) : 0, (size_t)lds);
+    const int gp = gp_req > 0 ? gp_req : (e1 ? std::atoi(e1) : tuned.gp);
+    c->st = choose_step_tiles(N, c->P, cap, ncu, gp, (size_t)lds);
     if (c->st.smem > 64 * 1024)   // dynamic LDS beyond 64 KiB is requested per kernel
       HIPCHK(hipFuncSetAttribute((const void*)step_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)c->st.smem));
@@ -560,6 +646,8 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r) {
   B.Yb = c->alloc<double2>((size_t)cap * N * s.Z * 3 * M);
   B.twZ = c->alloc<double2>(s.Z);
   B.twT = c->alloc<double2>(s.T);
+  B.smap = nullptr;
+  c->smapDev = c->alloc<int>(cap);
   auto tz = twiddles(s.Z), tt = twiddles(s.T);
   HIPCHK(hipMemcpy(B.twZ, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(B.twT, tt.data(), tt.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -709,16 +797,87 @@ RunPlan run_begin(opf_ctx* c, const std::vector<int>& slots, const std::vector<i
   return p;
 }
 
-void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
-  for (long i = 0; i < iterations && p.ns > 0; ++i) launch_iteration(c, p.lo, p.ns);
+// Iterations over [s0, s0 + ns) (of the slot list when c->B.smap is set):
+// replays of one captured iteration where the launch sequence allows it
+// (no per-slot plugin launches, no event timing), else eager launches.
+void enqueue_iterations(opf_ctx* c, int s0, int ns, long iterations) {
+  if (iterations <= 0 || ns <= 0) return;
+  const int gi = c->use_graph;   // iterations per captured graph (0: eager launches)
+  const bool graph = gi > 0 && iterations >= gi && !c->tm.on && c->n_nn == 0 && c->n_les == 0;
+  if (!graph) {
+    for (long i = 0; i < iterations; ++i) launch_iteration(c, s0, ns);
+    return;
+  }
+  opf_ctx::GraphKey key;
+  key.s0 = s0;
+  key.ns = ns;
+  key.symm = c->n_symm > 0;
+  key.adapt = c->n_adaptive > 0;
+  key.smap = c->B.smap;
+  key.iters = gi;
+  if (!c->gexec || !(key == c->gkey)) {
+    if (c->gexec) {
+      HIPCHK(hipGraphExecDestroy(c->gexec));
+      c->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int i = 0; i < gi; ++i) launch_iteration(c, s0, ns);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      throw OpfError(OPF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c->gkey = key;
+    ++c->graph_captures;
+  }
+  const long reps = iterations / gi;
+  for (long i = 0; i < reps; ++i) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  for (long i = reps * gi; i < iterations; ++i) launch_iteration(c, s0, ns);
+  c->graph_launches += reps;
 }
 
-int run_remaining(opf_ctx* c, const RunPlan& p) {
+void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
+  if (p.ns == 0 || iterations <= 0) return;
+  const int nl = (int)p.live.size();
+  if (c->use_compact && nl > 0 && 2 * nl <= p.ns) {
+    // the run's tail: only the listed slots are still active
+    if (c->smapHost != p.live) {
+      c->smapHost = p.live;
+      HIPCHK(hipMemcpyAsync(c->smapDev, c->smapHost.data(), sizeof(int) * nl, hipMemcpyHostToDevice, c->stream));
+    }
+    c->B.smap = c->smapDev;
+    try {
+      enqueue_iterations(c, 0, nl, iterations);
+    } catch (...) {
+      c->B.smap = nullptr;
+      throw;
+    }
+    c->B.smap = nullptr;
+    c->compact_iterations += iterations;
+    return;
+  }
+  enqueue_iterations(c, p.lo, p.ns, iterations);
+}
+
+int run_remaining(opf_ctx* c, RunPlan& p) {
+  p.live.clear();
   if (p.ns == 0) return 0;
   auto ctl = read_ctl(c);
   int remaining = 0;
   for (int s : p.slots)
-    if (ctl[s].active) remaining = std::max(remaining, std::max(1, ctl[s].target_steps - ctl[s].step));
+    if (ctl[s].active) {
+      remaining = std::max(remaining, std::max(1, ctl[s].target_steps - ctl[s].step));
+      p.live.push_back(s);
+    }
   return remaining;
 }
 
@@ -728,7 +887,7 @@ namespace {
 
 void run_plan(opf_ctx* c, const std::vector<int>& slots, const std::vector<int>& nsteps,
               const std::vector<int>& corr_fixed, int diag_every) {
-  const RunPlan p = run_begin(c, slots, nsteps, corr_fixed, diag_every, true);
+  RunPlan p = run_begin(c, slots, nsteps, corr_fixed, diag_every, true);
   if (p.ns == 0) return;
   // Most steps take one corrector; enqueue 2 iterations per step, then top
   // up until every slot has reached its target.  A step takes at most 11
@@ -847,13 +1006,15 @@ extern "C" {
 const char* opf_last_error(void) { return g_err.c_str(); }
 int opf_version(void) { return 100; }
 
-int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r) {
+int opf_ctx_create_tiles(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r, int pairs,
+                         int tm, int tz) {
   return guard([&] {
     if (!out) throw OpfError(OPF_ERR_ARG, "null out");
     *out = nullptr;
+    if (pairs < 0 || tm < 0 || tz < 0) throw OpfError(OPF_ERR_ARG, "negative tile size");
     auto* c = new opf_ctx();
     try {
-      create(c, N, K, M, Mp, capacity, r);
+      create(c, N, K, M, Mp, capacity, r, pairs, tm, tz);
     } catch (...) {
       opf_ctx_destroy(c);
       throw;
@@ -862,9 +1023,14 @@ int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, con
   });
 }
 
+int opf_ctx_create(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r) {
+  return opf_ctx_create_tiles(out, N, K, M, Mp, capacity, r, 0, 0, 0);
+}
+
 int opf_ctx_destroy(opf_ctx* c) {
   if (!c) return OPF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : c->tm.pool) (void)hipEventDestroy(e);
@@ -1470,6 +1636,28 @@ extern "C" int opf_kernel_timing(opf_ctx* c, int enable) {
     for (auto& r : c->tm.recs) { c->tm.pool.push_back(r.a); c->tm.pool.push_back(r.b); }
     c->tm.recs.clear();
     c->tm.on = enable != 0;
+  });
+}
+
+extern "C" int opf_launch_mode(opf_ctx* c, int graph, int compact, long long* graph_launches,
+                               long long* graph_captures, long long* compact_iterations) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    if (graph > 64) throw OpfError(OPF_ERR_ARG, "at most 64 iterations per captured graph");
+    if (graph >= 0) c->use_graph = graph;
+    if (compact >= 0) c->use_compact = compact != 0;
+    if (graph_launches) *graph_launches = c->graph_launches;
+    if (graph_captures) *graph_captures = c->graph_captures;
+    if (compact_iterations) *compact_iterations = c->compact_iterations;
+  });
+}
+
+extern "C" int opf_tile_config(const opf_ctx* c, int* pairs, int* tm, int* tz) {
+  return guard([&] {
+    if (!c) throw OpfError(OPF_ERR_ARG, "null context");
+    if (pairs) *pairs = c->st.gp;
+    if (tm) *tm = c->tiles.tm;
+    if (tz) *tz = c->tiles.tz;
   });
 }
 

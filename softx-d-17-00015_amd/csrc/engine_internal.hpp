@@ -12,6 +12,8 @@ struct RunPlan {
   std::vector<int> slots;   // slots advancing in this run
   int lo = 0, ns = 0;       // launch range [lo, lo+ns) (0 slots: nothing to do)
   int max_steps = 0;        // largest step count requested
+  std::vector<int> live;    // slots still active at the last run_remaining: once they are at
+                            // most half the range, iterations launch over this list only
 };
 
 // Mark `slots` active for nsteps[i] more steps with corr_fixed[i] correctors
@@ -21,7 +23,8 @@ RunPlan run_begin(opf_ctx* c, const std::vector<int>& slots, const std::vector<i
                   const std::vector<int>& corr_fixed, int diag_every, bool strict);
 // Enqueue evaluation + step iterations on the context's stream (no wait).
 void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations);
-// Wait for the stream; the most steps any slot of the plan still needs.
-int run_remaining(opf_ctx* c, const RunPlan& p);
+// Wait for the stream; the most steps any slot of the plan still needs
+// (and the plan's list of slots still active).
+int run_remaining(opf_ctx* c, RunPlan& p);
 
 }  // namespace opf

@@ -31,7 +31,15 @@ struct Bufs {
   double2* Yb;          // backward intermediates [slot][N][Z][3][M]
   double2* twZ;         // twiddles exp(2 pi i q / Z), q < Z
   double2* twT;         // twiddles exp(2 pi i q / T), q < T
+  const int* smap;      // null: grid z counts slots from slot0; else the slots of a
+                        // compacted launch, smap[slot0 + z]
 };
+
+// The slot a workgroup serves: slot0 + blockIdx.z, or its entry of the
+// compacted slot list.
+__device__ __forceinline__ int slot_of(const Bufs& B, int slot0) {
+  return B.smap ? B.smap[slot0 + (int)blockIdx.z] : slot0 + (int)blockIdx.z;
+}
 
 // Tile shapes of the transform kernels (chosen by the host / autotuner).
 struct FftTiles {

@@ -330,7 +330,7 @@ extern __shared__ double2 smem[];
 // ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
 __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles T, int slot0) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
   constexpr int LZ = plan_len<R0, R1, R2>();
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
 constexpr int TLD = THETA_ITEMS / FFT_THREADS;
 template <int R0, int R1, int R2>
 __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles T, int slot0) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
   constexpr int LT = plan_len<R0, R1, R2>();
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
 // ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
 __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles T, int slot0) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active) return;
   constexpr int LZ = plan_len<R0, R1, R2>();

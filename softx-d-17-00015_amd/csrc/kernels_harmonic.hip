@@ -576,7 +576,7 @@ __device__ __forceinline__ double pr0_of(const double2* uz0, size_t hs, int N, c
 // the others stream factors during the solves).
 template <int GP, int IT, int THR>
 __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const StepTiles& ST, int slot0) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   SlotCtl* ctl = &B.ctl[slot];
   if (!ctl->active) return;
   // XCD-aware group order: consecutive workgroup ids go to different XCDs,

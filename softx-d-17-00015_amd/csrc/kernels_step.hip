@@ -243,7 +243,7 @@ __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alp
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;  // pair (|k|, m)
@@ -325,7 +325,7 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated) {
 __global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const double* F = lu_block(B, slot, mat, p, P, N);
@@ -426,7 +426,7 @@ __global__ void k_norm_sum(Geo G, const double* __restrict__ b, double* __restri
 // ---------------------------------------------------------------------------
 __global__ void k_influence(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;
@@ -476,7 +476,7 @@ __global__ void k_influence(Geo G, Bufs B, int slot0, int gated) {
 // Inverted 4x4 influence matrices per harmonic (velocity.f90:219-225).
 __global__ void k_ainv(Geo G, Bufs B, int slot0, int gated) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   if (h >= G.H || (gated && !B.ctl[slot].refactor)) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
@@ -546,7 +546,7 @@ __device__ void curl_lane(const Geo& G, int h, int k, int m, double alpha,
 // which = 0: curl of U[ucur]; used after put_state and after every step.
 __global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active || h >= G.H) return;
   if (gated && (c->mode != MODE_PRED || B.prm[slot].symm == 0)) return;
@@ -568,7 +568,7 @@ __global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
 // enabled modes in order in registers.  gated: only slots about to run a
 // predictor (the start of a step, main.f90:21) with a symmetry enabled.
 __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   const int symm = B.prm[slot].symm;
   if (!c->active || (gated && (c->mode != MODE_PRED || symm == 0))) return;
@@ -906,7 +906,7 @@ __global__ void k_nn_addN(Geo G, Bufs B, int slot, int comp, const double2* __re
 // with the C library's cos and sin as the reference's dcos and dsin.  One
 // thread per (component, n, harmonic).
 __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ tab) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const double2* tb = tab + (size_t)blockIdx.z * (2 * G.K + G.M);
   const double2 on = tb[0];
   if (on.x == 0.0 && on.y == 0.0) return;
@@ -931,7 +931,7 @@ __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ ta
 // var_imposesym(0) (variables.f90:611-616) on harmonic 0 of U[ucur].
 __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (n >= G.N) return;
   double2* u = B.U[c->ucur] + (size_t)slot * 3 * G.coll;
@@ -1033,7 +1033,7 @@ __device__ __forceinline__ void control_decide(SlotCtl& k, const SlotParams& prm
 }
 
 __global__ void k_control(Geo G, Bufs B, int slot0) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   const SlotCtl k0 = B.ctl[slot];   // one batch of loads
   if (!k0.active) return;
   const SlotParams prm = B.prm[slot];
@@ -1115,7 +1115,7 @@ __global__ void k_control(Geo G, Bufs B, int slot0) {
 // n = l, l+64, ...; the minima are exact, so their order is free.
 // ---------------------------------------------------------------------------
 __global__ void k_newdt(Geo G, Bufs B, int slot0, int what) {
-  const int slot = slot0 + blockIdx.z;
+  const int slot = slot_of(B, slot0);
   SlotCtl* c = &B.ctl[slot];
   SlotParams* prm = &B.prm[slot];
   const int lane = threadIdx.x;

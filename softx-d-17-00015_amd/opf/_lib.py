@@ -67,6 +67,11 @@ PROTOTYPES = {
     "opf_vel_maxtstep": (C.c_int, [vp, C.c_int, dp, ip]),
     "opf_vel_matrices": (C.c_int, [vp, C.c_int, C.c_double]),
     "opf_step_config": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong)]),
+    "opf_tile_config": (C.c_int, [vp, ip, ip, ip]),
+    "opf_ctx_create_tiles": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                       C.c_int, C.c_int]),
+    "opf_launch_mode": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
     "opf_var_coll_norm": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
     "opf_friction": (C.c_int, [vp, C.c_int, dp, dp, dp]),

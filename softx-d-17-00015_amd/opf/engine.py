@@ -82,14 +82,17 @@ def read_state_file(path, N, K, M, Mp=1, r=None):
 
 
 class Context:
-    def __init__(self, N, K, M, Mp=1, capacity=1, r=None):
+    def __init__(self, N, K, M, Mp=1, capacity=1, r=None, pairs=0, tm=0, tz=0):
+        """One resolution class with `capacity` slots; pairs / tm / tz > 0 fix
+        the launch shapes (opf_ctx_create_tiles), 0 takes the tuning table's
+        or the built-in choice."""
         self.lib = _lib.load()
         h = C.c_void_p()
         rr = None
         if r is not None:
             rr = np.ascontiguousarray(r, dtype=np.float64)
-        check(self.lib.opf_ctx_create(C.byref(h), N, K, M, Mp, capacity,
-                                      _dptr(rr) if rr is not None else None))
+        check(self.lib.opf_ctx_create_tiles(C.byref(h), N, K, M, Mp, capacity,
+                                            _dptr(rr) if rr is not None else None, int(pairs), int(tm), int(tz)))
         self.h = h
         dims = [C.c_int() for _ in range(6)]
         nbytes = C.c_size_t()
@@ -359,6 +362,20 @@ class Context:
 
     def step_pairs(self):
         return self.step_config()[0]
+
+    def tile_config(self):
+        """(step pairs per workgroup, z-transform m columns, theta z-lines) in use."""
+        v = [C.c_int() for _ in range(3)]
+        check(self.lib.opf_tile_config(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def launch_mode(self, graph=-1, compact=-1):
+        """Set the time loop's launch modes (graph replay of one iteration,
+        compacted tail launches; -1 leaves one as it is) and return the
+        counters (graph_launches, graph_captures, compact_iterations)."""
+        v = [C.c_longlong() for _ in range(3)]
+        check(self.lib.opf_launch_mode(self.h, int(graph), int(compact), *[C.byref(x) for x in v]))
+        return dict(graph_launches=v[0].value, graph_captures=v[1].value, compact_iterations=v[2].value)
 
     def kernel_time(self, name):
         ms = C.c_double()

@@ -45,6 +45,85 @@ def test_batch_equals_alone_bitwise():
         assert batch.status(s)["step"] == NSTEPS
 
 
+def _mix_context(opf, graph, compact, slow_dt=0.08):
+    """The MIX slots plus one slot at a large time step, which takes several
+    correctors per step and so iterates on alone at the end of every run."""
+    ctx = opf.Context(N, K, M, capacity=len(MIX) + 1)
+    ctx.launch_mode(graph=graph, compact=compact)
+    for s, (Re, al, dt, E0, seed) in enumerate(MIX + [(2400.0, 1.25, slow_dt, 1e-3, 21)]):
+        ctx.setup(s, Re, al, dt)
+        ctx.put_state(s, *_ic(ctx, al, E0, seed))
+    return ctx
+
+
+def test_graph_replay_and_compaction_equal_eager_launches():
+    """The time loop replayed as a captured HIP graph (4 iterations per
+    graph, the remainder launched eagerly), with the run's tail
+    launched over the list of slots still iterating, against eager launches
+    over the whole range: every slot's state and diagnostics are bit for bit
+    equal after each of three runs, with host reads of the states and an
+    eager launch (vel_imposesym) between the runs."""
+    import opf
+    eager = _mix_context(opf, 0, 0)
+    fast = _mix_context(opf, 4, 1)
+    for r in range(3):
+        print(f"run {r}: 10 steps, eager and graph", flush=True)
+        eager.run(10, diag_every=1)
+        fast.run(10, diag_every=1)
+        for s in range(len(MIX) + 1):
+            ue, uf = eager.get_state(s), fast.get_state(s)
+            for c in range(3):
+                assert np.array_equal(ue[c], uf[c]), (r, s, c)
+            assert np.array_equal(eager.diag(s), fast.diag(s)), (r, s)
+            assert eager.status(s) == fast.status(s), (r, s)
+        eager.imposesym(0)
+        fast.imposesym(0)
+    st = fast.status(len(MIX))
+    assert st["last_ncorr"] >= 2, st            # the slow slot iterated on alone
+    counts = fast.launch_mode()
+    assert counts["graph_launches"] > 0 and counts["compact_iterations"] > 0, counts
+    assert eager.launch_mode()["graph_launches"] == 0
+
+
+def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
+    """Every launch shape the autotuner may choose (step pairs per
+    workgroup, z-transform tile widths, even theta tile widths) gives bit for
+    bit the same states after 5 steps at (32,24,12) and (64,64,48); an odd
+    theta tile is refused; a tuning table row is taken by opf_ctx_create for
+    its (N, K, M) and capacity."""
+    import opf
+    from opf import ic
+    for (n, k, m, Re, al, dt) in ((32, 24, 12, 2400.0, 1.25, 0.01), (64, 64, 48, 5300.0, 0.5, 0.005)):
+        base = opf.Context(n, k, m)
+        p0, tm0, tz0 = base.tile_config()
+        base.close()
+        me = opf.host_mesh(n)
+        f = ic.seeded_perturbation(me["r"], me["intrdr"], k, m, al, 5, 1e-3)
+        ref = None
+        tz_half = tz0 if tz0 % 2 else max(2, 2 * (tz0 // 4))
+        for shape in ((p0, tm0, tz0), (1, tm0, tz0), (2, max(1, tm0 // 2), tz0), (4, tm0, tz_half),
+                      (2, 1, 2 if tz0 % 2 == 0 else tz0)):
+            print(f"({n},{k},{m}) shape {shape}", flush=True)
+            ctx = opf.Context(n, k, m, pairs=shape[0], tm=shape[1], tz=shape[2])
+            assert ctx.tile_config() == shape
+            ctx.setup(0, Re, al, dt)
+            ctx.put_state(0, *f)
+            ctx.run(5)
+            u = ctx.get_state(0)
+            ctx.close()
+            if ref is None:
+                ref = u
+            for c in range(3):
+                assert np.array_equal(u[c], ref[c]), (shape, c)
+    with pytest.raises(opf.OpfError):
+        opf.Context(32, 24, 12, tz=9)
+    table = tmp_path / "tuning.tsv"
+    table.write_text("# test table\n32 24 12 1 1 3 2 0 0\n32 24 12 4 2 4 6 0 0\n")
+    monkeypatch.setenv("OPF_TUNING", str(table))
+    assert opf.Context(32, 24, 12).tile_config() == (1, 3, 2)
+    assert opf.Context(32, 24, 12, capacity=6).tile_config() == (2, 4, 6)
+
+
 def test_batch_slot_follows_reference_series():
     import opf
     g = load_golden("mini")
