@@ -248,6 +248,19 @@ int opf_var_norm(opf_ctx* ctx, int slot, int comp, double* E, double* Ek, double
  * Ek(0:K-1), Em(0:M-1) (NULL skips).  Runs on the device. */
 int opf_var_coll_norm(opf_ctx* ctx, int slot, const double* a_re, const double* a_im, double* E, double* Ek,
                       double* Em);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): opf_var_coll_norm
+ * on device arrays, allocating nothing.  a_re, a_im: device arrays of N*H
+ * doubles in the coll layout; out: device array of 1 + K + M doubles that
+ * receives E, Ek(0:K-1), Em(0:M-1).  The per-harmonic partials use a static
+ * per-device buffer of 65536 harmonics (512 KiB, held until the process
+ * ends); H above that is rejected with OPF_ERR_ARG.  The two kernels are
+ * enqueued on `stream` (a hipStream_t of the context's device), or, when
+ * stream is NULL, on the context's stream, which is then waited for.  All
+ * calls on one GPU share the buffer: calls that overlap in time on one GPU
+ * (different streams or threads) race. */
+int opf_var_coll_norm_noalloc(opf_ctx* ctx, int slot, const double* a_re, const double* a_im, double* out,
+                              void* stream);
 /* io_write_friction (io.f90:627-639): Ub, Uc, u_tau of the current state. */
 int opf_friction(opf_ctx* ctx, int slot, double* Ub, double* Uc, double* Ufr);
 /* vel_transform's physical fields (velocity.f90:304-317) of the current

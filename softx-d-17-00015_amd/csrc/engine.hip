@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -1006,6 +1007,8 @@ extern "C" {
 const char* opf_last_error(void) { return g_err.c_str(); }
 int opf_version(void) { return 100; }
 
+static double* scratch_on(int dev);
+
 int opf_ctx_create_tiles(opf_ctx** out, int N, int K, int M, int Mp, int capacity, const double* r, int pairs,
                          int tm, int tz) {
   return guard([&] {
@@ -1015,6 +1018,7 @@ int opf_ctx_create_tiles(opf_ctx** out, int N, int K, int M, int Mp, int capacit
     auto* c = new opf_ctx();
     try {
       create(c, N, K, M, Mp, capacity, r, pairs, tm, tz);
+      (void)scratch_on(c->device);   // opf_var_coll_norm_noalloc's buffer on this device
     } catch (...) {
       opf_ctx_destroy(c);
       throw;
@@ -1475,6 +1479,52 @@ int opf_var_coll_norm(opf_ctx* c, int slot, const double* re, const double* im, 
     if (E) *E = h[0];
     if (Ek) std::copy(h.begin() + 1, h.begin() + 1 + c->K, Ek);
     if (Em) std::copy(h.begin() + 1 + c->K, h.end(), Em);
+  });
+}
+
+// ---- var_coll_norm without allocation --------------------------------------
+// The per-harmonic partials of opf_var_coll_norm_noalloc live in a file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel: its device address, looked up
+// once per device with hipGetSymbolAddress (at context creation), is kept in
+// g_scratch and used instead.
+constexpr int NORM_SCRATCH_MAX = 65536;   // harmonics (512 KiB of doubles)
+static __device__ __align__(256) double scratch[NORM_SCRATCH_MAX];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch;   // device -> address of its scratch
+
+// Address of `scratch` on device `dev` (looked up with dev current).
+static double* scratch_on(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) return it->second;
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != dev) HIPCHK(hipSetDevice(dev));
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !ptr)
+    throw OpfError(OPF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[dev] = (double*)ptr;
+  return (double*)ptr;
+}
+
+int opf_var_coll_norm_noalloc(opf_ctx* c, int slot, const double* re, const double* im, double* out,
+                              void* stream) {
+  // opf_var_coll_norm on device arrays; no device memory is allocated
+  return guard([&] {
+    check_slot(c, slot);
+    if (!re || !im || !out) throw OpfError(OPF_ERR_ARG, "null device pointer");
+    if (c->H > NORM_SCRATCH_MAX)
+      throw OpfError(OPF_ERR_ARG, "H = " + std::to_string(c->H) + " exceeds the " +
+                                      std::to_string(NORM_SCRATCH_MAX) + " harmonics of the norm scratch buffer");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (alpha unknown)");
+    double* b = scratch_on(c->device);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    launch_norm(c, slot, re, im, b, out, s);
+    if (!stream) HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -74,6 +74,7 @@ PROTOTYPES = {
                                   C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
     "opf_var_coll_norm": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
+    "opf_var_coll_norm_noalloc": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
     "opf_friction": (C.c_int, [vp, C.c_int, dp, dp, dp]),
     "opf_get_phys": (C.c_int, [vp, C.c_int, C.c_int, dp]),
     "opf_add_nonlinear": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
