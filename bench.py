@@ -293,68 +293,111 @@ def batched_leg(nstates, steps, cpu_s=0.0):
 
 # serving: Poisson arrivals of integration requests of three sizes (5:3:2),
 # one corrector per step (steporbit), T ~ U[5, 50] at dt = 0.01
-SERVE = dict(classes=[(32, 24, 12), (48, 48, 24), (64, 64, 48)], ratio=(5, 3, 2), dt=0.01, seed=77)
-
-
+SERVE = dict(classes=[(32, 24, 12), (48, 48, 24), (64, 64, 48)], ratio=(5, 3, 2), dt=0.01, seed=77,
+             block_slots=(32, 8, 4))
 SERVE_EXES = ("opf_ref_c0", "opf_ref_s48", "opf_ref_c1")   # reference builds of the three classes
 
 
-def serving_leg(nreq, rate, budget_gb, chunk, cpu_s=0.0):
-    """p50/p99 request latency and throughput of the request server; parity
-    of a served request against the reference, and of a served request
-    against the same integration run alone.  With cpu_s > 0, the CPU
-    baseline of the mix: the reference on the first requests, one per host
-    core, all at once (summed state-steps/s), and the CPU latency of every
-    request of the mix run alone on one core."""
-    import opf
+def _serve_requests(nreq, rate, seed, meshes):
+    """The configs' request stream: class by the 5:3:2 ratio, Re ~ U[2000,
+    5300], alpha in {0.5, 1.25}, T ~ U[5, 50] time units at dt = 0.01,
+    seeded initial states, Poisson arrivals at `rate` per second (all at
+    once for rate = inf)."""
     from opf import ic
     classes, ratio, dt = SERVE["classes"], np.array(SERVE["ratio"], float), SERVE["dt"]
-    rng = np.random.default_rng(SERVE["seed"])
-    per = [opf.bytes_needed(N, K, M, 1) for (N, K, M) in classes]
-    w = ratio * np.array(per, float)
-    caps = [max(1, int(budget_gb * 1e9 * wi / w.sum() / pi)) for wi, pi in zip(w, per)]
-    meshes = [opf.host_mesh(N) for (N, _, _) in classes]
-    reqs = []
-    t = 0.0
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
     for i in range(nreq):
         cls = int(rng.choice(len(classes), p=ratio / ratio.sum()))
         Re = float(rng.uniform(2000.0, 5300.0))
         alpha = float(rng.choice([0.5, 1.25]))
         nsteps = int(round(float(rng.uniform(5.0, 50.0)) / dt))
         E0 = float(10.0 ** rng.uniform(-5.0, -3.0))
-        t += float(rng.exponential(1.0 / rate))
+        t += 0.0 if not np.isfinite(rate) else float(rng.exponential(1.0 / rate))
         N, K, M = classes[cls]
-        st = ic.seeded_perturbation(meshes[cls]["r"], meshes[cls]["intrdr"], K, M, alpha, 1000 + i, E0)
+        st = ic.seeded_perturbation(meshes[cls]["r"], meshes[cls]["intrdr"], K, M, alpha, 1000 * seed + i, E0)
         reqs.append(dict(cls=cls, Re=Re, alpha=alpha, nsteps=nsteps, state=st, arrival=t))
-    gpath = os.path.join(REPO, "tests", "golden", "golden_c0.npz")
-    golden = None
-    if os.path.exists(gpath):
-        with np.load(gpath) as g:
-            golden = dict(ic=g["ic"], state_100=g["state_100"])
-        reqs[0].update(cls=0, Re=2400.0, alpha=1.25, nsteps=100, state=tuple(golden["ic"]))
-    srv = opf.Server(classes, caps, chunk=chunk)
+    return reqs
+
+
+def _serve_once(reqs, pool_bytes, depth):
+    """Submit the stream at its arrival times; latencies and span."""
+    import opf
+    srv = opf.Server(SERVE["classes"], SERVE["block_slots"], pool_bytes, depth=depth)
     t0 = time.perf_counter()
     ids = []
+    peak = 0
     for r in reqs:
         wait = r["arrival"] - (time.perf_counter() - t0)
         if wait > 0:
             time.sleep(wait)
-        ids.append(srv.submit(r["cls"], r["Re"], r["alpha"], dt, r["nsteps"], r["state"], corr_fixed=1))
+        ids.append(srv.submit(r["cls"], r["Re"], r["alpha"], SERVE["dt"], r["nsteps"], r["state"], corr_fixed=1))
     while not all(srv.poll(i)["done"] for i in ids):
+        peak = max(peak, srv.pool()["used"])
         time.sleep(0.005)
     polls = [srv.poll(i) for i in ids]
-    lat = np.array([p["t_done"] - p["t_submit"] for p in polls])
-    span = max(p["t_done"] for p in polls) - min(p["t_submit"] for p in polls)
-    steps = sum(r["nsteps"] for r in reqs)
-    failed = sum(1 for p in polls if p["status"] != 0)
+    return srv, ids, polls, peak
+
+
+def _plain_step_seconds(reps=60):
+    """Seconds per step of each serving class run alone (plain context, one
+    corrector per step), the per-request loop the server is compared with."""
+    import opf
+    from opf import ic
+    out = []
+    for (N, K, M) in SERVE["classes"]:
+        me = opf.host_mesh(N)
+        one = opf.Context(N, K, M)
+        one.setup(0, 3500.0, 1.25, SERVE["dt"])
+        one.put_state(0, *ic.seeded_perturbation(me["r"], me["intrdr"], K, M, 1.25, 7, 1e-4))
+        one.run(5, corr_fixed=1)
+        t0 = time.perf_counter()
+        one.run(reps, corr_fixed=1)
+        out.append((time.perf_counter() - t0) / reps)
+        one.close()
+    return out
+
+
+def serving_leg(nreq, rates, pool_gb, depth, cpu_s=0.0):
+    """The request server on the configs' stream: for each arrival rate, p50/
+    p99 latency, served state-steps/s and each request's slowdown against
+    the same request run alone; the served throughput at saturation against
+    the plain per-request loop over the same mix; parity of a served request
+    against the reference and against the same integration run alone.  With
+    cpu_s > 0, the CPU baseline of the mix: the reference on the first
+    requests, one per host core, all at once (summed state-steps/s), and
+    the CPU latency of every request run alone on one core."""
+    import opf
+    classes, dt = SERVE["classes"], SERVE["dt"]
+    meshes = [opf.host_mesh(N) for (N, _, _) in classes]
+    pool = int(pool_gb * 1e9)
+    t_alone = _plain_step_seconds()
+    sweep = []
+    best = None
+    for k, rate in enumerate(rates):
+        reqs = _serve_requests(nreq, rate, SERVE["seed"] + k, meshes)
+        srv, ids, polls, peak = _serve_once(reqs, pool, depth)
+        lat = np.array([p["t_done"] - p["t_submit"] for p in polls])
+        alone = np.array([r["nsteps"] * t_alone[r["cls"]] for r in reqs])
+        slow = lat / alone
+        span = max(p["t_done"] for p in polls) - min(p["t_submit"] for p in polls)
+        steps = sum(r["nsteps"] for r in reqs)
+        failed = sum(1 for p in polls if p["status"] != 0)
+        row = dict(rate_per_s=rate if np.isfinite(rate) else "all at once", requests=nreq, state_steps=steps,
+                   throughput=steps / span, p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                   slowdown_p50=float(np.percentile(slow, 50)), slowdown_p99=float(np.percentile(slow, 99)),
+                   peak_pool_gb=peak / 1e9, failed=failed)
+        sweep.append(row)
+        print(f"serving: rate {row['rate_per_s']}: {row['throughput']:.0f} state-steps/s, p50 {row['p50_s']:.2f} s, "
+              f"p99 {row['p99_s']:.2f} s, slowdown p50 {row['slowdown_p50']:.2f}", file=sys.stderr, flush=True)
+        if k == len(rates) - 1:
+            best = (reqs, srv, ids, polls)
+        else:
+            srv.close()
+    reqs, srv, ids, polls = best
+    # parity: a served request against the same integration alone
     parity = {}
-    if golden is not None:
-        u = srv.result(ids[0], 0)
-        parity["reference_c0_100_steps_rel_err"] = max(
-            float(np.abs(u[k] - golden["state_100"][k]).max() / np.abs(golden["state_100"][k]).max())
-            for k in range(3))
-    # a request of the mix against the same integration alone
-    j = next(k for k in range(1, len(reqs)) if reqs[k]["cls"] == 0 and polls[k]["status"] == 0)
+    j = next(k for k in range(len(reqs)) if reqs[k]["cls"] == 0 and polls[k]["status"] == 0)
     u = srv.result(ids[j], 0)
     srv.close()
     r = reqs[j]
@@ -366,6 +409,23 @@ def serving_leg(nreq, rate, budget_gb, chunk, cpu_s=0.0):
     v = one.get_state(0)
     parity["alone_vs_in_mix_bitwise"] = bool(all(np.array_equal(u[k], v[k]) for k in range(3)))
     one.close()
+    # the reference's (32,24,12) case served (one corrector per step, as the
+    # reference's own run needed) against the reference's state after 100 steps
+    gpath = os.path.join(REPO, "tests", "golden", "golden_c0.npz")
+    if os.path.exists(gpath):
+        with np.load(gpath) as g:
+            gic, g100 = g["ic"], g["state_100"]
+        srv = opf.Server(classes[:1], SERVE["block_slots"][:1], pool, depth=depth)
+        rid = srv.submit(0, 2400.0, 1.25, dt, 100, tuple(gic), corr_fixed=1)
+        while not srv.poll(rid)["done"]:
+            time.sleep(0.005)
+        u = srv.result(rid, 0)
+        srv.close()
+        parity["reference_c0_100_steps_rel_err"] = max(
+            float(np.abs(u[k] - g100[k]).max() / np.abs(g100[k]).max()) for k in range(3))
+    sat = max(sweep, key=lambda row: row["throughput"])
+    mix_steps = sum(r["nsteps"] for r in reqs)
+    plain_loop = mix_steps / sum(r["nsteps"] * t_alone[r["cls"]] for r in reqs)
     cpu = None
     exes = [os.path.join(REPO, "oracle", "_ref", e) for e in SERVE_EXES]
     if cpu_s > 0 and all(os.path.exists(e) for e in exes):
@@ -389,20 +449,19 @@ def serving_leg(nreq, rate, budget_gb, chunk, cpu_s=0.0):
                                     "no queueing")
         except Exception as e:                       # noqa: BLE001
             print(f"serving cpu baseline failed ({e})", file=sys.stderr)
-    by_cls = {}
-    for c, (N, K, M) in enumerate(classes):
-        lc = lat[[k for k, rq in enumerate(reqs) if rq["cls"] == c]]
-        if lc.size:
-            by_cls[f"({N},{K},{M})"] = dict(requests=int(lc.size), p50_s=float(np.percentile(lc, 50)),
-                                            p99_s=float(np.percentile(lc, 99)))
+    last = sweep[-1]
     return {"metric": "request latency p50/p99 (s) and served state-steps/s",
-            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "throughput": steps / span, "unit": "state-steps/s",
-            "config": {"workload": f"{nreq} requests, Poisson arrivals at {rate}/s, sizes "
-                                   f"{[list(c) for c in classes]} in ratio 5:3:2, Re~U[2000,5300], "
-                                   f"alpha in {{0.5,1.25}}, T~U[5,50] at dt={dt}, one corrector per step",
-                       "slots": caps, "chunk_steps": chunk, "hbm_budget_gb": budget_gb},
-            "failed": failed, "by_class": by_cls, "parity": parity, "cpu_baseline": cpu}
+            "p50_s": last["p50_s"], "p99_s": last["p99_s"], "throughput": sat["throughput"],
+            "unit": "state-steps/s",
+            "config": {"workload": f"{nreq} requests per arrival rate, sizes {[list(c) for c in classes]} in ratio "
+                                   f"5:3:2, Re~U[2000,5300], alpha in {{0.5,1.25}}, T~U[5,50] at dt={dt}, one "
+                                   f"corrector per step, Poisson arrivals",
+                       "rates_per_s": [r if np.isfinite(r) else "all at once" for r in rates],
+                       "block_slots": list(SERVE["block_slots"]), "pool_gb": pool_gb, "depth": depth},
+            "sweep": sweep,
+            "plain_per_request_loop": plain_loop, "plain_s_per_step": t_alone,
+            "speedup_vs_plain_loop": sat["throughput"] / plain_loop,
+            "failed": sum(row["failed"] for row in sweep), "parity": parity, "cpu_baseline": cpu}
 
 
 def load_traffic(kernel, workload):
@@ -424,10 +483,10 @@ def main():
     ap.add_argument("--batch-states", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-rate", type=float, default=2.0)
-    ap.add_argument("--serve-budget-gb", type=float, default=48.0)
-    ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests per arrival rate")
+    ap.add_argument("--serve-rates", default="2,4,8,inf", help="arrival rates per second (inf: all at once)")
+    ap.add_argument("--serve-pool-gb", type=float, default=64.0)
+    ap.add_argument("--serve-depth", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -512,7 +571,8 @@ def main():
 
     serving = None
     if not args.no_serving:
-        serving = serving_leg(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_chunk,
+        rates = [float(v) for v in args.serve_rates.split(",")]
+        serving = serving_leg(args.serve_requests, rates, args.serve_pool_gb, args.serve_depth,
                               0.0 if args.no_cpu_baseline else args.cpu_seconds / 4)
 
     cpu = None

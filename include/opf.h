@@ -309,16 +309,25 @@ int opf_steporbit(opf_ctx* ctx, int nslots, const int* slots, const int* ndts, c
  * A request integrates one state for `nsteps` steps with its own Re, alpha
  * and dt (the time integration steporbit hands to the stepper,
  * newton.f90:743-787, with corr_fixed = 1 correctors per step there;
- * 0 = to convergence).  The server holds one context per resolution class
- * (N, K, M of nkm[3i..3i+2]) with capacity[i] slots, sized by the caller
- * with opf_ctx_bytes_needed; a scheduler thread admits queued requests into
- * free slots, advances all occupied slots of all classes together in chunks
- * of at most `chunk` steps, and retires finished ones.  Results do not depend
- * on what else is being served.  Thread-safe; no counterpart in the
- * reference (its states are integrated one per process). */
+ * 0 = to convergence).  No counterpart in the reference (its states are
+ * integrated one per process).
+ *
+ * Memory: all classes (N, K, M of nkm[3i..3i+2]) draw on one device pool of
+ * pool_bytes.  A class's slots come in blocks of block_slots[i] (one context
+ * and stream each, opf_ctx_bytes_needed(N, K, M, block_slots[i]) bytes),
+ * created when a queued request finds no free slot and the pool has room,
+ * released when idle and another class needs the room.  A request's
+ * footprint is its block's share.
+ * Scheduling: a scheduler thread keeps `depth` (1..8) iterations queued on
+ * every block with running slots; between iterations it admits queued
+ * requests into free slots (they join at the next iteration) and retires
+ * finished ones (final state copied behind the queued iterations).  Slots
+ * advance independently, so results do not depend on what else is served.
+ * Thread-safe. */
 typedef struct opf_server opf_server;
 const char* opf_server_last_error(void);
-int opf_server_create(opf_server** out, int nclass, const int* nkm, const int* capacity, int chunk);
+int opf_server_create(opf_server** out, int nclass, const int* nkm, const int* block_slots, size_t pool_bytes,
+                      int depth);
 int opf_server_destroy(opf_server* srv);
 int opf_server_submit(opf_server* srv, int cls, double Re, double alpha, double dt, int nsteps, int corr_fixed,
                       const double* ur_re, const double* ur_im, const double* ut_re, const double* ut_im,
@@ -329,7 +338,10 @@ int opf_server_poll(opf_server* srv, long long id, int* done, int* status, doubl
 /* Final state of a finished request (coll planes), then forgets the request. */
 int opf_server_result(opf_server* srv, long long id, double* ur_re, double* ur_im, double* ut_re,
                       double* ut_im, double* uz_re, double* uz_im, double* t);
+/* queued requests, requests in slots, state-steps of finished requests */
 int opf_server_stats(opf_server* srv, int* queued, int* running, long long* state_steps);
+/* device pool bytes in use and its limit, blocks (contexts) alive */
+int opf_server_pool(opf_server* srv, size_t* used, size_t* limit, int* blocks);
 
 /* ---- timing hooks --------------------------------------------------------- */
 
@@ -340,17 +352,18 @@ int opf_server_stats(opf_server* srv, int* queued, int* running, long long* stat
 int opf_kernel_timing(opf_ctx* ctx, int enable);
 int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
 
-/* Launch modes of the device-resident time loop (no reference counterpart;
- * for tests and benchmarks).  graph = n > 0: n iterations of the loop are
- * captured as one HIP graph and replayed (not with the variable-viscosity
- * plugins or kernel timing on, which launch eagerly; n <= 64); graph = 0:
- * eager launches; compact = 1 (default):
- * once at most half of a run's slots are still iterating, iterations launch
- * over the list of those slots only.  A negative value leaves a mode as it
- * is.  The counters (any pointer may be NULL) count graph replays, graph
- * captures and compacted iterations since the context was created. */
-int opf_launch_mode(opf_ctx* ctx, int graph, int compact, long long* graph_launches,
-                    long long* graph_captures, long long* compact_iterations);
+/* Launch modes of the device-resident time loop (no reference counterpart).
+ * graph = n > 0: n iterations of the loop are captured as one HIP graph and
+ * replayed (not with the variable-viscosity plugins or kernel timing on,
+ * which launch eagerly; n <= 64); graph = 0: eager launches; the default
+ * comes from the tuning table's row of the context, else 0.  compact = 1
+ * (default): once at most half of a run's slots are still iterating,
+ * iterations launch over the list of those slots only.  A negative value
+ * leaves a mode as it is.  Outputs (any pointer may be NULL): the modes now,
+ * and the graph replays, graph captures and compacted iterations since the
+ * context was created. */
+int opf_launch_mode(opf_ctx* ctx, int graph, int compact, int* graph_now, int* compact_now,
+                    long long* graph_launches, long long* graph_captures, long long* compact_iterations);
 
 /* Launch shapes in use (no reference counterpart): pairs per workgroup of the
  * step kernel, m columns per z-transform workgroup, z-lines per theta

@@ -118,6 +118,7 @@ struct opf_ctx {
   double2* tC = nullptr;
   double2* shiftTab = nullptr;   // var_coll_shift tables, (2K + M) per slot (first use)
   double* normBuf = nullptr;     // opf_var_coll_norm: field, partials and sums (first use)
+  SlotParams* prmHost = nullptr; // pinned parameters of asynchronous admissions (first use)
   int device = 0;                // HIP device of the context
   // launch modes of the time loop: one iteration captured as a HIP graph
   // and replayed; iterations of a run's tail over the list of slots still
@@ -204,11 +205,12 @@ FftTiles choose_tiles(int Z, int T, int M) {
 // kernel's pairs per workgroup and the transforms' tile widths per (N, K, M,
 // capacity) on the device and writes the fastest into a table, tuning.tsv
 // beside libopf.so, or the file OPF_TUNING names (OPF_TUNING=none: no table).
-// Rows: N K M min_capacity pairs tm tz [us per step]; '#' starts a comment.
+// Rows: N K M min_capacity pairs tm tz graph [measured figures]; '#' starts
+// a comment; graph = iterations per captured HIP graph (0: eager launches).
 // A context takes the row of its (N, K, M) with the largest min_capacity not
 // above its capacity.
 struct Tuned {
-  int gp = 0, tm = 0, tz = 0;
+  int gp = 0, tm = 0, tz = 0, graph = 0;
 };
 std::string tuning_path() {
   if (const char* e = std::getenv("OPF_TUNING")) return std::string(e);
@@ -230,13 +232,14 @@ Tuned lookup_tuning(int N, int K, int M, int cap) {
   while (std::getline(f, line)) {
     if (line.empty() || line[0] == '#') continue;
     std::istringstream is(line);
-    int n, k, m, cmin, gp, tm, tz;
-    if (!(is >> n >> k >> m >> cmin >> gp >> tm >> tz)) continue;
+    int n, k, m, cmin, gp, tm, tz, graph;
+    if (!(is >> n >> k >> m >> cmin >> gp >> tm >> tz >> graph)) continue;
     if (n == N && k == K && m == M && cmin <= cap && cmin > best) {
       best = cmin;
       t.gp = gp;
       t.tm = tm;
       t.tz = tz;
+      t.graph = std::max(0, std::min(64, graph));
     }
   }
   return t;
@@ -571,6 +574,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r, i
   c->tiles = choose_tiles(c->Z, c->T, M);
   c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
   const Tuned tuned = lookup_tuning(N, K, M, cap);
+  c->use_graph = tuned.graph;
   {
     // narrower transform tiles than the defaults (the kernels hold at most
     // the default width) from the caller or the table
@@ -869,6 +873,77 @@ void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
   enqueue_iterations(c, p.lo, p.ns, iterations);
 }
 
+hipStream_t ctx_stream(opf_ctx* c) { return c->stream; }
+
+size_t state_rows(const opf_ctx* c) { return 3 * c->G.coll; }
+
+void coll_to_rows(const opf_ctx* c, const double* const planes[6], double2* rows) {
+  for (int comp = 0; comp < 3; ++comp) {
+    double2* d = rows + (size_t)comp * c->G.coll;
+    const double* re = planes[2 * comp];
+    const double* im = planes[2 * comp + 1];
+    for (int n = 0; n < c->N; ++n) {
+      double2* row = d + (size_t)n * c->Hp;
+      for (int hh = 0; hh < c->H; ++hh) row[hh] = make_double2(re[(size_t)hh * c->N + n], im[(size_t)hh * c->N + n]);
+      for (int hh = c->H; hh < c->Hp; ++hh) row[hh] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
+void rows_to_coll(const opf_ctx* c, const double2* rows, double* const planes[6]) {
+  for (int comp = 0; comp < 3; ++comp) {
+    const double2* d = rows + (size_t)comp * c->G.coll;
+    for (int hh = 0; hh < c->H; ++hh)
+      for (int n = 0; n < c->N; ++n) {
+        const double2 v = d[(size_t)n * c->Hp + hh];
+        planes[2 * comp][(size_t)hh * c->N + n] = v.x;
+        planes[2 * comp + 1][(size_t)hh * c->N + n] = v.y;
+      }
+  }
+}
+
+void admit_async(opf_ctx* c, int slot, double Re, double alpha, double dt, int ucur, const double2* rows,
+                 int nsteps, int corr_fixed) {
+  check_slot(c, slot);
+  if (!(dt > 0.0) || !(Re > 0.0) || alpha == 0.0) throw OpfError(OPF_ERR_ARG, "Re, dt must be > 0 and alpha != 0");
+  if (ucur < 0 || ucur > 2 || nsteps < 1 || corr_fixed < 0) throw OpfError(OPF_ERR_ARG, "bad admission");
+  if (c->n_nn || c->n_les || c->n_adaptive || c->n_symm) throw OpfError(OPF_ERR_STATE, "plain slots only");
+  if (!c->prmHost) HIPCHK(hipHostMalloc((void**)&c->prmHost, sizeof(SlotParams) * c->cap, hipHostMallocDefault));
+  SlotParams& p = c->prmHost[slot];
+  std::memset(&p, 0, sizeof p);
+  p.Re = Re; p.alpha = alpha; p.dt = dt; p.implicit = 0.51; p.nu = 1.0;   // velocity.f90:51
+  p.dterr_tol = 1e-5;
+  p.courant = 0.5;
+  p.const_flux = 1;
+  p.maxdt = 1e99;
+  c->alpha_h[slot] = alpha;
+  c->Re_h[slot] = Re;
+  hipLaunchKernelGGL(k_admit, dim3(1), dim3(1), 0, c->stream, c->B, slot, 0, 0, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->B.prm + slot, &p, sizeof p, hipMemcpyHostToDevice, c->stream));
+  launch_matrices(c, slot, 1, 0);
+  HIPCHK(hipMemcpyAsync(c->B.U[ucur] + (size_t)slot * 3 * c->G.coll, rows, sizeof(double2) * 3 * c->G.coll,
+                        hipMemcpyHostToDevice, c->stream));
+  launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
+  hipLaunchKernelGGL(k_admit, dim3(1), dim3(1), 0, c->stream, c->B, slot, 1, nsteps, corr_fixed);
+  HIPCHK(hipGetLastError());
+  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+  c->setup[slot] = 1;
+  c->occupied[slot] = 1;
+}
+
+void enqueue_iteration_range(opf_ctx* c, int s0, int ns) { launch_iteration(c, s0, ns); }
+
+void ctl_snapshot_async(opf_ctx* c, SlotCtl* out) {
+  HIPCHK(hipMemcpyAsync(out, c->B.ctl, sizeof(SlotCtl) * c->cap, hipMemcpyDeviceToHost, c->stream));
+}
+
+void fetch_async(opf_ctx* c, int slot, int ucur, double2* rows) {
+  check_slot(c, slot);
+  HIPCHK(hipMemcpyAsync(rows, c->B.U[ucur] + (size_t)slot * 3 * c->G.coll, sizeof(double2) * 3 * c->G.coll,
+                        hipMemcpyDeviceToHost, c->stream));
+}
+
 int run_remaining(opf_ctx* c, RunPlan& p) {
   p.live.clear();
   if (p.ns == 0) return 0;
@@ -1035,6 +1110,7 @@ int opf_ctx_destroy(opf_ctx* c) {
   if (!c) return OPF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->prmHost) (void)hipHostFree(c->prmHost);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : c->tm.pool) (void)hipEventDestroy(e);
@@ -1689,13 +1765,16 @@ extern "C" int opf_kernel_timing(opf_ctx* c, int enable) {
   });
 }
 
-extern "C" int opf_launch_mode(opf_ctx* c, int graph, int compact, long long* graph_launches,
-                               long long* graph_captures, long long* compact_iterations) {
+extern "C" int opf_launch_mode(opf_ctx* c, int graph, int compact, int* graph_now, int* compact_now,
+                               long long* graph_launches, long long* graph_captures,
+                               long long* compact_iterations) {
   return guard([&] {
     if (!c) throw OpfError(OPF_ERR_ARG, "null context");
     if (graph > 64) throw OpfError(OPF_ERR_ARG, "at most 64 iterations per captured graph");
     if (graph >= 0) c->use_graph = graph;
     if (compact >= 0) c->use_compact = compact != 0;
+    if (graph_now) *graph_now = c->use_graph;
+    if (compact_now) *compact_now = c->use_compact;
     if (graph_launches) *graph_launches = c->graph_launches;
     if (graph_captures) *graph_captures = c->graph_captures;
     if (compact_iterations) *compact_iterations = c->compact_iterations;

@@ -113,6 +113,7 @@ __global__ void k_curl(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_symmetry(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_newdt(Geo G, Bufs B, int slot0, int what);
 __global__ void k_imposesym(Geo G, Bufs B, int slot0);
+__global__ void k_admit(Bufs B, int slot, int phase, int nsteps, int corr_fixed);
 __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ tab);
 // non-Newtonian plugin: physical-grid fields of one slot (row-major S(i,j) at S[3i+j])
 struct NnPhys {

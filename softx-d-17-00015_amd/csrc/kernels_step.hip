@@ -928,6 +928,39 @@ __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ ta
   *u = x;
 }
 
+// A request's admission into a free slot, in stream order between
+// iterations of the other slots (one thread).  phase 0, before its factors
+// are built: clear the failure status the factor kernels report into.
+// phase 1, after the state is loaded: a fresh run of nsteps steps with
+// corr_fixed correctors per step (put_state + run_begin of the synchronous
+// path), active only if the factorisation succeeded.
+__global__ void k_admit(Bufs B, int slot, int phase, int nsteps, int corr_fixed) {
+  SlotCtl k = B.ctl[slot];
+  if (phase == 0) {
+    k.status = 0;
+    k.active = 0;
+  } else {
+    k.step = 0;
+    k.t = 0.0;
+    k.mode = MODE_PRED;
+    k.it = 0;
+    k.ncorr = 0;
+    k.ndiag = 0;
+    k.dterr_bits = 0;
+    k.corr_dt = 0.0;
+    k.cfl_dt = 0.0;
+    k.cfl_dir = 0;
+    k.ntc = 0;
+    k.refactor = 0;
+    k.target_steps = nsteps;
+    k.corr_fixed = corr_fixed;
+    k.diag_every = 0;
+    k.host_driven = 0;
+    k.active = k.status == 0 ? 1 : 0;
+  }
+  B.ctl[slot] = k;
+}
+
 // var_imposesym(0) (variables.f90:611-616) on harmonic 0 of U[ucur].
 __global__ void k_imposesym(Geo G, Bufs B, int slot0) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;

@@ -70,7 +70,7 @@ PROTOTYPES = {
     "opf_tile_config": (C.c_int, [vp, ip, ip, ip]),
     "opf_ctx_create_tiles": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                        C.c_int, C.c_int]),
-    "opf_launch_mode": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+    "opf_launch_mode": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
     "opf_var_coll_norm": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
@@ -79,13 +79,14 @@ PROTOTYPES = {
     "opf_get_phys": (C.c_int, [vp, C.c_int, C.c_int, dp]),
     "opf_add_nonlinear": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
     "opf_server_last_error": (C.c_char_p, []),
-    "opf_server_create": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, C.c_int]),
+    "opf_server_create": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, C.c_size_t, C.c_int]),
     "opf_server_destroy": (C.c_int, [vp]),
     "opf_server_submit": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                     dp, dp, dp, dp, dp, dp, C.POINTER(C.c_longlong)]),
     "opf_server_poll": (C.c_int, [vp, C.c_longlong, ip, ip, dp, dp, dp]),
     "opf_server_result": (C.c_int, [vp, C.c_longlong, dp, dp, dp, dp, dp, dp, dp]),
     "opf_server_stats": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong)]),
+    "opf_server_pool": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), ip]),
 }
 
 

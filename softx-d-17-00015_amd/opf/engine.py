@@ -372,10 +372,14 @@ class Context:
     def launch_mode(self, graph=-1, compact=-1):
         """Set the time loop's launch modes (graph replay of one iteration,
         compacted tail launches; -1 leaves one as it is) and return the
-        counters (graph_launches, graph_captures, compact_iterations)."""
+        modes now and the counters (graph_launches, graph_captures,
+        compact_iterations)."""
+        g, cp = C.c_int(), C.c_int()
         v = [C.c_longlong() for _ in range(3)]
-        check(self.lib.opf_launch_mode(self.h, int(graph), int(compact), *[C.byref(x) for x in v]))
-        return dict(graph_launches=v[0].value, graph_captures=v[1].value, compact_iterations=v[2].value)
+        check(self.lib.opf_launch_mode(self.h, int(graph), int(compact), C.byref(g), C.byref(cp),
+                                       *[C.byref(x) for x in v]))
+        return dict(graph=g.value, compact=cp.value, graph_launches=v[0].value, graph_captures=v[1].value,
+                    compact_iterations=v[2].value)
 
     def kernel_time(self, name):
         ms = C.c_double()

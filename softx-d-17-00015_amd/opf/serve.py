@@ -1,9 +1,10 @@
 """Python host of the request server (opf_server_* in include/opf.h).
 
-``Server`` holds one device context per resolution class and a native
-scheduler thread that continuously batches integration requests: a request
-is admitted into a free slot of its class, advanced together with every
-other occupied slot of every class, and retired when done.
+``Server`` holds a device memory pool shared by the resolution classes and a
+native scheduler thread that continuously batches integration requests: a
+request is admitted into a free slot of its class (a new block of slots is
+taken from the pool when none is free) between two iterations of the other
+slots, advanced with every running slot of its block, and retired when done.
 """
 from __future__ import annotations
 
@@ -25,20 +26,20 @@ def _check(lib, rc):
 
 
 class Server:
-    def __init__(self, classes, capacity, chunk=50):
-        """classes: list of (N, K, M); capacity: slots per class."""
+    def __init__(self, classes, block_slots, pool_bytes, depth=3):
+        """classes: list of (N, K, M); block_slots: slots per block of each
+        class; pool_bytes: the device memory the server may hold; depth:
+        iterations kept queued per block."""
         self.lib = _lib.load()
         self.classes = [tuple(int(v) for v in c) for c in classes]
         nkm = (C.c_int * (3 * len(classes)))(*[v for c in self.classes for v in c])
-        cap = (C.c_int * len(classes))(*[int(v) for v in capacity])
+        bs = (C.c_int * len(classes))(*[int(v) for v in block_slots])
         h = C.c_void_p()
-        _check(self.lib, self.lib.opf_server_create(C.byref(h), len(classes), nkm, cap, int(chunk)))
+        _check(self.lib, self.lib.opf_server_create(C.byref(h), len(classes), nkm, bs, int(pool_bytes),
+                                                    int(depth)))
         self.h = h
         self.H = [(2 * K - 1) * M - (K - 1) for (_, K, M) in self.classes]
-
-    @staticmethod
-    def bytes_needed(classes, capacity):
-        return sum(bytes_needed(N, K, M, c) for (N, K, M), c in zip(classes, capacity))
+        self.block_bytes = [bytes_needed(N, K, M, b) for (N, K, M), b in zip(self.classes, block_slots)]
 
     def close(self):
         if getattr(self, "h", None):
@@ -87,3 +88,9 @@ class Server:
         ss = C.c_longlong()
         _check(self.lib, self.lib.opf_server_stats(self.h, C.byref(q), C.byref(r), C.byref(ss)))
         return dict(queued=q.value, running=r.value, state_steps=ss.value)
+
+    def pool(self):
+        used, limit = C.c_size_t(), C.c_size_t()
+        nb = C.c_int()
+        _check(self.lib, self.lib.opf_server_pool(self.h, C.byref(used), C.byref(limit), C.byref(nb)))
+        return dict(used=used.value, limit=limit.value, blocks=nb.value)

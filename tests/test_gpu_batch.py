@@ -118,10 +118,13 @@ def test_launch_shapes_do_not_change_results(tmp_path, monkeypatch):
     with pytest.raises(opf.OpfError):
         opf.Context(32, 24, 12, tz=9)
     table = tmp_path / "tuning.tsv"
-    table.write_text("# test table\n32 24 12 1 1 3 2 0 0\n32 24 12 4 2 4 6 0 0\n")
+    table.write_text("# test table\n32 24 12 1 1 3 2 0 0 0\n32 24 12 4 2 4 6 16 0 0\n")
     monkeypatch.setenv("OPF_TUNING", str(table))
     assert opf.Context(32, 24, 12).tile_config() == (1, 3, 2)
-    assert opf.Context(32, 24, 12, capacity=6).tile_config() == (2, 4, 6)
+    c6 = opf.Context(32, 24, 12, capacity=6)
+    assert c6.tile_config() == (2, 4, 6)
+    assert c6.launch_mode()["graph"] == 16
+    assert opf.Context(32, 24, 12).launch_mode()["graph"] == 0
 
 
 def test_batch_slot_follows_reference_series():

@@ -1,9 +1,10 @@
-"""Request server: integrations of mixed resolution admitted into free slots,
-batched with whatever else is running, retired when done.  Every request's
-result is bit-identical to the same integration run alone in a plain
-context, whatever it was served with, and the reference's mini case served
-with one corrector per step (the reference's run needed exactly one every
-step) reproduces the reference's state after 100 steps."""
+"""Request server: integrations of mixed resolution admitted into free slots
+from one device memory pool, between iterations of whatever else is
+running, retired when done.  Every request's result is bit-identical to the
+same integration run alone in a plain context, whatever it was served with,
+and the reference's mini case served with one corrector per step (the
+reference's run needed exactly one every step) reproduces the reference's
+state after 100 steps."""
 from __future__ import annotations
 
 import time
@@ -38,7 +39,9 @@ def test_served_equals_alone_and_reference():
     for (c, Re, al, dt, n, cf, E0, seed) in REQS:
         N, K, M = CLASSES[c]
         states.append(tuple(g["ic"]) if E0 is None else _ic(N, K, M, al, E0, seed))
-    srv = opf.Server(CLASSES, capacity=[3, 2], chunk=7)
+    # a pool of one block per class: 5 + 3 requests wait for 3 + 2 slots
+    pool = opf.bytes_needed(*CLASSES[0], 3) + opf.bytes_needed(*CLASSES[1], 2)
+    srv = opf.Server(CLASSES, block_slots=[3, 2], pool_bytes=pool)
     ids = [srv.submit(c, Re, al, dt, n, st, corr_fixed=cf)
            for (c, Re, al, dt, n, cf, _, _), st in zip(REQS, states)]
     print("submitted", len(ids), "requests", flush=True)
@@ -62,7 +65,7 @@ def test_served_equals_alone_and_reference():
         assert abs(u[3] - n * dt) < 1e-9 * max(1.0, n * dt)
     srv.close()
     # the reference's case
-    srv = opf.Server(CLASSES[:1], capacity=[1], chunk=50)
+    srv = opf.Server(CLASSES[:1], block_slots=[1], pool_bytes=opf.bytes_needed(*CLASSES[0], 1))
     rid = srv.submit(0, 2400.0, 1.25, 0.01, 100, tuple(g["ic"]), corr_fixed=1)
     while not srv.poll(rid)["done"]:
         time.sleep(0.01)
@@ -72,9 +75,47 @@ def test_served_equals_alone_and_reference():
     srv.close()
 
 
+def test_pool_shared_between_classes():
+    """A pool that holds one block of either class but not both: requests of
+    the second class wait until the first class's block is idle and released,
+    then run; the pool never holds more than its limit; results equal the
+    integrations run alone."""
+    import opf
+    b0, b1 = opf.bytes_needed(*CLASSES[0], 2), opf.bytes_needed(*CLASSES[1], 2)
+    pool = max(b0, b1) + min(b0, b1) // 2
+    srv = opf.Server(CLASSES, block_slots=[2, 2], pool_bytes=pool)
+    reqs = [(0, 2400.0, 1.25, 0.01, 30, 1, 1e-3, 31), (1, 2600.0, 1.0, 0.01, 20, 1, 1e-3, 32),
+            (0, 2100.0, 1.7, 0.01, 15, 1, 1e-4, 33), (1, 2300.0, 1.25, 0.01, 10, 0, 2e-3, 34)]
+    states = [_ic(*CLASSES[c], al, E0, seed) for (c, _, al, _, _, _, E0, seed) in reqs]
+    ids = [srv.submit(c, Re, al, dt, n, st, corr_fixed=cf) for (c, Re, al, dt, n, cf, _, _), st in zip(reqs, states)]
+    peak = 0
+    deadline = time.time() + 60
+    while not all(srv.poll(i)["done"] for i in ids):
+        assert time.time() < deadline, srv.stats()
+        pl = srv.pool()
+        peak = max(peak, pl["used"])
+        assert pl["used"] <= pool and pl["blocks"] <= 1, pl
+        time.sleep(0.002)
+    assert peak > 0
+    for rid, (c, Re, al, dt, n, cf, _, _), st in zip(ids, reqs, states):
+        assert srv.poll(rid)["status"] == 0
+        u = srv.result(rid, c)
+        ctx = opf.Context(*CLASSES[c])
+        ctx.setup(0, Re, al, dt)
+        ctx.put_state(0, *st)
+        ctx.run(n, corr_fixed=cf)
+        v = ctx.get_state(0)
+        ctx.close()
+        for k in range(3):
+            assert np.array_equal(u[k], v[k]), (rid, k)
+    srv.close()
+
+
 def test_server_rejects_bad_requests():
     import opf
-    srv = opf.Server(CLASSES[:1], capacity=[1], chunk=10)
+    with pytest.raises(opf.serve.ServerError):            # a pool smaller than one block
+        opf.Server(CLASSES[:1], block_slots=[1], pool_bytes=opf.bytes_needed(*CLASSES[0], 1) - 1)
+    srv = opf.Server(CLASSES[:1], block_slots=[1], pool_bytes=opf.bytes_needed(*CLASSES[0], 1))
     N, K, M = CLASSES[0]
     st = _ic(N, K, M, 1.25, 1e-3, 5)
     with pytest.raises(opf.serve.ServerError):
@@ -132,7 +173,8 @@ def test_serving_mix_alone_equals_in_mix_and_reference():
                        golden=cls)
         states[j] = ic.seeded_perturbation(me["r"], me["intrdr"], K, M, float(g["alpha"]), int(g["seed"]),
                                            float(g["E0"]))
-    srv = opf.Server(MIX_CLASSES, capacity=[6, 4, 2], chunk=10)
+    pool = sum(opf.bytes_needed(N, K, M, b) for (N, K, M), b in zip(MIX_CLASSES, [6, 4, 2]))
+    srv = opf.Server(MIX_CLASSES, block_slots=[6, 4, 2], pool_bytes=pool)
     t0 = time.time()
     ids = []
     for r, st in zip(reqs, states):
