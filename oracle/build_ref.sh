@@ -17,9 +17,14 @@
 # that many MPI ranks (_Nr, _Ns of parallel.h:5-6, a configured copy) against
 # MPICH's mpif.h and libraries in /opt/conda, driven by
 # oracle/ref_mpi_driver.f90; the executable is oracle/_ref/opf_ref_mpi_NAME.
+# With PLUGIN=nonnewtonian or PLUGIN=les the plugin's module and procedures
+# are cut from the reference's plugin source by line range (the module,
+# init_params_model, and get_nu .. update_nonlin_terms; its util, file
+# output and the main.f90 it includes are left out) and the driver's
+# var_null(2) points call them through oracle/plugin_PLUGIN.f90.
 set -euo pipefail
 NAME=$1; N=$2; K=$3; M=$4; MP=${5:-1}; SYMM=${6:-}
-NR=${NR:-1}; NS=${NS:-1}
+NR=${NR:-1}; NS=${NS:-1}; PLUGIN=${PLUGIN:-none}
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
@@ -47,6 +52,22 @@ for f in mpi meshs variables transform.fftw3 timestep velocity; do
   cpp -traditional-cpp -P -I"$TMP/program" "$REF/$f.f90" > "$TMP/pp/$f.f90"
 done
 cpp -traditional-cpp -P -I"$TMP/program" parameters.f90 > "$TMP/pp/parameters.f90"
+PLUGOBJ=""
+# the plugin sources get parallel.h's loop macros through the main.f90 they
+# include; the cut-out procedures include it directly
+echo '#include "../parallel.h"' > "$TMP/program/plugin.f90"
+case "$PLUGIN" in
+  none) ;;
+  nonnewtonian)   # module 8-27, init_params_model 62-86, get_nu + update_nonlin_terms 92-247
+    sed -n -e '8,27p' -e '62,86p' -e '92,247p' "$REF/nonnewtonian.f90" >> "$TMP/program/plugin.f90" ;;
+  les)            # module 20-34, init_params_model 69-96, get_nu .. update_nonlin_terms 102-339
+    sed -n -e '20,34p' -e '69,96p' -e '102,339p' "$REF/LES.f90" >> "$TMP/program/plugin.f90" ;;
+  *) echo "unknown PLUGIN $PLUGIN" >&2; exit 2 ;;
+esac
+if [ "$PLUGIN" != none ]; then
+  cpp -traditional-cpp -P -I"$TMP/program" "$TMP/program/plugin.f90" > "$TMP/pp/plugin.f90"
+  PLUGOBJ="plugin.o"
+fi
 cd "$TMP/pp"
 FFLAGS="-O2"
 # the module arrays of the largest sizes exceed 2 GiB of static data
@@ -60,8 +81,10 @@ $FC $FFLAGS -c "$REF/cheby.f" -o cheby.o
 for f in parameters mpi meshs variables transform.fftw3 timestep velocity; do
   $FC $FFLAGS -c $f.f90 2>/dev/null
 done
+if [ -n "$PLUGOBJ" ]; then $FC $FFLAGS -c plugin.f90 -o plugin.o; fi
+$FC $FFLAGS -c "$HERE/plugin_$PLUGIN.f90" -o hook.o
 $FC $FFLAGS -c "$HERE/$DRIVER.f90" -o driver.o
-$FC $FFLAGS -o "$EXE" driver.o parameters.o mpi.o cheby.o meshs.o \
+$FC $FFLAGS -o "$EXE" driver.o hook.o $PLUGOBJ parameters.o mpi.o cheby.o meshs.o \
     variables.o transform.fftw3.o timestep.o velocity.o \
     -L/opt/conda/lib $LIBS -Wl,-rpath,/opt/conda/lib
 echo "built $EXE (N=$N K=$K M=$M Mp=$MP ranks=${NR}x${NS})" >&2

@@ -10,6 +10,11 @@
 ! state is exchanged as raw little-endian float64 in the reference's own
 ! coll layout: Re(1:N,0:H-1) then Im(1:N,0:H-1) for ur, ut, uz.
 !
+! The var_null(2) points of main.f90:23,38 call plugin_hook: nothing
+! (oracle/plugin_none.f90) or the FLAG = 2 branch of a plugin's util
+! (oracle/plugin_nonnewtonian.f90, oracle/plugin_les.f90), which then reads
+! params_model.in from the working directory.
+!
 ! usage: opf_ref_<cfg> state_in out_prefix Re alpha dt nsteps [dump steps...]
 !   dt < 0 selects the adaptive time step of main.f90:24-29 (d_timestep < 0)
 !   with |dt| as the initial tim_dt: vel_maxtstep, tim_new_tstep and, on a
@@ -99,6 +104,7 @@
       call vel_imposesym()
       call vel_transform()
       call vel_nonlinear()
+      call plugin_hook()                      ! var_null(2), main.f90:23
       if(tim_step==0) call dump_stage('N', vel_Nr, vel_Nt, vel_Nz)
       if(adaptive) then                       ! main.f90:24-29
          call vel_maxtstep()
@@ -116,6 +122,7 @@
       do while(tim_it/=0)
          call vel_transform()
          call vel_nonlinear()
+         call plugin_hook()                   ! var_null(2), main.f90:38
          if(tim_step==0) call dump_stage('N', vel_Nr, vel_Nt, vel_Nz)
          call vel_corrector()
          if(tim_step==0) call dump_stage('u', vel_ur, vel_ut, vel_uz)

@@ -42,10 +42,28 @@ WORKLOADS = {
     "sym_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, -1, (), True),
     # shift-reflect symmetry imposed every step (b_shiftrefl, parameters.f90:27)
     "sr_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 20, (1, 20), True),
+    # the variable-viscosity plugins at the var_null(2) points (one corrector
+    # per step, tim_pcn = 1), parameters in PLUGIN_PARAMS
+    "nn_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 10, (1, 10), True),
+    "les_smag_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 10, (1, 10), True),
+    "les_smag3_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 10, (1, 10), True),
+    "les_vreman_mini": (16, 6, 4, 2400.0, 1.25, 0.01, 1e-3, 1, 10, (1, 10), True),
 }
-# executable of each workload (the reference is compiled per resolution and
-# symmetry flag: oracle/build_ref.sh mini_sr 16 6 4 1 shiftrefl)
-EXE = {"adapt_mini": "mini", "adapt_c0": "c0", "sym_mini": "mini", "sr_mini": "mini_sr"}
+# executable of each workload (the reference is compiled per resolution,
+# symmetry flag and plugin: oracle/build_ref.sh mini_sr 16 6 4 1 shiftrefl,
+# PLUGIN=les oracle/build_ref.sh les_mini 16 6 4)
+EXE = {"adapt_mini": "mini", "adapt_c0": "c0", "sym_mini": "mini", "sr_mini": "mini_sr", "nn_mini": "nn_mini",
+       "les_smag_mini": "les_mini", "les_smag3_mini": "les_mini", "les_vreman_mini": "les_mini"}
+# params_model.in of the plugin workloads (nonnewtonian.f90:69-80: rho radius /
+# model / mu_0 mu_inf / lam_CY n_CY a_CY / fc1; LES.f90:76-78: model /
+# C_s filter A_plus) and the same parameters as recorded in the fixture
+PLUGIN_PARAMS = {
+    "nn_mini": ("1000.0 0.01\n1\n0.056 0.0056\n3e-3 0.6 2.0\n0.0\n",
+                dict(rho=1000.0, radius=0.01, mu_0=0.056, mu_inf=0.0056, lam_CY=3e-3, n_CY=0.6, a_CY=2.0)),
+    "les_smag_mini": ("1\n0.1 1 25.0\n", dict(model=1, C_s=0.1, filter=1, A_plus=25.0)),
+    "les_smag3_mini": ("1\n0.1 3 25.0\n", dict(model=1, C_s=0.1, filter=3, A_plus=25.0)),
+    "les_vreman_mini": ("2\n0.1 2 25.0\n", dict(model=2, C_s=0.1, filter=2, A_plus=25.0)),
+}
 SAMPLE_STRIDE = 37      # harmonics kept from the large states (every 37th)
 SAMPLE_COUNT = 400      # at most this many harmonics, evenly spaced, from the largest states
 
@@ -213,6 +231,11 @@ def make(name):
         icf = os.path.join(td, "ic.bin")
         O.write_coll3(icf, ur, ut, uz)
         pre = os.path.join(td, name)
+        if name in PLUGIN_PARAMS:
+            text, prm = PLUGIN_PARAMS[name]
+            with open(os.path.join(td, "params_model.in"), "w") as f:
+                f.write(text)
+            out["plugin"] = np.array(list(prm.items()), dtype=object).astype(str)
         env = dict(os.environ, MKL_THREADING_LAYER="SEQUENTIAL", MKL_NUM_THREADS="1")
         subprocess.run([exe, icf, pre, repr(Re), repr(alpha), repr(dt), str(nsteps)]
                        + [str(s) for s in saves], check=True, cwd=td, env=env,
@@ -254,7 +277,8 @@ def make(name):
 
 if __name__ == "__main__":
     for nm in (sys.argv[1:] or ["mini", "c0", "s48", "c1", "adapt_mini", "adapt_c0", "sym_mini", "sr_mini",
-                                "orbit_mini", "orbit_sr_mini", "newton_alg", "arnoldi_mat", "mesh", "c2"]):
+                                "orbit_mini", "orbit_sr_mini", "newton_alg", "arnoldi_mat", "mesh", "c2",
+                                "nn_mini", "les_smag_mini", "les_smag3_mini", "les_vreman_mini"]):
         if nm == "mesh":
             make_mesh(nm)
         elif nm == "newton_alg":

@@ -263,3 +263,29 @@ def test_les_with_adaptive_time_step_matches_oracle():
     assert o.dt != 0.01 and o.nu > 1.0
     if t_ref is not None:
         assert np.max(np.abs(d[:, 1] - t_ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["nn_mini", "les_smag_mini", "les_smag3_mini", "les_vreman_mini"])
+def test_viscosity_plugins_match_reference_plugin_runs(name):
+    """The device's variable-viscosity plugins against the reference's own
+    plugin procedures (nonnewtonian.f90 / LES.f90 compiled by
+    oracle/build_ref.sh, run by the reference's time loop with the plugin at
+    the var_null(2) points) from a three-dimensional state: the state after
+    1 step <= 1e-12 and after 10 steps <= 1e-10 normwise relative, energy
+    series <= 1e-10, one corrector per step."""
+    from test_oracle_golden import plugin_params
+    g = load_golden(name)
+    kind, prm = plugin_params(g)
+    ctx = _ctx(g)
+    getattr(ctx, kind)(0, **prm)
+    ctx.put_state(0, *g["ic"])
+    for n, tol in ((1, 1e-12), (10, 1e-10)):
+        print(name, n, flush=True)
+        ctx.run(n - ctx.status(0)["step"], diag_every=1)
+        u = ctx.get_state(0)
+        for c in range(3):
+            assert rel_err(u[c], g[f"state_{n}"][c]) <= tol, (n, c)
+    d = ctx.diag(0)
+    rs = g["series"]
+    assert np.max(np.abs(d[:, 2] - rs[:, 2]) / rs[:, 2]) <= 1e-10
+    assert np.all(d[:, 8] == 1)

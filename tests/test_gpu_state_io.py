@@ -2,7 +2,7 @@
 through the C ABI: a saved state continues bit for bit in a fresh context,
 an adaptive slot takes tim_dt, tim_corr_dt and tim_cfl_dt from the file,
 and a state loaded onto another resolution equals the host reading of the
-same file (io_load_coll with interpolation and resizing)."""
+same file and the oracle's io_load_coll (interpolation and resizing)."""
 from __future__ import annotations
 
 import numpy as np
@@ -67,7 +67,7 @@ def test_adaptive_slot_takes_time_step_from_file(tmp_path):
     assert c.timestep(0)["dt"] == abs(float(g["dt"]))
 
 
-def test_load_onto_other_resolution_matches_host_reading(tmp_path):
+def test_load_onto_other_resolution_matches_host_reading_and_oracle(tmp_path):
     import opf
     g = load_golden("mini")
     a = _ctx(opf, g)
@@ -83,5 +83,23 @@ def test_load_onto_other_resolution_matches_host_reading(tmp_path):
     ub = b.get_state(0)
     for c, nm in enumerate(("ur", "ut", "uz")):
         assert np.array_equal(ub[c], host[nm]), nm
+    # against the oracle's io_load_coll of the saved fields (io.f90:171-232,
+    # axis parity S = 1, 1, 0), then vel_imposesym's harmonic-0 projection
+    import opf_oracle as O
+    ua = a.get_state(0)
+    rf, r = opf.host_mesh(int(g["N"]))["r"], b.mesh()["r"]
+    want = [O.io_load_coll(ua[c], int(g["K"]), int(g["M"]), rf, K, M, 1, r, S) for c, S in enumerate((1, 1, 0))]
+    O.Ctx.imposesym0(*want)
+    for c in range(3):
+        assert np.abs(ub[c] - want[c]).max() <= 1e-12 * np.abs(want[c]).max(), c
     b.run(2)
     assert b.status(0)["status"] == 0
+    # the same radial points, other (K, M): harmonics matched by (k, m), bit for bit
+    d = opf.Context(int(g["N"]), K, M)
+    d.setup(0, float(g["Re"]), float(g["alpha"]), float(g["dt"]))
+    d.load_state(0, p)
+    ud = d.get_state(0)
+    want = [O.io_load_coll(ua[c], int(g["K"]), int(g["M"]), rf, K, M, 1, rf, S) for c, S in enumerate((1, 1, 0))]
+    O.Ctx.imposesym0(*want)
+    for c in range(3):
+        assert np.array_equal(ud[c], want[c]), c
