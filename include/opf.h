@@ -284,9 +284,18 @@ int opf_tra_phys2coll(opf_ctx* ctx, const double* phys, double* c_re, double* c_
  * column-major per harmonic), ipiv: lumesh%ipiv(N) per harmonic (1-based).
  * Rows 1..KL (fill-in workspace) are returned as the factorisation left them. */
 int opf_tim_lumesh_get(opf_ctx* ctx, int slot, int which, double* ab, int* ipiv);
-/* tim_lumesh_invert (timestep.f90:102-123) with the slot's factor `which`,
- * BC as for that matrix, on a host coll field (in place). */
-int opf_tim_lumesh_invert(opf_ctx* ctx, int slot, int which, double* b_re, double* b_im);
+/* tim_lumesh_init (timestep.f90:67-96) into factor set `which` (0..3) of a
+ * set-up slot: A = c1 I + c2 Lap for every harmonic (PM = +1 / -1: the
+ * modified Laplacian of u+ / u-), the axis row of parity
+ * S = (m Mp + |PM|) mod 2 and the wall row dr1(:, BC), factored by DGBTF2
+ * (the (0,0) matrix is skipped when BC = 1).  d_alpha is the slot's.  The
+ * set replaces what vel_matrices put there; OPF_ERR_SINGULAR is the
+ * reference's stop 'tim_lumesh_init'. */
+int opf_tim_lumesh_init(opf_ctx* ctx, int slot, int which, int PM, int BC, double c1, double c2);
+/* tim_lumesh_invert(BC, A, b) (timestep.f90:102-123) with the slot's factor
+ * set `which` as A, on a host coll field (in place); BC = 1 zeroes the
+ * (0,0) harmonic, BC < 0 takes the BC the set was built with. */
+int opf_tim_lumesh_invert(opf_ctx* ctx, int slot, int which, int BC, double* b_re, double* b_im);
 
 /* ---- Newton-Krylov clients (newton.f90) -------------------------------------
  * var_coll_shift (variables.f90:487-528) of a slot's state: harmonic (k, m)

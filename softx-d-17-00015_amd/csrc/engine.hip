@@ -119,6 +119,7 @@ struct opf_ctx {
   double2* shiftTab = nullptr;   // var_coll_shift tables, (2K + M) per slot (first use)
   double* normBuf = nullptr;     // opf_var_coll_norm: field, partials and sums (first use)
   SlotParams* prmHost = nullptr; // pinned parameters of asynchronous admissions (first use)
+  std::vector<int> lu_bc;        // BC of each factor set [slot][4] (tim_lumesh_invert's (0,0) rule)
   int device = 0;                // HIP device of the context
   // launch modes of the time loop: one iteration captured as a HIP graph
   // and replayed; iterations of a run's tail over the list of slots still
@@ -353,7 +354,7 @@ void launch_step(opf_ctx* c, int s0, int ns) {
 // block asks for it, the tim_new_dt of this step)
 void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
   const dim3 gp((c->P + 63) / 64, 4, ns);
-  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, s0, gated);
+  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, s0, gated, LuSpec{-1, 0, 0, 0.0, 0.0});
   launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, s0, gated);
   launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, ns), dim3(64), 0, c->G, c->B, s0, gated);
   launch(c, "ainv", k_ainv, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, gated);
@@ -675,6 +676,8 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r, i
   write_ctl(c, ctl);
   c->occupied.assign(cap, 0);
   c->setup.assign(cap, 0);
+  c->lu_bc.assign((size_t)cap * 4, 0);
+  for (int i = 0; i < cap; ++i) c->lu_bc[(size_t)i * 4 + MAT_LNP] = 1;
   c->adapt.assign(cap, opf_ctx::Adapt{});
   c->symm.assign(cap, 0);
   c->alpha_h.assign(cap, 0.0);
@@ -1739,19 +1742,46 @@ int opf_tim_lumesh_get(opf_ctx* c, int slot, int which, double* ab, int* ipiv) {
 
 // tim_lumesh_invert on a host field: solved on the device, lane per harmonic.
 namespace opf {
-__global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, double2* x);
+__global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, int bc, double2* x);
 }
 
-extern "C" int opf_tim_lumesh_invert(opf_ctx* c, int slot, int which, double* bre, double* bim) {
+extern "C" int opf_tim_lumesh_invert(opf_ctx* c, int slot, int which, int BC, double* bre, double* bim) {
   return guard([&] {
     check_slot(c, slot);
     if (which < 0 || which > 3) throw OpfError(OPF_ERR_ARG, "which must be 0..3");
+    if (BC > 1) throw OpfError(OPF_ERR_ARG, "BC must be 0 or 1 (negative: the matrix's own)");
     if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up");
     if (!bre || !bim) throw OpfError(OPF_ERR_ARG, "null field");
+    const int bc = BC >= 0 ? BC : c->lu_bc[(size_t)slot * 4 + which];
     ensure_temps(c);
     coll_to_dev(c, bre, bim, c->tC);
-    launch(c, "lumesh_invert", k_lumesh_invert, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, which, c->tC);
+    launch(c, "lumesh_invert", k_lumesh_invert, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, which, bc, c->tC);
     dev_to_coll(c, c->tC, bre, bim);
+  });
+}
+
+extern "C" int opf_tim_lumesh_init(opf_ctx* c, int slot, int which, int PM, int BC, double c1, double c2) {
+  // tim_lumesh_init (timestep.f90:67-96): c1 I + c2 Lap (PM = +-1: the
+  // modified Laplacian of u+-), the axis parity row and the BC row, then
+  // DGBTF2, for every (|k|, m) pair, into factor set `which` of the slot
+  return guard([&] {
+    check_slot(c, slot);
+    if (which < 0 || which > 3) throw OpfError(OPF_ERR_ARG, "which must be 0..3");
+    if (PM < -1 || PM > 1 || BC < 0 || BC > 1) throw OpfError(OPF_ERR_ARG, "PM must be -1, 0, 1 and BC 0 or 1");
+    if (!c->setup[slot]) throw OpfError(OPF_ERR_STATE, "slot not set up (alpha unknown)");
+    auto ctl = read_ctl(c);
+    ctl[slot].status = 0;
+    write_ctl(c, ctl);
+    const dim3 gp((c->P + 63) / 64, 4, 1);
+    launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot, 0, LuSpec{which, PM, BC, c1, c2});
+    launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, slot, 0);
+    ctl = read_ctl(c);
+    c->lu_bc[(size_t)slot * 4 + which] = BC;
+    if (ctl[slot].status != 0) {
+      ctl[slot].status = 0;
+      write_ctl(c, ctl);
+      throw OpfError(OPF_ERR_SINGULAR, "tim_lumesh_init");
+    }
   });
 }
 

@@ -105,7 +105,14 @@ __host__ __device__ constexpr int tile_m(int Z) { return FFT_TILE / Z; }
 
 __host__ __device__ inline int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
 
-__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated);
+// The matrix tim_lumesh_init builds: mat < 0 the four of vel_matrices
+// (LDp, LDm, LDz with c1 = 1/dt, c2 = -implicit nu/Re; LNp with 0, 1, BC = 1);
+// mat >= 0 only that one, with the given PM, BC, c1, c2.
+struct LuSpec {
+  int mat, PM, BC;
+  double c1, c2;
+};
+__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec);
 __global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_influence(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_ainv(Geo G, Bufs B, int slot0, int gated);

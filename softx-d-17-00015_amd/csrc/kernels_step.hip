@@ -240,11 +240,11 @@ __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alp
 // tim_lumesh_init (timestep.f90:67-96) + DGBTF2, one lane per (pair, matrix).
 // grid: x = pairs/64, y = 4 matrices, z = slot.
 // ---------------------------------------------------------------------------
-__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated) {
+__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
   const int slot = slot_of(B, slot0);
-  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
+  if (p >= G.P || (gated && !B.ctl[slot].refactor) || (spec.mat >= 0 && mat != spec.mat)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;  // pair (|k|, m)
   const SlotParams prm = B.prm[slot];
@@ -252,7 +252,8 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated) {
   const double* me = G.mesh;
   int PM, BC;
   double c1, c2;
-  if (mat == MAT_LNP) { PM = 0; BC = 1; c1 = 0.0; c2 = 1.0; }
+  if (spec.mat >= 0) { PM = spec.PM; BC = spec.BC; c1 = spec.c1; c2 = spec.c2; }
+  else if (mat == MAT_LNP) { PM = 0; BC = 1; c1 = 0.0; c2 = 1.0; }
   else {
     PM = mat == MAT_LDP ? 1 : (mat == MAT_LDM ? -1 : 0);
     BC = 0;
@@ -1207,14 +1208,14 @@ __global__ void k_newdt(Geo G, Bufs B, int slot0, int what) {
 
 // tim_lumesh_invert (timestep.f90:102-123) of one coll field x[n][Hp] with
 // the slot's factor `which`; (0,0) of the pressure matrix gives zero.
-__global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, double2* x) {
+__global__ void k_lumesh_invert(Geo G, Bufs B, int slot, int which, int bc, double2* x) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= G.H) return;
   int k, m;
   harmonic_km(h, G.K, k, m);
   const int N = G.N, P = G.P;
   const int p = m * G.K + (k < 0 ? -k : k);
-  if (which == MAT_LNP && k == 0 && m == 0) {
+  if (bc == 1 && k == 0 && m == 0) {   // the (0,0) pressure system is singular: b = 0
     for (int n = 0; n < N; ++n) x[(size_t)n * G.Hp + h] = make_double2(0.0, 0.0);
     return;
   }

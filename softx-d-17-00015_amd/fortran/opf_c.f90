@@ -287,11 +287,32 @@ module opf_c
          integer(c_int) :: rc
       end function
 
-      function opf_tim_lumesh_invert(ctx, slot, which, b_re, b_im) bind(C, name="opf_tim_lumesh_invert") result(rc)
+      ! tim_lumesh_invert(BC, A, b) (timestep.f90:102-123) with factor set `which`
+      function opf_tim_lumesh_invert(ctx, slot, which, BC, b_re, b_im) bind(C, name="opf_tim_lumesh_invert") &
+            result(rc)
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx
-         integer(c_int), value :: slot, which
+         integer(c_int), value :: slot, which, BC
          real(c_double), intent(inout) :: b_re(*), b_im(*)
+         integer(c_int) :: rc
+      end function
+
+      ! tim_lumesh_init(PM, BC, c1, c2, A) (timestep.f90:67-96) into factor set `which`
+      function opf_tim_lumesh_init(ctx, slot, which, PM, BC, c1, c2) bind(C, name="opf_tim_lumesh_init") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot, which, PM, BC
+         real(c_double), value :: c1, c2
+         integer(c_int) :: rc
+      end function
+
+      ! var_coll_norm(a, E, Ek, Em) (variables.f90:718-751) of any coll field
+      function opf_var_coll_norm(ctx, slot, a_re, a_im, E, Ek, Em) bind(C, name="opf_var_coll_norm") result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: slot
+         real(c_double), intent(in) :: a_re(*), a_im(*)
+         real(c_double), intent(out) :: E, Ek(*), Em(*)
          integer(c_int) :: rc
       end function
 

@@ -58,7 +58,8 @@ PROTOTYPES = {
     "opf_tra_coll2phys": (C.c_int, [vp, dp, dp, dp]),
     "opf_tra_phys2coll": (C.c_int, [vp, dp, dp, dp]),
     "opf_tim_lumesh_get": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
-    "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
+    "opf_tim_lumesh_invert": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
+    "opf_tim_lumesh_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
     "opf_kernel_timing": (C.c_int, [vp, C.c_int]),
     "opf_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
     "opf_slot_adaptive": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),

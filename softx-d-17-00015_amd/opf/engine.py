@@ -299,10 +299,16 @@ class Context:
                                           ipiv.ctypes.data_as(C.POINTER(C.c_int))))
         return ab.transpose(0, 2, 1).copy(), ipiv
 
-    def lumesh_invert(self, slot, which, b):
+    def lumesh_invert(self, slot, which, b, BC=-1):
+        """tim_lumesh_invert(BC, A, b) with factor set `which` of the slot as A
+        (BC < 0: the BC the set was built with)."""
         re, im = _split(b, self.N, self.H)
-        check(self.lib.opf_tim_lumesh_invert(self.h, slot, which, _dptr(re), _dptr(im)))
+        check(self.lib.opf_tim_lumesh_invert(self.h, slot, which, int(BC), _dptr(re), _dptr(im)))
         return _join(re, im)
+
+    def lumesh_init(self, slot, which, PM, BC, c1, c2):
+        """tim_lumesh_init(PM, BC, c1, c2, A) into factor set `which` of the slot."""
+        check(self.lib.opf_tim_lumesh_init(self.h, slot, which, int(PM), int(BC), float(c1), float(c2)))
 
     # -- timing ------------------------------------------------------------------
     def kernel_timing(self, enable=True):

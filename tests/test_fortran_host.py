@@ -1,11 +1,13 @@
-"""The Fortran host (softx-d-17-00015_amd/fortran): the reference's main loop
-(main.f90:18-46) written against procedures with the reference's names
-(vel_predictor, vel_corrector, tim_check_cgce, ...) that call the C ABI.
+"""The Fortran drop-in (softx-d-17-00015_amd/fortran): modules named and
+shaped as the reference's (mpif, meshs, variables, transform, timestep,
+velocity, io) whose procedures call the C ABI.
 
-CPU: the host builds and links against libopf.so.  GPU: run from the
-reference's initial states, its states after 1, 10 and 100 steps match the
-reference's own to the step tolerances, and its corrector counts match the
-reference's series."""
+CPU: the modules build against a parameters module and export the
+reference's procedure names.  GPU: the reference's own main.f90, compiled
+unmodified with its own parameters.f90 against these modules
+(fortran/build_dropin.sh, from __graft_entry__.build), run from a state file
+of the reference's initial state: its vel_energy.dat, vel_friction.dat,
+tim_step.dat and final state file match the reference's own run."""
 from __future__ import annotations
 
 import os
@@ -19,118 +21,129 @@ import pytest
 from conftest import PKG, load_golden, rel_err
 
 FDIR = os.path.join(PKG, "fortran")
-EXE = os.path.join(FDIR, "opf_main")
-
-
-def _write_state(path, u):
-    parts = []
-    for f in u:                       # Re(1:N,0:H-1), Im(1:N,0:H-1), N fastest
-        parts.append(np.asarray(f.real, dtype="<f8").T.ravel())
-        parts.append(np.asarray(f.imag, dtype="<f8").T.ravel())
-    np.concatenate(parts).tofile(path)
-
-
-def _read_state(path, N, H):
-    a = np.fromfile(path, dtype="<f8").reshape(6, H, N)
-    return [(a[2 * c] + 1j * a[2 * c + 1]).T for c in range(3)]
+DROPIN = os.path.join(FDIR, "_dropin")
 
 
 @pytest.mark.skipif(shutil.which("amdflang") is None, reason="amdflang not available")
-def test_fortran_host_builds():
-    subprocess.run(["make", "-s", "-C", FDIR], check=True)
-    assert os.access(EXE, os.X_OK)
-    out = subprocess.run(["nm", "-u", EXE], check=True, capture_output=True, text=True).stdout
-    for sym in ("opf_ctx_create", "opf_slot_setup", "opf_vel_predictor", "opf_vel_corrector",
-                "opf_vel_transform", "opf_vel_nonlinear", "opf_put_state", "opf_get_state"):
-        assert sym in out, sym
-
-
-def test_fortran_host_rejects_missing_arguments():
-    if not os.access(EXE, os.X_OK):
-        pytest.skip("Fortran host not built")
-    r = subprocess.run([EXE], capture_output=True, text=True)
-    assert r.returncode != 0 or "usage" in (r.stdout + r.stderr)
-
-
-@pytest.mark.gpu
-@pytest.mark.parametrize("name,nsteps,key", [("mini", 10, "state_10"), ("c0", 1, "state_1"),
-                                             ("c0", 100, "state_100")])
-def test_fortran_host_matches_reference(name, nsteps, key):
-    if not os.access(EXE, os.X_OK):
-        pytest.skip("Fortran host not built")
-    g = load_golden(name)
-    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
-    H = (2 * K - 1) * M - (K - 1)
+def test_dropin_modules_build():
+    """The modules compile against a parameters module and provide the
+    reference's procedures (timestep.f90, velocity.f90, io.f90 names)."""
     with tempfile.TemporaryDirectory() as td:
-        fin, fout, fser = (os.path.join(td, x) for x in ("in.bin", "out.bin", "series.txt"))
-        _write_state(fin, g["ic"])
-        args = [EXE, str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])),
-                repr(float(g["dt"])), str(nsteps), fin, fout, fser]
-        print("fortran host:", name, nsteps, flush=True)
-        subprocess.run(args, check=True, timeout=120)
-        u = _read_state(fout, N, H)
-        ser = np.loadtxt(fser, ndmin=2)
-    tol = 1e-12 if nsteps == 1 else 1e-10
-    for c in range(3):
-        assert rel_err(u[c], g[key][c]) <= tol, c
-    # correctors per step equal the reference's (series column 8: ncorr)
-    ref_ncorr = g["series"][:nsteps, 8].astype(int)
-    assert np.array_equal(ser[:, 1].astype(int), ref_ncorr)
+        subprocess.run(["make", "-s", "-C", FDIR, f"BUILD={td}"], check=True)
+        lib = os.path.join(td, "libopf_dropin.a")
+        assert os.path.exists(lib)
+        syms = subprocess.run(["nm", lib], check=True, capture_output=True, text=True).stdout
+        for mod, proc in (("velocity", "vel_predictor"), ("velocity", "vel_corrector"), ("velocity", "vel_matrices"),
+                          ("velocity", "vel_precompute"), ("timestep", "tim_lumesh_init"),
+                          ("timestep", "tim_lumesh_invert"), ("timestep", "tim_check_cgce"),
+                          ("transform", "tra_spec2phys"), ("transform", "tra_phys2spec"),
+                          ("variables", "var_coll_norm"), ("variables", "var_coll2spec"),
+                          ("io", "io_load_state"), ("io", "io_write2files")):
+            assert f"M{mod}P{proc}" in syms, (mod, proc)
+        for sym in ("opf_tim_lumesh_init", "opf_vel_predictor", "opf_var_coll_norm", "opf_state_file_read"):
+            assert sym in syms, sym
 
 
-@pytest.mark.gpu
-def test_fortran_host_adaptive_matches_reference():
-    """d_timestep < 0 (main.f90:24-29) in the Fortran host: vel_maxtstep on
-    the device, the host's tim_new_tstep and vel_matrices on a change; the
-    reference's tim_dt of every step and its state after 120 steps."""
-    if not os.access(EXE, os.X_OK):
-        pytest.skip("Fortran host not built")
-    g = load_golden("adapt_mini")
+def _run_dropin(name, td, state_in, timeout=300):
+    exe = os.path.join(DROPIN, f"opf_dropin_{name}")
+    if not os.access(exe, os.X_OK):
+        pytest.skip(f"{exe} not built (needs the reference's main.f90 at build time)")
+    shutil.copy(state_in, os.path.join(td, "state.cdf.in"))
+    print("drop-in main:", name, flush=True)
+    r = subprocess.run([exe], cwd=td, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "...done!" in r.stdout, r.stdout
+    return r.stdout
+
+
+def _state_file(opf, path, g, t=0.0, dt=None):
     N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
-    H = (2 * K - 1) * M - (K - 1)
-    n = int(g["nsteps"])
-    with tempfile.TemporaryDirectory() as td:
-        fin, fout, fser = (os.path.join(td, x) for x in ("in.bin", "out.bin", "series.txt"))
-        _write_state(fin, g["ic"])
-        args = [EXE, str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])),
-                repr(float(g["dt"])), str(n), fin, fout, fser]
-        print("fortran host: adaptive", n, flush=True)
-        subprocess.run(args, check=True, timeout=120)
-        u = _read_state(fout, N, H)
-        ser = np.loadtxt(fser, ndmin=2)
-    ref_dt = g["dtser"][:, 1]
-    assert np.max(np.abs(ser[:, 3] - ref_dt) / ref_dt) <= 1e-12
-    assert np.array_equal(ser[:, 1].astype(int), g["series"][:, 8].astype(int))
-    for c in range(3):
-        assert rel_err(u[c], g[f"state_{n}"][c]) <= 1e-10, c
+    r = opf.host_mesh(N)["r"]
+    ic = g["ic"]
+    opf.write_state_file(path, r, K, M, 1, ic[0], ic[1], ic[2], t=t, Re=float(g["Re"]),
+                         alpha=float(g["alpha"]), dt=abs(float(g["dt"])) if dt is None else dt)
 
 
 @pytest.mark.gpu
-def test_fortran_host_restarts_from_state_file():
-    """io_save_state / io_load_state in the Fortran host: 4 steps written to a
-    state file, 6 more steps from it, against the continuous 10-step run (bit
-    for bit: a fixed-dt run depends only on the state) and the reference's
-    state after 10 steps; the file carries tim_t = 4 dt."""
-    if not os.access(EXE, os.X_OK):
-        pytest.skip("Fortran host not built")
+def test_reference_main_c0_matches_reference_run():
+    """main.f90 + parameters.f90 (32,24,12), Re 2400, alpha 1.25, dt 0.01,
+    100 steps (i_maxtstep), energy and friction every 10 steps
+    (i_save_rate2): every line of vel_energy.dat and vel_friction.dat equals
+    the reference's step to 1e-10 (the files print 12 digits), the state
+    saved at the end (state0001.cdf.dat) equals the reference's after 100
+    steps to 1e-10, with t = 1."""
+    import opf
+    g = load_golden("c0")
+    with tempfile.TemporaryDirectory() as td:
+        _state_file(opf, os.path.join(td, "in.cdf"), g)
+        _run_dropin("c0", td, os.path.join(td, "in.cdf"))
+        en = np.loadtxt(os.path.join(td, "vel_energy.dat"), ndmin=2)
+        fr = np.loadtxt(os.path.join(td, "vel_friction.dat"), ndmin=2)
+        end = opf.read_state_file(os.path.join(td, "state0001.cdf.dat"), 32, 24, 12)
+        start = opf.read_state_file(os.path.join(td, "state0000.cdf.dat"), 32, 24, 12)
+    rs = g["series"]
+    steps = np.arange(0, 100, 10)
+    assert en.shape == (10, 4) and fr.shape == (10, 4)
+    assert np.allclose(en[:, 0], rs[steps, 1], rtol=0, atol=1e-10)
+    for col, ref in ((1, 2), (2, 3), (3, 4)):
+        assert np.max(np.abs(en[:, col] - rs[steps, ref]) / np.abs(rs[steps, ref])) <= 1e-10, col
+    for col, ref in ((1, 5), (2, 6), (3, 7)):
+        assert np.max(np.abs(fr[:, col] - rs[steps, ref]) / np.abs(rs[steps, ref])) <= 1e-10, col
+    for c, nm in enumerate(("ur", "ut", "uz")):
+        assert rel_err(end[nm], g["state_100"][c]) <= 1e-10, nm
+    assert abs(end["t"] - 1.0) <= 1e-12 and start["t"] == 0.0
+
+
+@pytest.mark.gpu
+def test_reference_main_mini_every_step_and_restart():
+    """main.f90 at (16,6,4): the energy of every step of a 10-step run; a
+    4-step run whose final state file restarts a run with d_time < 0 (tim_t
+    from the file) for 6 more steps ends bit for bit where the 10-step run
+    ends, and on the reference's state after 10 steps (1e-10)."""
     import opf
     g = load_golden("mini")
-    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
-    H = (2 * K - 1) * M - (K - 1)
-    common = [str(N), str(K), str(M), repr(float(g["Re"])), repr(float(g["alpha"])), repr(float(g["dt"]))]
     with tempfile.TemporaryDirectory() as td:
-        fin, fmid, fout, fall = (os.path.join(td, x) for x in ("in.bin", "state0001.cdf.dat", "out.bin",
-                                                                "all.bin"))
-        _write_state(fin, g["ic"])
-        print("fortran host: restart", flush=True)
-        subprocess.run([EXE, *common, "4", fin, fmid], check=True, timeout=120)
-        info = opf.state_file_info(fmid)
-        assert (info["N"], info["K"], info["M"]) == (N, K, M)
-        assert abs(info["t"] - 4 * float(g["dt"])) <= 1e-15
-        subprocess.run([EXE, *common, "6", fmid, fout], check=True, timeout=120)
-        subprocess.run([EXE, *common, "10", fin, fall], check=True, timeout=120)
-        u = _read_state(fout, N, H)
-        v = _read_state(fall, N, H)
-    for c in range(3):
-        assert np.array_equal(u[c], v[c]), c
-        assert rel_err(u[c], g["state_10"][c]) <= 1e-10, c
+        a, b, c = (os.path.join(td, x) for x in ("a", "b", "c"))
+        for d in (a, b, c):
+            os.mkdir(d)
+        _state_file(opf, os.path.join(td, "in.cdf"), g)
+        _run_dropin("mini", a, os.path.join(td, "in.cdf"))
+        en = np.loadtxt(os.path.join(a, "vel_energy.dat"), ndmin=2)
+        ten = opf.read_state_file(os.path.join(a, "state0001.cdf.dat"), 16, 6, 4)
+        _run_dropin("mini4", b, os.path.join(td, "in.cdf"))
+        four = os.path.join(b, "state0001.cdf.dat")
+        assert abs(opf.state_file_info(four)["t"] - 0.04) <= 1e-15
+        _run_dropin("mini6r", c, four)
+        rest = opf.read_state_file(os.path.join(c, "state0001.cdf.dat"), 16, 6, 4)
+    rs = g["series"]
+    assert en.shape == (10, 4)
+    assert np.max(np.abs(en[:, 1] - rs[:10, 2]) / rs[:10, 2]) <= 1e-10
+    for k, nm in enumerate(("ur", "ut", "uz")):
+        assert np.array_equal(rest[nm], ten[nm]), nm
+        assert rel_err(ten[nm], g["state_10"][k]) <= 1e-10, nm
+    assert abs(rest["t"] - 0.1) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_reference_main_adaptive_matches_reference():
+    """main.f90 with d_timestep < 0 (main.f90:24-29): the initial tim_dt
+    from the state file, vel_maxtstep on the device, tim_new_tstep and
+    vel_matrices on a change; every tim_dt change in tim_step.dat equals
+    the reference's, and the final state equals the reference's after 120
+    steps (1e-10)."""
+    import opf
+    g = load_golden("adapt_mini")
+    with tempfile.TemporaryDirectory() as td:
+        _state_file(opf, os.path.join(td, "in.cdf"), g, dt=0.01)
+        _run_dropin("adapt_mini", td, os.path.join(td, "in.cdf"))
+        lines = open(os.path.join(td, "tim_step.dat")).read().split("\n")
+        end = opf.read_state_file(os.path.join(td, "state0001.cdf.dat"), 16, 6, 4)
+    changes = [ln.split() for ln in lines if len(ln.split()) == 6]    # format 11: t step dt corr_dt cfl_dt dir
+    ref = g["dtser"]
+    ref_changes = ref[ref[:, 5] == 1]
+    assert len(changes) == len(ref_changes)
+    for got, want in zip(changes, ref_changes):
+        assert int(got[1]) == int(want[0])
+        assert abs(float(got[2]) - want[1]) <= 1e-8 * want[1]          # printed with 9 digits
+    for c, nm in enumerate(("ur", "ut", "uz")):
+        assert rel_err(end[nm], g["state_120"][c]) <= 1e-10, nm

@@ -51,6 +51,37 @@ def test_lumesh_factors_match_oracle(opf, name):
 
 
 @pytest.mark.parametrize("name", ["mini", "c0"])
+def test_lumesh_init_of_any_operator(opf, name):
+    """tim_lumesh_init(PM, BC, c1, c2, A) into a slot's factor set
+    (opf_tim_lumesh_init): with vel_matrices' own coefficients it rebuilds
+    the set bit for bit; with others (u- with c1 = 3, c2 = -0.2, BC = 1,
+    and the plain Laplacian with c1 = 0.5, c2 = 1, BC = 0) its factors and
+    pivots match the oracle's tim_lumesh_init and tim_lumesh_invert(BC, A, b)
+    its solve, including the zero (0,0) harmonic for BC = 1."""
+    g = load_golden(name)
+    ctx = _setup(opf, g)
+    oc = _oracle(g)
+    before = ctx.lumesh_get(0, 0)
+    ctx.lumesh_init(0, 0, 1, 0, 1.0 / float(g["dt"]), -0.51 / float(g["Re"]))
+    after = ctx.lumesh_get(0, 0)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal((oc.N, oc.H)) + 1j * rng.standard_normal((oc.N, oc.H))
+    for which, (PM, BC, c1, c2) in ((1, (-1, 1, 3.0, -0.2)), (2, (0, 0, 0.5, 1.0))):
+        ctx.lumesh_init(0, which, PM, BC, c1, c2)
+        AB, ipiv, skip = oc.lumesh_init(PM, BC, c1, c2)
+        ab, piv = ctx.lumesh_get(0, which)
+        live = ~skip
+        assert np.array_equal(piv[live], ipiv[live]), which
+        assert np.abs(ab[live] - AB[live]).max() <= 1e-13 * np.abs(AB[live]).max(), which
+        x = ctx.lumesh_invert(0, which, b, BC=BC)
+        ref = oc.lumesh_invert((AB, ipiv, skip), b)
+        assert rel_err(x, ref) <= 1e-12, which
+        if BC == 1:
+            assert np.all(x[:, 0] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["mini", "c0"])
 def test_lumesh_invert_matches_oracle(opf, name):
     g = load_golden(name)
     ctx = _setup(opf, g)
