@@ -367,12 +367,16 @@ int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
  * which launch eagerly; n <= 64); graph = 0: eager launches; the default
  * comes from the tuning table's row of the context, else 0.  compact = 1
  * (default): once at most half of a run's slots are still iterating,
- * iterations launch over the list of those slots only.  A negative value
- * leaves a mode as it is.  Outputs (any pointer may be NULL): the modes now,
- * and the graph replays, graph captures and compacted iterations since the
- * context was created. */
-int opf_launch_mode(opf_ctx* ctx, int graph, int compact, int* graph_now, int* compact_now,
-                    long long* graph_launches, long long* graph_captures, long long* compact_iterations);
+ * iterations launch over the list of those slots only.  streams = 2: a run
+ * over four or more slots is split into two halves on two streams, so one
+ * half's step kernels overlap the other half's transforms (default from
+ * the tuning table, else 1).  A negative value (0 for streams) leaves a
+ * mode as it is.  Outputs (any pointer may be NULL): the modes now, and the
+ * graph replays, graph captures and compacted iterations since the context
+ * was created. */
+int opf_launch_mode(opf_ctx* ctx, int graph, int compact, int streams, int* graph_now, int* compact_now,
+                    int* streams_now, long long* graph_launches, long long* graph_captures,
+                    long long* compact_iterations);
 
 /* Launch shapes in use (no reference counterpart): pairs per workgroup of the
  * step kernel, m columns per z-transform workgroup, z-lines per theta

@@ -125,6 +125,9 @@ struct opf_ctx {
   // and replayed; iterations of a run's tail over the list of slots still
   // active (compaction)
   int use_graph = 0, use_compact = 1;   // use_graph: iterations per captured graph (0: eager)
+  int use_streams = 1;           // streams a run's slots are split over (1 or 2)
+  hipStream_t stream2 = nullptr; // the second stream of a split run (first use)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int* smapDev = nullptr;        // compacted slot list on the device [cap]
   std::vector<int> smapHost;     // its host copy (what the device holds)
   struct GraphKey {
@@ -853,6 +856,34 @@ void enqueue_iterations(opf_ctx* c, int s0, int ns, long iterations) {
   c->graph_launches += reps;
 }
 
+// Iterations over two halves of [s0, s0 + ns) on two streams, so that one
+// half's step kernels overlap the other half's transforms (the slots are
+// independent).  The second stream first waits for the context stream's
+// earlier work, and the context stream waits for the second stream's at the
+// end, so every later call on the context sees both.
+void enqueue_split(opf_ctx* c, int s0, int ns, long iterations) {
+  if (!c->stream2) {
+    HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  const int na = ns / 2;
+  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+  hipStream_t main = c->stream;
+  c->stream = c->stream2;
+  try {
+    for (long i = 0; i < iterations; ++i) launch_iteration(c, s0 + na, ns - na);
+  } catch (...) {
+    c->stream = main;
+    throw;
+  }
+  c->stream = main;
+  for (long i = 0; i < iterations; ++i) launch_iteration(c, s0, na);
+  HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+}
+
 void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
   if (p.ns == 0 || iterations <= 0) return;
   const int nl = (int)p.live.size();
@@ -871,6 +902,11 @@ void run_enqueue(opf_ctx* c, const RunPlan& p, long iterations) {
     }
     c->B.smap = nullptr;
     c->compact_iterations += iterations;
+    return;
+  }
+  // two streams for runs over many slots without per-slot plugin launches
+  if (c->use_streams > 1 && p.ns >= 4 && !c->tm.on && c->n_nn == 0 && c->n_les == 0) {
+    enqueue_split(c, p.lo, p.ns, iterations);
     return;
   }
   enqueue_iterations(c, p.lo, p.ns, iterations);
@@ -1113,6 +1149,10 @@ int opf_ctx_destroy(opf_ctx* c) {
   if (!c) return OPF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->prmHost) (void)hipHostFree(c->prmHost);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -1795,16 +1835,19 @@ extern "C" int opf_kernel_timing(opf_ctx* c, int enable) {
   });
 }
 
-extern "C" int opf_launch_mode(opf_ctx* c, int graph, int compact, int* graph_now, int* compact_now,
-                               long long* graph_launches, long long* graph_captures,
+extern "C" int opf_launch_mode(opf_ctx* c, int graph, int compact, int streams, int* graph_now, int* compact_now,
+                               int* streams_now, long long* graph_launches, long long* graph_captures,
                                long long* compact_iterations) {
   return guard([&] {
     if (!c) throw OpfError(OPF_ERR_ARG, "null context");
     if (graph > 64) throw OpfError(OPF_ERR_ARG, "at most 64 iterations per captured graph");
+    if (streams > 2) throw OpfError(OPF_ERR_ARG, "at most 2 streams");
     if (graph >= 0) c->use_graph = graph;
     if (compact >= 0) c->use_compact = compact != 0;
+    if (streams >= 1) c->use_streams = streams;
     if (graph_now) *graph_now = c->use_graph;
     if (compact_now) *compact_now = c->use_compact;
+    if (streams_now) *streams_now = c->use_streams;
     if (graph_launches) *graph_launches = c->graph_launches;
     if (graph_captures) *graph_captures = c->graph_captures;
     if (compact_iterations) *compact_iterations = c->compact_iterations;

@@ -31,6 +31,13 @@ namespace opf {
 
 namespace {
 
+// 1 - r^2 with the product rounded before the difference, as the
+// reference's host arithmetic does (no fused multiply-add)
+__device__ __forceinline__ double one_minus_sq(double r) {
+#pragma clang fp contract(off)
+  return 1.0 - r * r;
+}
+
 __device__ __forceinline__ double2 cmul_tw(double2 a, double2 w, double sgn) {
   // a * (w.x + i sgn w.y)
   const double wy = sgn * w.y;
@@ -435,7 +442,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   // (velocity.f90:464-475)
   const double scale = 1.0 / (double)(G.Z * TT);
   const double rn = G.mesh[MeshOff::make(G.N).r + n];
-  const double ubase = __dsub_rn(1.0, __dmul_rn(rn, rn));
+  const double ubase = one_minus_sq(rn);
   const bool cor = B.prm[slot].coriolis != 0;
   const double cor_d1 = B.prm[slot].cor_d1, cor_s = B.prm[slot].cor_sphi, cor_c = B.prm[slot].cor_cphi;
   double mr = 0.0, mt = 0.0, mz = 0.0;
@@ -450,7 +457,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
     mr = fmax(mr, fabs(ur));
     mt = fmax(mt, fabs(ut));
-    mz = fmax(mz, fabs(__dadd_rn(uz, ubase)));
+    mz = fmax(mz, fabs(uz + ubase));
     double Nr = ut * cz - uz * ct;
     double Nt = uz * cr - ur * cz;
     double Nz = ur * ct - ut * cr;

@@ -921,8 +921,8 @@ __global__ void k_shift(Geo G, Bufs B, int slot0, const double2* __restrict__ ta
   double2 x = *u;
   auto rot = [&](const double2 cs) {
     const double re = x.x;
-    x.x = __dadd_rn(__dmul_rn(re, cs.x), __dmul_rn(x.y, cs.y));
-    x.y = __dsub_rn(__dmul_rn(x.y, cs.x), __dmul_rn(re, cs.y));
+    x.x = re * cs.x + x.y * cs.y;   // products rounded before the sum (no FMA in this file)
+    x.y = x.y * cs.x - re * cs.y;
   };
   if (on.x != 0.0) rot(tb[k + G.K]);
   if (on.y != 0.0) rot(tb[2 * G.K + m]);

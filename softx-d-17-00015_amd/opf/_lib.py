@@ -71,8 +71,8 @@ PROTOTYPES = {
     "opf_tile_config": (C.c_int, [vp, ip, ip, ip]),
     "opf_ctx_create_tiles": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                        C.c_int, C.c_int]),
-    "opf_launch_mode": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
-                                  C.POINTER(C.c_longlong)]),
+    "opf_launch_mode": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, ip, ip, C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "opf_var_norm": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
     "opf_var_coll_norm": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
     "opf_var_coll_norm_noalloc": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),

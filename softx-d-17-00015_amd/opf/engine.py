@@ -375,17 +375,17 @@ class Context:
         check(self.lib.opf_tile_config(self.h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
-    def launch_mode(self, graph=-1, compact=-1):
-        """Set the time loop's launch modes (graph replay of one iteration,
-        compacted tail launches; -1 leaves one as it is) and return the
-        modes now and the counters (graph_launches, graph_captures,
-        compact_iterations)."""
-        g, cp = C.c_int(), C.c_int()
+    def launch_mode(self, graph=-1, compact=-1, streams=0):
+        """Set the time loop's launch modes (iterations per captured graph,
+        compacted tail launches, one or two streams; -1 / 0 leaves one as it
+        is) and return the modes now and the counters (graph_launches,
+        graph_captures, compact_iterations)."""
+        g, cp, ns = C.c_int(), C.c_int(), C.c_int()
         v = [C.c_longlong() for _ in range(3)]
-        check(self.lib.opf_launch_mode(self.h, int(graph), int(compact), C.byref(g), C.byref(cp),
-                                       *[C.byref(x) for x in v]))
-        return dict(graph=g.value, compact=cp.value, graph_launches=v[0].value, graph_captures=v[1].value,
-                    compact_iterations=v[2].value)
+        check(self.lib.opf_launch_mode(self.h, int(graph), int(compact), int(streams), C.byref(g), C.byref(cp),
+                                       C.byref(ns), *[C.byref(x) for x in v]))
+        return dict(graph=g.value, compact=cp.value, streams=ns.value, graph_launches=v[0].value,
+                    graph_captures=v[1].value, compact_iterations=v[2].value)
 
     def kernel_time(self, name):
         ms = C.c_double()

@@ -27,6 +27,7 @@ path).
 from __future__ import annotations
 
 import math
+import time
 
 import numpy as np
 import scipy.linalg
@@ -448,6 +449,70 @@ class DeviceStepper:
         self.ctx.put_state(s, *fields)
         self.ctx.imposesym(s)
         return self.ctx.get_state(s)[:3]
+
+
+def coll_shift(field, K, M, Mp, alpha, dth, dz):
+    """var_coll_shift (variables.f90:487-528) of a host field (N, H): harmonic
+    (k, m) times e^{-i alpha k dz}, then e^{-i m Mp dth}, each factor rounded
+    as the reference applies it (Re' = Re c + Im s, Im' = Im c - Re s), with
+    the C library's cos and sin."""
+    ks, ms = harmonic_km(K, M)
+    out = np.array(field, dtype=np.complex128)
+    re, im = out.real.copy(), out.imag.copy()
+
+    def rot(c, s_):
+        nonlocal re, im
+        r0 = re
+        re = r0 * c[None, :] + im * s_[None, :]
+        im = im * c[None, :] - r0 * s_[None, :]
+
+    if dz != 0.0:
+        a = [dz * float(k) * alpha for k in ks]
+        rot(np.array([math.cos(v) for v in a]), np.array([math.sin(v) for v in a]))
+    if dth != 0.0:
+        a = [dth * float(m) * float(Mp) for m in ms]
+        rot(np.array([math.cos(v) for v in a]), np.array([math.sin(v) for v in a]))
+    return re + 1j * im
+
+
+class ServerStepper:
+    """steporbit (newton.f90:721-775) through the request server: every
+    segment is one integration request (ndts steps of dt = T/ndts, or one
+    step of the last dt for multJ's trajectory direction; one corrector per
+    step), so the searches' integrations are continuously batched with
+    whatever else the server runs.  The end of steporbit, vel_imposesym and
+    the shift back by the orbit's shifts, is applied to the returned states
+    on the host with the device's arithmetic, so results equal opf_steporbit's
+    bit for bit.  Runs without a symmetric subspace (the server's slots
+    impose none)."""
+
+    def __init__(self, server, cls, Re, alpha, Mp=1):
+        self.srv, self.cls = server, int(cls)
+        self.Re, self.alpha, self.Mp = float(Re), float(alpha), int(Mp)
+        _, self.K, self.M = server.classes[self.cls]
+        self.requests = 0
+
+    def steporbit(self, fields, ndts, T, sth, sz, dt1):
+        ids = []
+        for f, nd, t in zip(fields, ndts, T):
+            dt = t / float(nd) if nd != 1 else dt1
+            ids.append(self.srv.submit(self.cls, self.Re, self.alpha, dt, int(nd), f, corr_fixed=1))
+        self.requests += len(ids)
+        out = []
+        for rid, nd, a, b in zip(ids, ndts, sth, sz):
+            while not self.srv.poll(rid)["done"]:
+                time.sleep(0.0005)
+            st = self.srv.poll(rid)["status"]
+            if st != 0:
+                raise RuntimeError(f"steporbit request failed with status {st}")
+            u = [np.array(x) for x in self.srv.result(rid, self.cls)[:3]]
+            u[0][:, 0] = 0.0                          # vel_imposesym: var_imposesym(0) on harmonic 0
+            u[1][:, 0] = u[1][:, 0].real
+            u[2][:, 0] = u[2][:, 0].real
+            if nd != 1:
+                u = [coll_shift(x, self.K, self.M, self.Mp, self.alpha, -a, -b) for x in u]
+            out.append(tuple(u))
+        return out
 
 
 class BatchedStepper:

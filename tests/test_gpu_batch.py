@@ -58,7 +58,7 @@ def _mix_context(opf, graph, compact, slow_dt=0.08):
 
 def test_graph_replay_and_compaction_equal_eager_launches():
     """The time loop replayed as a captured HIP graph (4 iterations per
-    graph, the remainder launched eagerly), with the run's tail
+    graph, the remainder launched eagerly), and split over two streams, with the run's tail
     launched over the list of slots still iterating, against eager launches
     over the whole range: every slot's state and diagnostics are bit for bit
     equal after each of three runs, with host reads of the states and an
@@ -66,18 +66,23 @@ def test_graph_replay_and_compaction_equal_eager_launches():
     import opf
     eager = _mix_context(opf, 0, 0)
     fast = _mix_context(opf, 4, 1)
+    split = _mix_context(opf, 0, 1)
+    split.launch_mode(streams=2)
     for r in range(3):
         print(f"run {r}: 10 steps, eager and graph", flush=True)
         eager.run(10, diag_every=1)
         fast.run(10, diag_every=1)
+        split.run(10, diag_every=1)
         for s in range(len(MIX) + 1):
-            ue, uf = eager.get_state(s), fast.get_state(s)
-            for c in range(3):
-                assert np.array_equal(ue[c], uf[c]), (r, s, c)
-            assert np.array_equal(eager.diag(s), fast.diag(s)), (r, s)
-            assert eager.status(s) == fast.status(s), (r, s)
-        eager.imposesym(0)
-        fast.imposesym(0)
+            ue = eager.get_state(s)
+            for other in (fast, split):
+                uf = other.get_state(s)
+                for c in range(3):
+                    assert np.array_equal(ue[c], uf[c]), (r, s, c)
+                assert np.array_equal(eager.diag(s), other.diag(s)), (r, s)
+                assert eager.status(s) == other.status(s), (r, s)
+        for ctx in (eager, fast, split):
+            ctx.imposesym(0)
     st = fast.status(len(MIX))
     assert st["last_ncorr"] >= 2, st            # the slow slot iterated on alone
     counts = fast.launch_mode()

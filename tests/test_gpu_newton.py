@@ -267,3 +267,43 @@ def test_independent_searches_share_launches():
     # the busiest search needs
     assert batch.launches == max(calls) < sum(calls)
     print("launches batched", batch.launches, "alone", sum(calls), flush=True)
+
+
+def test_searches_through_the_request_server():
+    """Three orbit searches run as threads whose integrations go through one
+    request server (ServerStepper): their requests are batched with each
+    other's on the server, and each search reaches bit for bit the iterates
+    it reaches alone on a plain context (DeviceStepper)."""
+    import threading
+    import opf
+    from opf.newton import DeviceStepper, OrbitNewton, ServerStepper
+    g = load_golden("orbit_mini")
+    N, K, M = int(g["N"]), int(g["K"]), int(g["M"])
+    Re, alpha = float(g["Re"]), float(g["alpha"])
+    me = O.build_mesh(N)
+    starts = [[a * s for a in g["ic"]] for s in (1.0, 0.7, 1.3)]
+    args = dict(m=6, nits=2, rel_err=1e-10, delta=-1.0, epsJ=1e-7)
+    srv = opf.Server([(N, K, M)], block_slots=[4], pool_bytes=opf.bytes_needed(N, K, M, 4))
+    steppers = [ServerStepper(srv, 0, Re, alpha) for _ in starts]
+    searches = [OrbitNewton(st_, N, K, M, 1, alpha, me.intrdr, st, 0.0, 0.0, 0.1, [10], r=me.r)
+                for st_, st in zip(steppers, starts)]
+    res = [None] * len(searches)
+
+    def work(i):
+        res[i] = searches[i].solve(**args)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(searches))]
+    print("searches through the server", flush=True)
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    srv.close()
+    for i, st in enumerate(starts):
+        one = _ctx(opf, g)
+        alone = OrbitNewton(DeviceStepper(one, [0]), N, K, M, 1, alpha, me.intrdr, st, 0.0, 0.0, 0.1, [10], r=me.r)
+        info, nh = alone.solve(**args)
+        assert res[i] is not None and info == res[i][0], i
+        assert np.array_equal(nh.x, res[i][1].x), i
+        assert [r["tol"] for r in alone.history] == [r["tol"] for r in searches[i].history]
+    assert all(s.requests > 0 for s in steppers)

@@ -219,3 +219,15 @@ def test_multj_constraint_rows_for_fixed_period_and_frozen_shift():
     y = on.jacobian_action(nh, dx)
     assert y[0] == dx[0] and y[2] == dx[2]
     assert y[1] == on.dotprod(-1, on.getshiftdir(2, on.x0), dx)
+
+
+def test_host_coll_shift_equals_oracle():
+    """The host var_coll_shift the server-backed stepper applies equals the
+    oracle's var_coll_shift (variables.f90:487-528) bit for bit, for a z
+    shift, a theta shift and both."""
+    from opf.newton import coll_shift
+    on, u, O = _orbit_problem()
+    c = O.Ctx(on.N, on.K, on.M, alpha=on.alpha)
+    for dth, dz in ((0.0, 0.37), (0.21, 0.0), (-0.3, 1.1)):
+        for a in u:
+            assert np.array_equal(coll_shift(a, on.K, on.M, 1, on.alpha, dth, dz), c.coll_shift(a, dth, dz))
