@@ -370,7 +370,7 @@ int opf_kernel_time(opf_ctx* ctx, const char* name, double* ms, int* launches);
  * iterations launch over the list of those slots only.  streams = 2: a run
  * over four or more slots is split into two halves on two streams, so one
  * half's step kernels overlap the other half's transforms (default from
- * the tuning table, else 1).  A negative value (0 for streams) leaves a
+ * the tuning table, else 2).  A negative value (0 for streams) leaves a
  * mode as it is.  Outputs (any pointer may be NULL): the modes now, and the
  * graph replays, graph captures and compacted iterations since the context
  * was created. */

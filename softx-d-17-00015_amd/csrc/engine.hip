@@ -125,7 +125,7 @@ struct opf_ctx {
   // and replayed; iterations of a run's tail over the list of slots still
   // active (compaction)
   int use_graph = 0, use_compact = 1;   // use_graph: iterations per captured graph (0: eager)
-  int use_streams = 1;           // streams a run's slots are split over (1 or 2)
+  int use_streams = 2;           // streams a run's slots are split over (1 or 2; table or 2)
   hipStream_t stream2 = nullptr; // the second stream of a split run (first use)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int* smapDev = nullptr;        // compacted slot list on the device [cap]
@@ -214,7 +214,7 @@ FftTiles choose_tiles(int Z, int T, int M) {
 // A context takes the row of its (N, K, M) with the largest min_capacity not
 // above its capacity.
 struct Tuned {
-  int gp = 0, tm = 0, tz = 0, graph = 0;
+  int gp = 0, tm = 0, tz = 0, graph = 0, streams = 0;
 };
 std::string tuning_path() {
   if (const char* e = std::getenv("OPF_TUNING")) return std::string(e);
@@ -238,12 +238,18 @@ Tuned lookup_tuning(int N, int K, int M, int cap) {
     std::istringstream is(line);
     int n, k, m, cmin, gp, tm, tz, graph;
     if (!(is >> n >> k >> m >> cmin >> gp >> tm >> tz >> graph)) continue;
+    // an optional ninth integer column: streams (a token with a '.' is the
+    // first timing column of a row without it)
+    std::string tok;
+    int streams = 0;
+    if (is >> tok && tok.find('.') == std::string::npos) streams = std::atoi(tok.c_str());
     if (n == N && k == K && m == M && cmin <= cap && cmin > best) {
       best = cmin;
       t.gp = gp;
       t.tm = tm;
       t.tz = tz;
       t.graph = std::max(0, std::min(64, graph));
+      t.streams = streams == 1 || streams == 2 ? streams : 0;
     }
   }
   return t;
@@ -579,6 +585,7 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r, i
   c->fk = fft_kernels(c->tiles.pz, c->tiles.pt);
   const Tuned tuned = lookup_tuning(N, K, M, cap);
   c->use_graph = tuned.graph;
+  c->use_streams = tuned.streams > 0 ? tuned.streams : 2;
   {
     // narrower transform tiles than the defaults (the kernels hold at most
     // the default width) from the caller or the table

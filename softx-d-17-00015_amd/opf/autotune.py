@@ -4,9 +4,10 @@ For each resolution class and capacity it times the device-resident loop
 (opf_run, corrector iterated to convergence) under candidate launch shapes
 (opf_ctx_create_tiles): the fused step kernel's (|k|, m) pairs per workgroup
 (1, 2, 4), the transforms' tile widths (m columns per z-transform
-workgroup, z-lines per theta workgroup, each at most the kernels' default)
-and the number of loop iterations per captured HIP graph (0: eager
-launches), by coordinate descent in that order, and writes the
+workgroup, z-lines per theta workgroup, each at most the kernels' default),
+the number of loop iterations per captured HIP graph (0: eager launches)
+and, from four slots up, one or two streams for a run's slots, by
+coordinate descent in that order, and writes the
 fastest shape per (N, K, M, capacity) to the table that opf_ctx_create reads
 (tuning.tsv beside libopf.so).  Results do not depend on the shapes, so the
 table only changes speed.
@@ -45,11 +46,11 @@ def _states(N, K, M, alpha, cap, mesh):
 
 def time_shape(wl, shape, states):
     """Seconds per state-step of the loop under one launch shape (pairs, tm,
-    tz, graph iterations): best of 3 timed runs after a warm-up."""
+    tz, graph iterations, streams): best of 3 timed runs after a warm-up."""
     from .engine import Context
     N, K, M, cap, Re, alpha, dt, steps = wl
     ctx = Context(N, K, M, capacity=cap, pairs=shape[0], tm=shape[1], tz=shape[2])
-    ctx.launch_mode(graph=shape[3])
+    ctx.launch_mode(graph=shape[3], streams=shape[4])
     try:
         for s in range(cap):
             ctx.setup(s, Re, alpha, dt)
@@ -66,13 +67,14 @@ def time_shape(wl, shape, states):
 
 
 def tune(wl, log=print):
-    """Coordinate descent over (pairs, tm, tz) from the built-in shape;
-    returns (shape, seconds per state-step, default's seconds per state-step)."""
+    """Coordinate descent over (pairs, tm, tz, graph, streams) from the
+    built-in shape; returns (shape, seconds per state-step, default's seconds
+    per state-step)."""
     from .engine import Context, host_mesh
     N, K, M, cap, Re, alpha, dt, steps = wl
     os.environ["OPF_TUNING"] = "none"       # the built-in shape, not an older table's
     probe = Context(N, K, M, capacity=cap)
-    base = probe.tile_config() + (0,)
+    base = probe.tile_config() + (0, 2 if cap >= 4 else 1)
     probe.close()
     states = _states(N, K, M, alpha, cap, host_mesh(N))
     cache = {}
@@ -85,7 +87,7 @@ def tune(wl, log=print):
                 log(f"  {shape}: rejected ({e})")
                 cache[shape] = float("inf")
             log(f"  (N,K,M,cap)=({N},{K},{M},{cap}) pairs={shape[0]} tm={shape[1]} tz={shape[2]} "
-                f"graph={shape[3]}: {cache[shape] * 1e6:.2f} us per state-step")
+                f"graph={shape[3]} streams={shape[4]}: {cache[shape] * 1e6:.2f} us per state-step")
         return cache[shape]
 
     best = tuple(base)
@@ -98,8 +100,9 @@ def tune(wl, log=print):
     # theta tiles stay even (the N_z transforms pair z-lines 2q and 2q + 1)
     cands = {1: lambda d: {d, max(1, d - 1), max(1, (d + 1) // 2)},
              2: lambda d: {d} if d % 2 else {w for w in (d, d - 2, 2 * (d // 4)) if w >= 2},
-             3: lambda d: {0, 4, 16}}
-    for i, dflt in ((1, base[1]), (2, base[2]), (3, 0)):
+             3: lambda d: {0, 4, 16},
+             4: lambda d: {1, 2} if cap >= 4 else {1}}
+    for i, dflt in ((1, base[1]), (2, base[2]), (3, 0), (4, 1)):
         for w in sorted(cands[i](dflt)):
             cand = list(best)
             cand[i] = w
@@ -127,9 +130,9 @@ def main(argv=None):
         f.write("# launch shapes of the time step chosen by opf/autotune.py (python -m opf.autotune)\n")
         f.write(f"# measured {datetime.date.today().isoformat()} on one MI355X (gfx950); opf_ctx_create takes the\n")
         f.write("# row of its (N, K, M) with the largest min_capacity not above its capacity\n")
-        f.write("# N K M min_capacity pairs tm tz graph us_per_state_step default_us_per_state_step\n")
-        for (N, K, M, cap), (p, tm, tz, g), c, c0 in sorted(rows):
-            f.write(f"{N} {K} {M} {cap} {p} {tm} {tz} {g} {c * 1e6:.3f} {c0 * 1e6:.3f}\n")
+        f.write("# N K M min_capacity pairs tm tz graph streams us_per_state_step default_us_per_state_step\n")
+        for (N, K, M, cap), (p, tm, tz, g, ns), c, c0 in sorted(rows):
+            f.write(f"{N} {K} {M} {cap} {p} {tm} {tz} {g} {ns} {c * 1e6:.3f} {c0 * 1e6:.3f}\n")
     print(f"wrote {args.out}", flush=True)
     return 0
 
