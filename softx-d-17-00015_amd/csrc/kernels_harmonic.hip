@@ -626,6 +626,15 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   L.RED = (double*)base;
   base += 8 * sizeof(double);
   L.META = (int*)base;
+  // Stage 1's mesh weights (the radLap band, Ubase, 2r and 1/r^2 of every
+  // row: 12 N doubles) staged in LDS behind E, in the factor ring's second
+  // buffer, which the first solve fills only after stage 2.  Read from
+  // global memory instead, the stencil's nine weight loads per item each
+  // wait a full cache round trip behind the other workgroups' HBM traffic.
+  // Four pairs at N <= 64 always have the room (E + weights = 288 N bytes
+  // against the ring's 19 KiB).
+  constexpr bool MWL = GP == 4 && IT == 2;
+  double* MW = (double*)((unsigned char*)L.FC + e_bytes);
 
   const double th = prm.implicit, th1 = 1.0 - prm.implicit;
   const double c1 = 1.0 / prm.dt;
@@ -656,6 +665,18 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
   // for the corrector), N and, for the corrector, N_
   // (unconditional loads: in the predictor the current iterate is u_ and
   // N_ is not used, and the repeated addresses hit the cache)
+  // the weights first (one batch, ahead of the state loads)
+  constexpr int MWT = (12 * 64 + THR - 1) / THR;
+  double mwv[MWT];
+  if constexpr (MWL) {
+#pragma unroll
+    for (int u = 0; u < MWT; ++u) {
+      const int i = tid + u * THR, r = i - 9 * N;
+      const int src = i < 9 * N ? mo.lapb + i
+                    : r < N ? mo.Ubase + r : r < 2 * N ? mo.twor + (r - N) : mo.rinv2 + (r - 2 * N);
+      mwv[u] = i < 12 * N ? me[src] : 0.0;
+    }
+  }
   double2 vb[IT][3], vc[IT][3], vn[IT][3], vp[IT][3];
   const double2* ucl = corr ? uc : ub;
   const double2* npl = corr ? Np : Nc;
@@ -677,6 +698,11 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
 #pragma unroll
       for (int c = 0; c < 3; ++c) vb[q][c] = make_double2(0.0, 0.0);
     }
+  }
+  if constexpr (MWL) {
+#pragma unroll
+    for (int u = 0; u < MWT; ++u)
+      if (tid + u * THR < 12 * N) MW[tid + u * THR] = mwv[u];
   }
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
@@ -715,8 +741,8 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     }
     const size_t o = (size_t)n * hs + S.h;
     const double ak = prm.alpha * (double)S.k;
-    const double a = ak * me[mo.Ubase + n];
-    const double b = me[mo.twor + n];
+    const double a = ak * (MWL ? MW[9 * N + n] : me[mo.Ubase + n]);
+    const double b = MWL ? MW[10 * N + n] : me[mo.twor + n];
     const double2 ucr = vc[q][0], uct = vc[q][1], ucz = vc[q][2];
     double2 nr = vn[q][0], nt = vn[q][1], nz = vn[q][2];
     nr = make_double2(nr.x + a * ucr.y, nr.y - a * ucr.x);
@@ -743,7 +769,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
     double2 rp = make_double2(nr.x - nt.y, nr.y + nt.x);
     double2 rm = make_double2(nr.x + nt.y, nr.y - nt.x);
     double2 rz = nz;
-    const double r2i = me[mo.rinv2 + n];
+    const double r2i = MWL ? MW[11 * N + n] : me[mo.rinv2 + n];
 #pragma unroll
     for (int dd = -KL; dd <= KL; ++dd) {
       // rows outside the matrix contribute an exact zero (weight 0 on a
@@ -751,7 +777,7 @@ __device__ __forceinline__ void step_body(const Geo& G, const Bufs& B, const Ste
       const int jr = n + dd;
       const bool in = jr >= 0 && jr <= N - 1;
       const int j = in ? jr : n;
-      const double lap_ = me[mo.lapb + (KL - dd) * N + j];
+      const double lap_ = MWL ? MW[(KL - dd) * N + j] : me[mo.lapb + (KL - dd) * N + j];
       const double lap = in ? lap_ : 0.0;
       double ap = c2 * lap, am = c2 * lap, az = c2 * lap;
       if (dd == 0) {
