@@ -201,6 +201,15 @@ def cpu_concurrent(jobs, target_s):
 BATCH = dict(N=32, K=24, M=12, dt=0.01, seed=2024)
 
 
+def batched_step_roofline(N, K, M, nstates, s_per_step, ncorr):
+    """The batched step against the HBM peak: every state's B_alg per step
+    (at the ensemble's mean corrector count) over the time of one step of
+    the whole ensemble."""
+    b = nstates * alg_bytes_per_step(N, K, M, ncorr)
+    gbs = b / s_per_step / 1e9
+    return {"B_alg_bytes": b, "ncorr": ncorr, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
 def batched_leg(nstates, steps, cpu_s=0.0):
     """Batched throughput, the same states advanced one at a time (plain
     per-state loop), and the parity of slot 0, which holds the reference's
@@ -286,6 +295,7 @@ def batched_leg(nstates, steps, cpu_s=0.0):
                                    f"{steps} steps, corrector to convergence",
                        "states": nstates, "steps": steps},
             "ms_per_step": 1000.0 * secs / steps, "setup_s": setup_s,
+            "step_roofline": batched_step_roofline(N, K, M, nstates, secs / steps, float(np.mean(ncorr))),
             "plain_per_state_loop": alone, "speedup_vs_plain_loop": value / alone,
             "mean_last_ncorr": float(np.mean(ncorr)), "parity": parity, "cpu_baseline": cpu,
             "speedup_vs_cpu": value / cpu["value"] if cpu else None}
