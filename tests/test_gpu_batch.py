@@ -45,11 +45,11 @@ def test_batch_equals_alone_bitwise():
         assert batch.status(s)["step"] == NSTEPS
 
 
-def _mix_context(opf, graph, compact, slow_dt=0.08):
+def _mix_context(opf, graph, compact, slow_dt=0.08, streams=1):
     """The MIX slots plus one slot at a large time step, which takes several
     correctors per step and so iterates on alone at the end of every run."""
     ctx = opf.Context(N, K, M, capacity=len(MIX) + 1)
-    ctx.launch_mode(graph=graph, compact=compact)
+    ctx.launch_mode(graph=graph, compact=compact, streams=streams)
     for s, (Re, al, dt, E0, seed) in enumerate(MIX + [(2400.0, 1.25, slow_dt, 1e-3, 21)]):
         ctx.setup(s, Re, al, dt)
         ctx.put_state(s, *_ic(ctx, al, E0, seed))
@@ -66,8 +66,7 @@ def test_graph_replay_and_compaction_equal_eager_launches():
     import opf
     eager = _mix_context(opf, 0, 0)
     fast = _mix_context(opf, 4, 1)
-    split = _mix_context(opf, 0, 1)
-    split.launch_mode(streams=2)
+    split = _mix_context(opf, 0, 1, streams=2)
     for r in range(3):
         print(f"run {r}: 10 steps, eager and graph", flush=True)
         eager.run(10, diag_every=1)
