@@ -333,6 +333,7 @@ void launch(opf_ctx* c, const char* name, Kern k, dim3 grid, dim3 block, size_t 
 }
 
 dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); }
+dim3 grid_hn(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, c->N, nslots); }
 // blocks of k_zfwd_t / k_zbwd_t: their (m tile, n, field) tiles, 6 or 3
 // fields, rounded up to 8 XCDs
 unsigned ztile_blocks(const opf_ctx* c, int nfield) {
@@ -364,8 +365,7 @@ void launch_step(opf_ctx* c, int s0, int ns) {
 void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
   const dim3 gp((c->P + 63) / 64, 4, ns);
   launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, s0, gated, LuSpec{-1, 0, 0, 0.0, 0.0});
-  launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, s0, gated);
-  launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 1, ns), dim3(64), 0, c->G, c->B, s0, gated);
+  launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 5, ns), dim3(64), 0, c->G, c->B, s0, gated);
   launch(c, "ainv", k_ainv, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, gated);
 }
 
@@ -380,7 +380,7 @@ void launch_les(opf_ctx* c, int slot);
 void launch_iteration(opf_ctx* c, int s0, int ns) {
   if (c->n_symm > 0) {   // vel_imposesym at the start of a step (main.f90:21), then the new curl
     launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, s0, 1);
-    launch(c, "curl", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, 1);
+    launch(c, "curl", k_curl, grid_hn(c, ns), dim3(64), 0, c->G, c->B, s0, 1);
   }
   launch_eval(c, s0, ns);
   if (c->n_nn > 0 || c->n_les > 0)   // var_null(2) after vel_nonlinear (main.f90:23,38)
@@ -765,7 +765,7 @@ void put_state(opf_ctx* c, int slot, const double* const* f, double t) {
   k.refactor = 0;
   write_ctl(c, ctl);
   launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
-  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+  launch(c, "curl0", k_curl, grid_hn(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
   ctl = read_ctl(c);
   ctl[slot].active = 0;
@@ -973,7 +973,7 @@ void admit_async(opf_ctx* c, int slot, double Re, double alpha, double dt, int u
   launch(c, "imposesym", k_imposesym, dim3((c->N + 63) / 64, 1, 1), dim3(64), 0, c->G, c->B, slot);
   hipLaunchKernelGGL(k_admit, dim3(1), dim3(1), 0, c->stream, c->B, slot, 1, nsteps, corr_fixed);
   HIPCHK(hipGetLastError());
-  launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+  launch(c, "curl0", k_curl, grid_hn(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
   c->setup[slot] = 1;
   c->occupied[slot] = 1;
 }
@@ -1090,7 +1090,7 @@ void orbit_finish(opf_ctx* c, const std::vector<int>& sel, const std::vector<dou
   write_ctl(c, ctl);
   launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, ns), dim3(256), 0, c->G, c->B, lo, 0);
   launch_shift(c, lo, ns, sel, dth, dz);
-  launch(c, "curl0", k_curl, grid_h(c, ns), dim3(64), 0, c->G, c->B, lo, 0);
+  launch(c, "curl0", k_curl, grid_hn(c, ns), dim3(64), 0, c->G, c->B, lo, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
   ctl = read_ctl(c);
   for (int s : sel) ctl[s].active = 0;
@@ -1432,7 +1432,7 @@ int opf_vel_imposesym(opf_ctx* c, int slot) {
     with_active(c, slot, -1, [&] {
       launch(c, "symmetry", k_symmetry, dim3((3 * c->N * c->H + 255) / 256, 1, 1), dim3(256), 0, c->G, c->B, slot,
              0);
-      launch(c, "curl", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+      launch(c, "curl", k_curl, grid_hn(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
     });
     HIPCHK(hipStreamSynchronize(c->stream));
   });
@@ -1821,7 +1821,6 @@ extern "C" int opf_tim_lumesh_init(opf_ctx* c, int slot, int which, int PM, int 
     write_ctl(c, ctl);
     const dim3 gp((c->P + 63) / 64, 4, 1);
     launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot, 0, LuSpec{which, PM, BC, c1, c2});
-    launch(c, "pack", k_lu_pack, gp, dim3(64), 0, c->G, c->B, slot, 0);
     ctl = read_ctl(c);
     c->lu_bc[(size_t)slot * 4 + which] = BC;
     if (ctl[slot].status != 0) {
@@ -2001,7 +2000,7 @@ extern "C" int opf_vel_shift(opf_ctx* c, int slot, double dth, double dz) {
     ctl[slot].active = 1;
     write_ctl(c, ctl);
     launch_shift(c, slot, 1, {slot}, {dth}, {dz});
-    launch(c, "curl0", k_curl, grid_h(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
+    launch(c, "curl0", k_curl, grid_hn(c, 1), dim3(64), 0, c->G, c->B, slot, 0);
     HIPCHK(hipStreamSynchronize(c->stream));
     ctl = read_ctl(c);
     ctl[slot].active = 0;

@@ -113,7 +113,6 @@ struct LuSpec {
   double c1, c2;
 };
 __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec);
-__global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_influence(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_ainv(Geo G, Bufs B, int slot0, int gated);
 __global__ void k_curl(Geo G, Bufs B, int slot0, int gated);

@@ -237,10 +237,11 @@ __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alp
 }
 
 // ---------------------------------------------------------------------------
-// tim_lumesh_init (timestep.f90:67-96) + DGBTF2, one lane per (pair, matrix).
-// grid: x = pairs/64, y = 4 matrices, z = slot.
+// tim_lumesh_init (timestep.f90:67-96) + DGBTF2, one lane per (pair, matrix),
+// writing the factors in LAPACK band layout and the solve-ready copy (see
+// fw_block / bw_block, meta_block).  grid: x = pairs/64, y = 4 matrices, z = slot.
 // ---------------------------------------------------------------------------
-__global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
+__global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int mat = blockIdx.y;
   const int slot = slot_of(B, slot0);
@@ -262,25 +263,91 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
   }
   double* F = lu_block(B, slot, mat, p, P, N);
   unsigned char* pv = pv_block(B, slot, mat, p, P, N);
-  auto A = [&](int row, int col) -> double& { return F[(size_t)col * LUS + row]; };
-  for (int j = 0; j < N; ++j) {
-    for (int row = 0; row < KL; ++row) A(row, j) = 0.0;
-    for (int q = 0; q < LDA; ++q) A(KL + q, j) = c2 * me[mo.lapb + q * N + j];
-    const double d = diag_d(me[mo.rinv2 + j], G.Mp, m, prm.alpha, k, PM);
-    A(2 * KL, j) = (A(2 * KL, j) + c2 * d) + c1;
-  }
+  // The elimination at column j touches columns j .. j + KV only, so each
+  // thread keeps a window of KV + 1 band columns in LDS (column c in slot
+  // c mod 9, the threads' values interleaved) instead of working on the
+  // factor array in global memory, where every access of the serial
+  // elimination waited a full cache round trip.  A column leaves the window
+  // finished (stored once, 128 contiguous bytes) and the next enters built
+  // from the mesh.  The arithmetic is DGBTF2's, operation for operation.
+  constexpr int WC = 2 * KL + 1;   // window columns
+  __shared__ double Wd[WC * NLU * 64];
+  const int tl = threadIdx.x;
+  auto A = [&](int row, int col) -> double& { return Wd[((col % WC) * NLU + row) * 64 + tl]; };
   const int S = (m * G.Mp + (PM < 0 ? -PM : PM)) % 2;
-  for (int j1 = 1; j1 <= KL + 1; ++j1) A(2 * KL + 1 - j1, j1 - 1) = me[mo.dr0 + (j1 - 1) + (KL + 1) * (1 - S)];
-  for (int j1 = N - KL; j1 <= N; ++j1) A(2 * KL + N - j1, j1 - 1) = me[mo.dr1 + (KL - N + j1) + (KL + 1) * BC];
-  for (int j = 0; j < N; ++j) {
-    pv[j] = 0;
-    for (int row = NLU; row < LUS; ++row) A(row, j) = 0.0;
+  // column `col` of tim_lumesh_init's band (timestep.f90:67-96): the mesh
+  // operator, then the boundary rows of the first and last KL + 1 columns.
+  // fetch() reads the mesh values of a column into registers a column ahead
+  // of place(), so the loads overlap an elimination step.
+  struct Src {
+    double lap[LDA], r2, b0, b1;
+  };
+  auto fetch = [&](int col, Src& c) {
+    for (int q = 0; q < LDA; ++q) c.lap[q] = me[mo.lapb + q * N + col];
+    c.r2 = me[mo.rinv2 + col];
+    c.b0 = col <= KL ? me[mo.dr0 + col + (KL + 1) * (1 - S)] : 0.0;
+    c.b1 = col >= N - KL - 1 ? me[mo.dr1 + (KL - N + 1 + col) + (KL + 1) * BC] : 0.0;
+  };
+  auto place = [&](int col, const Src& c) {
+    for (int row = 0; row < KL; ++row) A(row, col) = 0.0;
+    for (int q = 0; q < LDA; ++q) A(KL + q, col) = c2 * c.lap[q];
+    const double d = diag_d(c.r2, G.Mp, m, prm.alpha, k, PM);
+    A(2 * KL, col) = (A(2 * KL, col) + c2 * d) + c1;
+    if (col <= KL) A(2 * KL - col, col) = c.b0;
+    if (col >= N - KL - 1) A(2 * KL + N - 1 - col, col) = c.b1;
+  };
+  auto build = [&](int col) {
+    Src c;
+    fetch(col, c);
+    place(col, c);
+  };
+  // A finished column leaves as the LAPACK band column (F, pv) and as the
+  // solve-ready copy the fused step kernel reads (fw, bw); meanwhile the
+  // widest superdiagonal of U holding a non-zero and the largest pivot
+  // offset are tracked for the meta block.  The step kernel skips the
+  // superdiagonals that are exactly zero (x - t*0 = x) and multiplies by
+  // r = 1/u0 (correctly rounded) instead of dividing by the diagonal.
+  double2* fw = fw_block(B, slot, mat, p, P, N);
+  double2* bw = bw_block(B, slot, mat, p, P, N);
+  int kue = 1, mxo = 0;
+  auto store = [&](int col, int pj) {
+    double cv[NLU];
+    for (int row = 0; row < NLU; ++row) cv[row] = A(row, col);
+    double* Fc = F + (size_t)col * LUS;
+    for (int row = 0; row < NLU; ++row) Fc[row] = cv[row];
+    for (int row = NLU; row < LUS; ++row) Fc[row] = 0.0;
+    fw[col * FW_Q + 0] = make_double2((double)pj, cv[9]);
+    fw[col * FW_Q + 1] = make_double2(cv[10], cv[11]);
+    fw[col * FW_Q + 2] = make_double2(cv[12], 0.0);
+    const double u0 = cv[2 * KL];
+    bw[col] = make_double2(u0, u0 != 0.0 ? 1.0 / u0 : 0.0);
+    for (int q = 1; q < BW_Q; ++q)
+      bw[(size_t)q * N + col] = make_double2(cv[2 * KL - (2 * q - 1)], cv[2 * KL - 2 * q]);
+    for (int d = 1; d <= 2 * KL; ++d)
+      if (col - d >= 0 && cv[2 * KL - d] != 0.0) kue = d > kue ? d : kue;
+    mxo = pj > mxo ? pj : mxo;
+  };
+  auto finish = [&] {
+    unsigned char* mt = meta_block(B, slot, mat, p, P);
+    mt[0] = (unsigned char)kue;
+    mt[1] = (unsigned char)mxo;
+  };
+  if (BC == 1 && k == 0 && m == 0) {   // pressure (0,0) is never factored
+    for (int j = 0; j < N; ++j) {
+      build(j);
+      store(j, 0);
+      pv[j] = 0;
+    }
+    finish();
+    return;
   }
-  if (BC == 1 && k == 0 && m == 0) return;   // pressure (0,0) is never factored
+  for (int col = 0; col < WC && col < N; ++col) build(col);
   // DGBTF2, KL = KU = 4, KV = 8 (0-based rows)
   const int KV = 2 * KL;
   int ju = 0;
+  Src nx;
   for (int j = 0; j < N; ++j) {
+    if (j + WC < N) fetch(j + WC, nx);
     const int km = N - 1 - j < KL ? N - 1 - j : KL;
     int jp = 0;
     double best = fabs(A(KV, j));
@@ -312,43 +379,10 @@ __global__ void k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
         }
       }
     }
+    store(j, jp);
+    if (j + WC < N) place(j + WC, nx);
   }
-}
-
-// ---------------------------------------------------------------------------
-// Solve-ready copy of the factors (see fw_block / bw_block) and, per
-// (pair, matrix), the widest superdiagonal of U holding a non-zero and the
-// largest pivot offset.  The fused step kernel skips the superdiagonals that
-// are exactly zero (x - t*0 = x) and multiplies by r = 1/u0 (correctly
-// rounded) instead of dividing by the diagonal.
-// grid: x = pairs/64, y = 4 matrices, z = slot.
-// ---------------------------------------------------------------------------
-__global__ void k_lu_pack(Geo G, Bufs B, int slot0, int gated) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int mat = blockIdx.y;
-  const int slot = slot_of(B, slot0);
-  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
-  const int N = G.N, P = G.P;
-  const double* F = lu_block(B, slot, mat, p, P, N);
-  const unsigned char* pv = pv_block(B, slot, mat, p, P, N);
-  double2* fw = fw_block(B, slot, mat, p, P, N);
-  double2* bw = bw_block(B, slot, mat, p, P, N);
-  int kue = 1, mo = 0;
-  for (int j = 0; j < N; ++j) {
-    const double* c = F + (size_t)j * LUS;
-    fw[j * FW_Q + 0] = make_double2((double)pv[j], c[9]);
-    fw[j * FW_Q + 1] = make_double2(c[10], c[11]);
-    fw[j * FW_Q + 2] = make_double2(c[12], 0.0);
-    const double u0 = c[2 * KL];
-    bw[j] = make_double2(u0, u0 != 0.0 ? 1.0 / u0 : 0.0);
-    for (int q = 1; q < BW_Q; ++q) bw[(size_t)q * N + j] = make_double2(c[2 * KL - (2 * q - 1)], c[2 * KL - 2 * q]);
-    for (int d = 1; d <= 2 * KL; ++d)
-      if (j - d >= 0 && c[2 * KL - d] != 0.0) kue = d > kue ? d : kue;
-    mo = pv[j] > mo ? pv[j] : mo;
-  }
-  unsigned char* mt = meta_block(B, slot, mat, p, P);
-  mt[0] = (unsigned char)kue;
-  mt[1] = (unsigned char)mo;
+  finish();
 }
 
 // ---------------------------------------------------------------------------
@@ -421,53 +455,165 @@ __global__ void k_norm_sum(Geo G, const double* __restrict__ b, double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// lu_solve<double> for one lane whose vector lives in LDS (element n at
+// X[n * 64], the workgroup's lanes interleaved) and whose factor columns and
+// pivot offsets are read D columns ahead into a register ring, so a solve
+// waits on LDS rather than on a cache round trip per column.  The
+// operations and their order are lu_solve's.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned char* pv, double* X, int N) {
+  double w[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i] = i < N ? X[i * 64] : 0.0;
+  double fl[D][KL];
+  int pr[D];
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (u < N - 1) {
+#pragma unroll
+      for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)u * LUS + 2 * KL + 1 + i];
+      pr[u] = pv[u];
+    }
+  for (int j0 = 0; j0 < N - 1; j0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 + u;
+      if (j < N - 1) {
+        const int off = pr[u];
+        double a[KL];
+#pragma unroll
+        for (int i = 0; i < KL; ++i) a[i] = fl[u][i];
+        if (j + D < N - 1) {
+#pragma unroll
+          for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)(j + D) * LUS + 2 * KL + 1 + i];
+          pr[u] = pv[j + D];
+        }
+        const double a0 = w[0];
+        double s = w[0];
+        s = off == 1 ? w[1] : s;
+        s = off == 2 ? w[2] : s;
+        s = off == 3 ? w[3] : s;
+        s = off == 4 ? w[4] : s;
+        w[0] = s;
+        w[1] = off == 1 ? a0 : w[1];
+        w[2] = off == 2 ? a0 : w[2];
+        w[3] = off == 3 ? a0 : w[3];
+        w[4] = off == 4 ? a0 : w[4];
+        const double t = -w[0];
+        const int lm = N - 1 - j < KL ? N - 1 - j : KL;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i)
+          if (i <= lm) w[i] = w[i] + a[i - 1] * t;
+        X[j * 64] = w[0];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = w[i + 1];
+        w[4] = j + 5 < N ? X[(j + 5) * 64] : 0.0;
+      }
+    }
+  }
+  double v[9];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = N - 9 + q;
+    v[q] = idx >= 0 ? X[idx * 64] : 0.0;
+  }
+  v[8] = w[0];
+  double fu[D][2 * KL + 1];
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (N - 1 - u >= 0) {
+#pragma unroll
+      for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)(N - 1 - u) * LUS + r];
+    }
+  for (int j0 = N - 1; j0 >= 0; j0 -= D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 - u;
+      if (j >= 0) {
+        double f[2 * KL + 1];
+#pragma unroll
+        for (int r = 0; r <= 2 * KL; ++r) f[r] = fu[u][r];
+        if (j - D >= 0) {
+#pragma unroll
+          for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)(j - D) * LUS + r];
+        }
+        v[8] = v[8] / f[8];
+        const double t = v[8];
+        X[j * 64] = t;
+#pragma unroll
+        for (int d = 1; d <= 8; ++d)
+          if (j - d >= 0) v[8 - d] = v[8 - d] - t * f[8 - d];
+#pragma unroll
+        for (int q = 8; q >= 1; --q) v[q] = v[q - 1];
+        v[0] = j - 9 >= 0 ? X[(j - 9) * 64] : 0.0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // vel_adjPPE(0) influence vectors (velocity.f90:191-218) per pair, and
 // vel_adjustFlux(0) (velocity.f90:160-168) on pair 0.
 // Uinf slots: 0 U1 (LDp), 1 U2 (LDm), 2 U3 (LDz), 3 U4, 4 U5, 5 p (U6 = p*alpha*k).
+// The solves are independent, so each gets its own lane: grid y = 0..2 the
+// three Helmholtz solves, 3 the pressure solve and its gradient, 4 the flux
+// solve (lane of pair 0 only).  grid: x = pairs/64, y = 5, z = slot.
 // ---------------------------------------------------------------------------
-__global__ void k_influence(Geo G, Bufs B, int slot0, int gated) {
+__global__ void __launch_bounds__(64) k_influence(Geo G, Bufs B, int slot0, int gated) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int which = blockIdx.y;
   const int slot = slot_of(B, slot0);
-  if (p >= G.P || (gated && !B.ctl[slot].refactor)) return;
+  if (p >= G.P || (gated && !B.ctl[slot].refactor) || (which == 4 && p != 0)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;
   const MeshOff mo = MeshOff::make(N);
   const double* me = G.mesh;
   double* U = B.Uinf + (size_t)slot * 6 * N * P + p;
+  __shared__ double Xs[MAXN * 64];
+  double* x = Xs + threadIdx.x;
   auto LUp = [&](int mat) { return (const double*)lu_block(B, slot, mat, p, P, N); };
   auto PVp = [&](int mat) { return (const unsigned char*)pv_block(B, slot, mat, p, P, N); };
-  for (int j = 0; j < 3; ++j) {
-    double* x = U + (size_t)j * N * P;
-    for (int n = 0; n < N; ++n) x[(size_t)n * P] = n == N - 1 ? 1.0 : 0.0;
-    lu_solve<double>(LUp(j), PVp(j), x, P, N);
-  }
-  double* pp = U + (size_t)5 * N * P;
-  if (k == 0 && m == 0) {
-    for (int n = 0; n < N; ++n) pp[(size_t)n * P] = 0.0;
-  } else {
-    for (int n = 0; n < N; ++n) pp[(size_t)n * P] = n == N - 1 ? -1.0 : 0.0;
-    lu_solve<double>(LUp(MAT_LNP), PVp(MAT_LNP), pp, P, N);
-  }
-  double* U4 = U + (size_t)3 * N * P;
-  double* U5 = U + (size_t)4 * N * P;
+  // right-hand side and factor of this lane's solve (one inlined solve for
+  // every case, so the vector stays in LDS and the factor loads in flight)
+  const SlotParams prm = B.prm[slot];
+  int mat;
+  bool solve = true;
+  if (which < 3) mat = which;
+  else if (which == 3) { mat = MAT_LNP; solve = !(k == 0 && m == 0); }
+  else { mat = MAT_LDZ; solve = prm.const_flux != 0; }
   for (int n = 0; n < N; ++n) {
-    // grad: dr1 . p (meshmult from 0), t.Im = p * m/r; rt2pm -> U4 = gr - tIm, U5 = gr + tIm
-    double gr = 0.0;
-    const int j0 = n - KL < 0 ? 0 : n - KL, j1 = n + KL > N - 1 ? N - 1 : n + KL;
-    for (int j = j0; j <= j1; ++j) gr = gr + me[mo.dr1b + (KL + n - j) * N + j] * pp[(size_t)j * P];
-    const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
-    const double tIm = pp[(size_t)n * P] * mr;
-    U4[(size_t)n * P] = gr - tIm;
-    U5[(size_t)n * P] = gr + tIm;
+    double v;
+    if (which < 3) v = n == N - 1 ? 1.0 : 0.0;
+    else if (which == 3) v = solve && n == N - 1 ? -1.0 : 0.0;
+    else v = (n == 0 || n == N - 1) ? 0.0 : 1.0;
+    x[n * 64] = v;
   }
-  if (p == 0) {
-    const SlotParams prm = B.prm[slot];
+  if (solve) lu_solve_lds<8>(LUp(mat), PVp(mat), x, N);
+  if (which < 3) {
+    double* u = U + (size_t)which * N * P;
+    for (int n = 0; n < N; ++n) u[(size_t)n * P] = x[n * 64];
+  } else if (which == 3) {
+    double* pp = U + (size_t)5 * N * P;
+    double* U4 = U + (size_t)3 * N * P;
+    double* U5 = U + (size_t)4 * N * P;
+    for (int n = 0; n < N; ++n) {
+      // grad: dr1 . p (meshmult from 0), t.Im = p * m/r; rt2pm -> U4 = gr - tIm, U5 = gr + tIm
+      double gr = 0.0;
+      const int j0 = n - KL < 0 ? 0 : n - KL, j1 = n + KL > N - 1 ? N - 1 : n + KL;
+      for (int j = j0; j <= j1; ++j) gr = gr + me[mo.dr1b + (KL + n - j) * N + j] * x[j * 64];
+      const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
+      const double tIm = x[n * 64] * mr;
+      pp[(size_t)n * P] = x[n * 64];
+      U4[(size_t)n * P] = gr - tIm;
+      U5[(size_t)n * P] = gr + tIm;
+    }
+  } else {
     double* Ui = B.Ui + (size_t)slot * N;
-    for (int n = 0; n < N; ++n) Ui[n] = (n == 0 || n == N - 1) ? 0.0 : 1.0;
+    for (int n = 0; n < N; ++n) Ui[n] = x[n * 64];
     if (prm.const_flux) {
-      lu_solve<double>(LUp(MAT_LDZ), PVp(MAT_LDZ), Ui, 1, N);
       double s = 0.0;
-      for (int n = 0; n < N; ++n) s = s + Ui[n] * me[mo.intrdr + n];
+      for (int n = 0; n < N; ++n) s = s + x[n * 64] * me[mo.intrdr + n];
       B.ctl[slot].flux_d1 = 2.0 * s;
       if (2.0 * s == 0.0) atomicMax(&B.ctl[slot].status, 110);
     }
@@ -521,8 +667,8 @@ __global__ void k_ainv(Geo G, Bufs B, int slot0, int gated) {
 // ---------------------------------------------------------------------------
 // curl (var_coll_curl, variables.f90:408-436) of the current iterate -> om
 // ---------------------------------------------------------------------------
-__device__ void curl_lane(const Geo& G, int h, int k, int m, double alpha,
-                          const double2* ur, const double2* ut, const double2* uz, double2* om) {
+__device__ __forceinline__ void curl_point(const Geo& G, int h, int n, int k, int m, double alpha,
+                                           const double2* ur, const double2* ut, const double2* uz, double2* om) {
   const int N = G.N;
   const size_t hs = G.Hp;
   const MeshOff mo = MeshOff::make(N);
@@ -531,22 +677,22 @@ __device__ void curl_lane(const Geo& G, int h, int k, int m, double alpha,
   double2* orr = om;
   double2* ot = om + G.coll;
   double2* oz = om + 2 * G.coll;
-  for (int n = 0; n < N; ++n) {
-    const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
-    const size_t o = (size_t)n * hs + h;
-    const double2 vr = ur[o], vt = ut[o], vz = uz[o];
-    orr[o] = make_double2(-(vz.y * mr) + vt.y * ak, vz.x * mr - vt.x * ak);
-    const double2 d1 = band_row(me + mo.dr1b, N, n, uz + h, hs, make_double2(0.0, 0.0));
-    ot[o] = make_double2(-(vr.y * ak) - d1.x, vr.x * ak - d1.y);
-    const double2 d2 = band_row(me + mo.dr1b, N, n, ut + h, hs, make_double2(0.0, 0.0));
-    const double ri = me[mo.rinv + n];
-    oz[o] = make_double2((vt.x * ri + d2.x) + vr.y * mr, (vt.y * ri + d2.y) - vr.x * mr);
-  }
+  const double mr = (me[mo.rinv + n] * (double)G.Mp) * (double)m;
+  const size_t o = (size_t)n * hs + h;
+  const double2 vr = ur[o], vt = ut[o], vz = uz[o];
+  orr[o] = make_double2(-(vz.y * mr) + vt.y * ak, vz.x * mr - vt.x * ak);
+  const double2 d1 = band_row(me + mo.dr1b, N, n, uz + h, hs, make_double2(0.0, 0.0));
+  ot[o] = make_double2(-(vr.y * ak) - d1.x, vr.x * ak - d1.y);
+  const double2 d2 = band_row(me + mo.dr1b, N, n, ut + h, hs, make_double2(0.0, 0.0));
+  const double ri = me[mo.rinv + n];
+  oz[o] = make_double2((vt.x * ri + d2.x) + vr.y * mr, (vt.y * ri + d2.y) - vr.x * mr);
 }
 
 // which = 0: curl of U[ucur]; used after put_state and after every step.
+// One thread per (h, n): grid x = harmonics/64, y = N, z = slot.
 __global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
   const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active || h >= G.H) return;
@@ -554,8 +700,8 @@ __global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
   int k, m;
   harmonic_km(h, G.K, k, m);
   const double2* u = B.U[c->ucur] + (size_t)slot * 3 * G.coll;
-  curl_lane(G, h, k, m, B.prm[slot].alpha, u, u + G.coll, u + 2 * G.coll,
-            B.om + (size_t)slot * 3 * G.coll);
+  curl_point(G, h, n, k, m, B.prm[slot].alpha, u, u + G.coll, u + 2 * G.coll,
+             B.om + (size_t)slot * 3 * G.coll);
 }
 
 // vel_imposesym (velocity.f90:61-74) with the slot's symmetries on U[ucur]:
