@@ -133,21 +133,24 @@ __device__ __forceinline__ void bfly(double2 (&v)[R], double sgn, const double2*
   }
 }
 
-// LDS layout of a batch of transforms of length L: element i of transform b
-// at buf[b*Lp + pidx(i)], pidx(i) = i + i/16, Lp = padded_len(L).  The pad
-// spreads the stride-R writes of the first Stockham pass and the strided
-// column fills over the LDS banks.
+// LDS layout of a batch of transforms of length L: a transform's input,
+// element i of transform b, at buf[b*Lp + i] (linear), its output at
+// buf[b*Lp + pidx(i)], pidx(i) = i + i/16, Lp = padded_len(L).  The pad
+// spreads the stride-R writes of the first Stockham pass over the LDS
+// banks; the first pass reads the linear input, whose stride-L/R reads
+// fall on consecutive addresses across a wave, with constant offsets.
 __host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
 // One Stockham pass of radix R over B transforms of length L stored at
-// buf[b*Lp + pidx(i)] (B*L <= CAP).  Ns = product of the radices of the
+// buf[b*Lp + pidx(i)] (B*L <= CAP; LIN: the first pass, reading the linear
+// input layout buf[b*Lp + i]).  Ns = product of the radices of the
 // previous passes (the twiddles of the first pass are all 1 and skipped);
 // tw: the length-L twiddle table in LDS, twg: the same table in global
 // memory, read with uniform indices for the butterflies' internal twiddles.
 // Outer twiddles are fetched four at a time to bound register use.
 // LC, NSC: the length and Ns at compile time (0: the run-time L_, Ns_), so
 // that the index divisions become multiplications by constants.
-template <int R, int LC, int NSC, int CAP = FFT_TILE>
+template <int R, int LC, int NSC, int CAP = FFT_TILE, bool LIN = false>
 __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, double sgn, const double2* tw,
                                          const double2* twg) {
   constexpr int MB = (CAP + R * FFT_THREADS - 1) / (R * FFT_THREADS);
@@ -158,6 +161,14 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
   const int total = nb * B;
   const int tid = threadIdx.x;
   const int stride_tw = L / (Ns * R);
+  // Where a stride is a compile-time multiple of the 16-element pad period,
+  // pidx(a + r*s) = pidx(a) + r*(s + s/16): one address per thread and
+  // constant offsets per element instead of a padded index per element.
+  // The same holds for the writes of a first pass whose radix divides 16
+  // (each thread's R outputs lie inside one period).
+  constexpr int NBC = LC ? LC / R : 0;
+  constexpr bool RSEP = LC && NBC % 16 == 0;
+  constexpr bool WSEP = LC && (NSC % 16 == 0 || (NSC == 1 && 16 % R == 0));
   double2 v[MB][R];
   int dst[MB];
 #pragma unroll
@@ -168,8 +179,18 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
       const int b = q / nb, j = q - b * nb;
       const double2* x = buf + b * Lp;
       const int kk = j % Ns;
+      if constexpr (LIN) {
+        const double2* xr = x + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = x[pidx(j + r * nb)];
+        for (int r = 0; r < R; ++r) v[i][r] = xr[r * nb];
+      } else if constexpr (RSEP) {
+        const double2* xr = x + pidx(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = xr[r * (NBC + NBC / 16)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = x[pidx(j + r * nb)];
+      }
       if (Ns > 1) {
 #pragma unroll
         for (int g = 1; g < R; g += 4) {
@@ -196,8 +217,14 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
       const int b = q / nb;
       const int e = dst[i] - b * Lp;          // unpadded index of output 0
       double2* x = buf + b * Lp;
+      if constexpr (WSEP) {
+        double2* xw = x + pidx(e);
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[pidx(e + r * Ns)] = v[i][perm<R>(r)];
+        for (int r = 0; r < R; ++r) xw[r * (NSC + NSC / 16)] = v[i][perm<R>(r)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[pidx(e + r * Ns)] = v[i][perm<R>(r)];
+      }
     }
   }
   __syncthreads();
@@ -207,18 +234,23 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
 template <int CAP = FFT_TILE>
 __device__ __forceinline__ void fft_lds(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
-  int Ns = 1;
-  for (int p = 0; p < pl.nrad; ++p) {
-    switch (pl.rad[p]) {
-      case 16: fft_pass<16, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 12: fft_pass<12, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 9: fft_pass<9, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 8: fft_pass<8, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 6: fft_pass<6, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 4: fft_pass<4, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      case 3: fft_pass<3, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
-      default: fft_pass<2, 0, 0, CAP>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+  auto pass = [&](auto lin, int R, int Ns) {
+    constexpr bool LN = decltype(lin)::value;
+    switch (R) {
+      case 16: fft_pass<16, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 12: fft_pass<12, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 9: fft_pass<9, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 8: fft_pass<8, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 6: fft_pass<6, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 4: fft_pass<4, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      case 3: fft_pass<3, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
+      default: fft_pass<2, 0, 0, CAP, LN>(buf, pl.L, B, Ns, sgn, tw, twg); break;
     }
+  };
+  pass(std::true_type{}, pl.rad[0], 1);
+  int Ns = pl.rad[0];
+  for (int p = 1; p < pl.nrad; ++p) {
+    pass(std::false_type{}, pl.rad[p], Ns);
     Ns *= pl.rad[p];
   }
 }
@@ -240,7 +272,7 @@ __device__ __forceinline__ void run_fft(double2* buf, int B, const FftPlan& pl, 
     fft_lds<CAP>(buf, B, pl, sgn, tw, twg);
   } else {
     constexpr int L = plan_len<R0, R1, R2>();
-    fft_pass<R0, L, 1, CAP>(buf, L, B, 1, sgn, tw, twg);
+    fft_pass<R0, L, 1, CAP, true>(buf, L, B, 1, sgn, tw, twg);
     if constexpr (R1 > 1) fft_pass<R1, L, R0, CAP>(buf, L, B, R0, sgn, tw, twg);
     if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1, CAP>(buf, L, B, R0 * R1, sgn, tw, twg);
   }
@@ -281,7 +313,7 @@ __device__ __forceinline__ void load_zcols(double2* buf, const double2* row, int
     const int b = ic / nk, e = ic - b * nk;
     const int k = e - (K - 1);
     v[u] = coef(row, K, k, m0 + b);
-    d[u] = ok ? b * Zp + pidx(k >= 0 ? k : k + Z) : -1;
+    d[u] = ok ? b * Zp + (k >= 0 ? k : k + Z) : -1;
   }
 #pragma unroll
   for (int u = 0; u < ZLD; ++u)
@@ -289,11 +321,11 @@ __device__ __forceinline__ void load_zcols(double2* buf, const double2* row, int
   const int nzero = K + 1;   // j = K .. 2K
   for (int i = threadIdx.x; i < tm * nzero; i += FFT_THREADS) {
     const int b = i / nzero, j = K + (i - b * nzero);
-    buf[b * Zp + pidx(j)] = make_double2(0.0, 0.0);
+    buf[b * Zp + j] = make_double2(0.0, 0.0);
   }
 }
 
-// column b of the tile (padded layout) = Y[z*ystride + m0 + b], one batch.
+// column b of the tile (linear input layout) = Y[z*ystride + m0 + b], one batch.
 // TMC: the full tile width at compile time (0: use tm), items of columns
 // b >= tm are skipped.
 template <int TMC>
@@ -309,7 +341,7 @@ __device__ __forceinline__ void load_ycols(double2* buf, const double2* Y, size_
     const int z = ic / tw, b = ic - z * tw;
     const bool ok = i < total && b < tm;
     v[u] = Y[(size_t)z * ystride + m0 + (ok ? b : 0)];
-    d[u] = ok ? b * Zp + pidx(z) : -1;
+    d[u] = ok ? b * Zp + z : -1;
   }
 #pragma unroll
   for (int u = 0; u < ZLD; ++u)
@@ -425,15 +457,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     if (mm == 0) {
       L[0] = make_double2(A[u].x, Bv[u].x);
     } else {
-      L[pidx(mm)] = make_double2(A[u].x - Bv[u].y, A[u].y + Bv[u].x);
-      L[pidx(TT - mm)] = make_double2(A[u].x + Bv[u].y, -A[u].y + Bv[u].x);
+      L[mm] = make_double2(A[u].x - Bv[u].y, A[u].y + Bv[u].x);
+      L[TT - mm] = make_double2(A[u].x + Bv[u].y, -A[u].y + Bv[u].x);
     }
   }
   const int nzero = TT - 2 * M + 1;   // bins M .. T-M
   for (int i = threadIdx.x; i < 3 * tz * nzero; i += FFT_THREADS) {
     const int pl = i / nzero, mm = M + (i - pl * nzero);
     const int pr = pl / tz, line = pl - pr * tz;
-    buf[((size_t)pr * tz + line) * Tp + pidx(mm)] = make_double2(0.0, 0.0);
+    buf[((size_t)pr * tz + line) * Tp + mm] = make_double2(0.0, 0.0);
   }
   __syncthreads();
   run_fft<R0, R1, R2, 3 * THETA_ITEMS>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
@@ -446,13 +478,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   const bool cor = B.prm[slot].coriolis != 0;
   const double cor_d1 = B.prm[slot].cor_d1, cor_s = B.prm[slot].cor_sphi, cor_c = B.prm[slot].cor_cphi;
   double mr = 0.0, mt = 0.0, mz = 0.0;
-  double nzv[TLD];   // N_z of this thread's items, written packed after a barrier
+  // N of this thread's items, written after a barrier as the forward
+  // transforms' linear input (it overlaps the inverse transforms' output)
+  double2 nrt[TLD];
+  double nzv[TLD];
 #pragma unroll
   for (int u = 0; u < TLD; ++u) {
     const int i = threadIdx.x + u * FFT_THREADS;
     if (i >= tz * TT) continue;
     const int line = i / TT, th = i - line * TT;
-    double2* L0 = buf + (size_t)line * Tp + pidx(th);
+    const double2* L0 = buf + (size_t)line * Tp + pidx(th);
     const double2 w0 = L0[0], w1 = L0[(size_t)tz * Tp], w2 = L0[(size_t)2 * tz * Tp];
     const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
     mr = fmax(mr, fabs(ur));
@@ -469,7 +504,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
       Nt += cor_d1 * (cor_c * ur - Or * uzU);
       Nz += cor_d1 * (Or * ut - Ot * ur);
     }
-    L0[0] = make_double2(scale * Nr, scale * Nt);
+    nrt[u] = make_double2(scale * Nr, scale * Nt);
     nzv[u] = scale * Nz;
   }
   // maxima are exact and order free: non-negative doubles order like their
@@ -497,16 +532,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     }
   }
   __syncthreads();
-  // N_z of lines 2q and 2q + 1 as the real and imaginary parts of transform
-  // tz + q (two real transforms in one complex one; an odd last line pairs
-  // with zero)
+  // (N_r, N_t) of each line as transform `line`; N_z of lines 2q and 2q + 1
+  // as the real and imaginary parts of transform tz + q (two real
+  // transforms in one complex one; an odd last line pairs with zero)
   const int nzt = (tz + 1) / 2;
 #pragma unroll
   for (int u = 0; u < TLD; ++u) {
     const int i = threadIdx.x + u * FFT_THREADS;
     if (i >= tz * TT) continue;
     const int line = i / TT, th = i - line * TT;
-    double* d = (double*)(buf + ((size_t)tz + (line >> 1)) * Tp + pidx(th));
+    buf[(size_t)line * Tp + th] = nrt[u];
+    double* d = (double*)(buf + ((size_t)tz + (line >> 1)) * Tp + th);
     d[line & 1] = nzv[u];
     if ((line & 1) == 0 && line + 1 == tz) d[1] = 0.0;
   }
@@ -612,7 +648,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_c2r(Geo G, Bufs B, FftTil
       const double2 a = A[q], b = two ? Bp[q] : make_double2(0.0, 0.0);
       v = make_double2(a.x + b.y, -a.y + b.x);
     }
-    buf[pidx(mm)] = v;
+    buf[mm] = v;
   }
   __syncthreads();
   fft_lds(buf, 1, T.pt, 1.0, twl, B.twT);
@@ -658,7 +694,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_lines(Geo G, Bufs B, FftTil
       const double2 a = A[q], c = two ? Bp[q] : make_double2(0.0, 0.0);
       v = make_double2(a.x + c.y, -a.y + c.x);
     }
-    buf[(size_t)b * Tp + pidx(mm)] = v;
+    buf[(size_t)b * Tp + mm] = v;
   }
   __syncthreads();
   fft_lds(buf, nt, T.pt, 1.0, twl, B.twT);
@@ -691,7 +727,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_r2c_lines(Geo G, Bufs B, FftTil
     const int b = i / TT, th = i - b * TT;
     const int z = z0 + 2 * b;
     const double* in = phys + ((size_t)n * Z + z) * TT + th;
-    buf[(size_t)b * Tp + pidx(th)] = make_double2(scale * in[0], z + 1 < Z ? scale * in[TT] : 0.0);
+    buf[(size_t)b * Tp + th] = make_double2(scale * in[0], z + 1 < Z ? scale * in[TT] : 0.0);
   }
   __syncthreads();
   fft_lds(buf, nt, T.pt, -1.0, twl, B.twT);
@@ -717,7 +753,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_r2c(Geo G, Bufs B, FftTil
   load_tw(twl, B.twT, TT);
   const double* in = phys + (size_t)n * TT * G.Z + z0;
   for (int th = threadIdx.x; th < TT; th += FFT_THREADS)
-    buf[pidx(th)] = make_double2(scale * in[(size_t)th * G.Z], two ? scale * in[(size_t)th * G.Z + 1] : 0.0);
+    buf[th] = make_double2(scale * in[(size_t)th * G.Z], two ? scale * in[(size_t)th * G.Z + 1] : 0.0);
   __syncthreads();
   fft_lds(buf, 1, T.pt, -1.0, twl, B.twT);
   double2* A = Y + ((size_t)n * G.Z + z0) * M;
