@@ -54,6 +54,8 @@ def alg_bytes(N, K, M):
     F = 4 * P * (8 * 13 * N + 4 * N) + H * (48 * N + 128)
     return dict(C=C, F=F,
                 zfwd=6 * C + 6 * Y, theta=9 * Y, zbwd=3 * Y + 3 * C,
+                # the whole-plane evaluation of small grids: u, curl u in, N out
+                plane=9 * C,
                 # predictor: read N and u_, write u and curl u; each corrector
                 # also reads N_ and the previous iterate
                 step_pred=12 * C + F, step_corr=18 * C + F, curl=6 * C)
@@ -560,7 +562,7 @@ def main():
     ctx.kernel_timing(True)
     ctx.run(args.timing_steps)
     kt = {}
-    for k in ("zfwd", "theta", "zbwd", "step", "curl", "control"):
+    for k in ("zfwd", "theta", "zbwd", "plane", "step", "curl", "control"):
         ms, n = ctx.kernel_time(k)
         if n:
             kt[k] = dict(ms_total=ms, launches=n, us_avg=1000.0 * ms / n)

@@ -88,6 +88,8 @@ struct opf_ctx {
   StepTiles st{};
   size_t smem_z = 0, smem_t = 0;
   FftKerns fk{};
+  PlaneKern plane = nullptr;   // whole-plane evaluation, where the grid has one
+  size_t smem_plane = 0;
   std::vector<void*> allocs;
   size_t bytes = 0;
   std::vector<int> occupied;   // slot holds a state
@@ -345,6 +347,10 @@ unsigned ztile_blocks(const opf_ctx* c, int nfield) {
 // terms) for slots [s0, s0+ns).
 void launch_eval(opf_ctx* c, int s0, int ns) {
   const FftTiles& t = c->tiles;
+  if (c->plane) {   // one workgroup per radial plane, the intermediates in LDS
+    launch(c, "plane", c->plane, dim3(c->N, 1, ns), dim3(PLANE_THREADS), c->smem_plane, c->G, c->B, t, s0);
+    return;
+  }
   launch(c, "zfwd", c->fk.zfwd, dim3(ztile_blocks(c, 6), 1, ns), dim3(FFT_THREADS), c->smem_z, c->G, c->B, t,
          s0);
   launch(c, "theta", c->fk.theta, dim3((c->Z + t.tz - 1) / t.tz, c->N, ns), dim3(FFT_THREADS), c->smem_t,
@@ -615,6 +621,14 @@ void create(opf_ctx* c, int N, int K, int M, int Mp, int cap, const double* r, i
     if (c->st.smem > 64 * 1024)   // dynamic LDS beyond 64 KiB is requested per kernel
       HIPCHK(hipFuncSetAttribute((const void*)step_kernel(c), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)c->st.smem));
+    // the whole-plane evaluation where the grid has one and its LDS fits
+    // (OPF_PLANE=0: the three transform kernels instead)
+    const char* e2 = std::getenv("OPF_PLANE");
+    c->plane = plane_kernel(c->tiles.pz, c->tiles.pt, &c->smem_plane);
+    if (c->plane && ((e2 && std::atoi(e2) == 0) || c->smem_plane > (size_t)lds)) c->plane = nullptr;
+    if (c->plane && c->smem_plane > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute((const void*)c->plane, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)c->smem_plane));
   }
   c->smem_z = ((size_t)c->tiles.tm * padded_len(c->Z) + c->Z) * sizeof(double2);       // data + twiddles
   c->smem_t = ((size_t)3 * c->tiles.tz * padded_len(c->T) + c->T) * sizeof(double2);

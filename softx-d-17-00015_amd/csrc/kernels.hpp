@@ -103,7 +103,7 @@ constexpr int THETA_ITEMS = ((FFT_TILE / 3 + FFT_THREADS - 1) / FFT_THREADS) * F
 // intermediates measured slower)
 __host__ __device__ constexpr int tile_m(int Z) { return FFT_TILE / Z; }
 
-__host__ __device__ inline int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
+__host__ __device__ constexpr int padded_len(int L) { return (L + (L - 1) / 16 + 1) | 1; }
 
 // The matrix tim_lumesh_init builds: mat < 0 the four of vel_matrices
 // (LDp, LDm, LDz with c1 = 1/dt, c2 = -implicit nu/Re; LNp with 0, 1, BC = 1);
@@ -162,6 +162,12 @@ struct FftKerns {
   FftKern zfwd, theta, zbwd;
 };
 FftKerns fft_kernels(const FftPlan& pz, const FftPlan& pt);
+// whole-plane evaluation (z, theta, products, theta, z in one workgroup per
+// radial plane) for the plan pairs that have one; nullptr otherwise.
+// smem: its dynamic LDS bytes.
+constexpr int PLANE_THREADS = 1024;
+using PlaneKern = void (*)(Geo, Bufs, FftTiles, int);
+PlaneKern plane_kernel(const FftPlan& pz, const FftPlan& pt, size_t* smem);
 // materialise physical fields / transform arbitrary coll fields (granular API)
 __global__ void k_zfwd_one(Geo G, Bufs B, FftTiles T, const double2* a, double2* Y, size_t fa, size_t fy);
 __global__ void k_theta_c2r(Geo G, Bufs B, FftTiles T, const double2* Y, double* phys);

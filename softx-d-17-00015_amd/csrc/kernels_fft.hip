@@ -150,10 +150,10 @@ __host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 // Outer twiddles are fetched four at a time to bound register use.
 // LC, NSC: the length and Ns at compile time (0: the run-time L_, Ns_), so
 // that the index divisions become multiplications by constants.
-template <int R, int LC, int NSC, int CAP = FFT_TILE, bool LIN = false>
+template <int R, int LC, int NSC, int CAP = FFT_TILE, bool LIN = false, int NT = FFT_THREADS>
 __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, double sgn, const double2* tw,
                                          const double2* twg) {
-  constexpr int MB = (CAP + R * FFT_THREADS - 1) / (R * FFT_THREADS);
+  constexpr int MB = (CAP + R * NT - 1) / (R * NT);
   const int L = LC ? LC : L_;
   const int Ns = LC ? NSC : Ns_;
   const int Lp = padded_len(L);
@@ -173,7 +173,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
   int dst[MB];
 #pragma unroll
   for (int i = 0; i < MB; ++i) {
-    const int q = tid + i * FFT_THREADS;
+    const int q = tid + i * NT;
     dst[i] = -1;
     if (q < total) {
       const int b = q / nb, j = q - b * nb;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, int L_, int B, int Ns_, d
 #pragma unroll
   for (int i = 0; i < MB; ++i) {
     if (dst[i] >= 0) {
-      const int q = tid + i * FFT_THREADS;
+      const int q = tid + i * NT;
       const int b = q / nb;
       const int e = dst[i] - b * Lp;          // unpadded index of output 0
       double2* x = buf + b * Lp;
@@ -264,17 +264,19 @@ constexpr int plan_len() {
 // Transform with the radix sequence fixed at compile time (R0 = 0: the
 // plan's own sequence).  Each pass then keeps only its own registers live,
 // and lengths and strides are constants.  CAP: the most values (B * L) a
-// call transforms.
-template <int R0, int R1, int R2, int CAP = FFT_TILE>
+// call transforms; NT: the workgroup's threads (the generic path takes
+// FFT_THREADS).
+template <int R0, int R1, int R2, int CAP = FFT_TILE, int NT = FFT_THREADS>
 __device__ __forceinline__ void run_fft(double2* buf, int B, const FftPlan& pl, double sgn, const double2* tw,
                                         const double2* twg) {
   if constexpr (R0 == 0) {
+    static_assert(NT == FFT_THREADS, "the generic transform runs on FFT_THREADS threads");
     fft_lds<CAP>(buf, B, pl, sgn, tw, twg);
   } else {
     constexpr int L = plan_len<R0, R1, R2>();
-    fft_pass<R0, L, 1, CAP, true>(buf, L, B, 1, sgn, tw, twg);
-    if constexpr (R1 > 1) fft_pass<R1, L, R0, CAP>(buf, L, B, R0, sgn, tw, twg);
-    if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1, CAP>(buf, L, B, R0 * R1, sgn, tw, twg);
+    fft_pass<R0, L, 1, CAP, true, NT>(buf, L, B, 1, sgn, tw, twg);
+    if constexpr (R1 > 1) fft_pass<R1, L, R0, CAP, false, NT>(buf, L, B, R0, sgn, tw, twg);
+    if constexpr (R2 > 1) fft_pass<R2, L, R0 * R1, CAP, false, NT>(buf, L, B, R0 * R1, sgn, tw, twg);
   }
 }
 
@@ -414,6 +416,47 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zfwd_t(Geo G, Bufs B, FftTiles 
 // then T twiddles.
 // ---------------------------------------------------------------------------
 constexpr int TLD = THETA_ITEMS / FFT_THREADS;
+
+// products (velocity.f90:325-329) at one physical point of radial plane n,
+// scaled as tra_phys2spec (:114-117), and the running maxima of |u_r|,
+// |u_t|, |u_z + 1 - r^2| for vel_maxtstep (velocity.f90:464-475).  Inputs:
+// the packed transforms' values (u_r, u_t), (u_z, w_r), (w_t, w_z); outputs
+// (N_r, N_t) and N_z.  One definition for every kernel that forms them, so
+// they round alike.
+struct NlPlane {
+  double scale, ubase, cor_d1, cor_s, cor_c;
+  bool cor;
+  __device__ static NlPlane make(const Geo& G, const Bufs& B, int slot, int n, int TT) {
+    NlPlane p;
+    p.scale = 1.0 / (double)(G.Z * TT);
+    p.ubase = one_minus_sq(G.mesh[MeshOff::make(G.N).r + n]);
+    p.cor = B.prm[slot].coriolis != 0;
+    p.cor_d1 = B.prm[slot].cor_d1;
+    p.cor_s = B.prm[slot].cor_sphi;
+    p.cor_c = B.prm[slot].cor_cphi;
+    return p;
+  }
+  __device__ __forceinline__ void point(double2 w0, double2 w1, double2 w2, const double2* twl, int th, double& mr,
+                                        double& mt, double& mz, double2& nrt, double& nz) const {
+    const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
+    mr = fmax(mr, fabs(ur));
+    mt = fmax(mt, fabs(ut));
+    mz = fmax(mz, fabs(uz + ubase));
+    double Nr = ut * cz - uz * ct;
+    double Nt = uz * cr - ur * cz;
+    double Nz = ur * ct - ut * cr;
+    if (cor) {   // Coriolis.f90:62-68: N += -1/(4 Ro) Omega^ x (U + u)
+      const double2 cs = twl[th];   // (cos, sin)(2 pi th / T)
+      const double Or = cor_s * cs.x, Ot = -cor_s * cs.y;
+      const double uzU = uz + ubase;
+      Nr += cor_d1 * (Ot * uzU - cor_c * ut);
+      Nt += cor_d1 * (cor_c * ur - Or * uzU);
+      Nz += cor_d1 * (Or * ut - Ot * ur);
+    }
+    nrt = make_double2(scale * Nr, scale * Nt);
+    nz = scale * Nz;
+  }
+};
 template <int R0, int R1, int R2>
 __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles T, int slot0) {
   const int slot = slot_of(B, slot0);
@@ -469,14 +512,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
   }
   __syncthreads();
   run_fft<R0, R1, R2, 3 * THETA_ITEMS>(buf, 3 * tz, T.pt, 1.0, twl, B.twT);
-  // products (velocity.f90:325-329), scaled as tra_phys2spec (:114-117);
-  // the plane's max |u_r|, |u_t|, |u_z + 1 - r^2| for vel_maxtstep
-  // (velocity.f90:464-475)
-  const double scale = 1.0 / (double)(G.Z * TT);
-  const double rn = G.mesh[MeshOff::make(G.N).r + n];
-  const double ubase = one_minus_sq(rn);
-  const bool cor = B.prm[slot].coriolis != 0;
-  const double cor_d1 = B.prm[slot].cor_d1, cor_s = B.prm[slot].cor_sphi, cor_c = B.prm[slot].cor_cphi;
+  // products and the plane's velocity maxima
+  const NlPlane pl = NlPlane::make(G, B, slot, n, TT);
   double mr = 0.0, mt = 0.0, mz = 0.0;
   // N of this thread's items, written after a barrier as the forward
   // transforms' linear input (it overlaps the inverse transforms' output)
@@ -488,24 +525,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_theta_t(Geo G, Bufs B, FftTiles
     if (i >= tz * TT) continue;
     const int line = i / TT, th = i - line * TT;
     const double2* L0 = buf + (size_t)line * Tp + pidx(th);
-    const double2 w0 = L0[0], w1 = L0[(size_t)tz * Tp], w2 = L0[(size_t)2 * tz * Tp];
-    const double ur = w0.x, ut = w0.y, uz = w1.x, cr = w1.y, ct = w2.x, cz = w2.y;
-    mr = fmax(mr, fabs(ur));
-    mt = fmax(mt, fabs(ut));
-    mz = fmax(mz, fabs(uz + ubase));
-    double Nr = ut * cz - uz * ct;
-    double Nt = uz * cr - ur * cz;
-    double Nz = ur * ct - ut * cr;
-    if (cor) {   // Coriolis.f90:62-68: N += -1/(4 Ro) Omega^ x (U + u)
-      const double2 cs = twl[th];   // (cos, sin)(2 pi th / T)
-      const double Or = cor_s * cs.x, Ot = -cor_s * cs.y;
-      const double uzU = uz + ubase;
-      Nr += cor_d1 * (Ot * uzU - cor_c * ut);
-      Nt += cor_d1 * (cor_c * ur - Or * uzU);
-      Nz += cor_d1 * (Or * ut - Ot * ur);
-    }
-    nrt[u] = make_double2(scale * Nr, scale * Nt);
-    nzv[u] = scale * Nz;
+    pl.point(L0[0], L0[(size_t)tz * Tp], L0[(size_t)2 * tz * Tp], twl, th, mr, mt, mz, nrt[u], nzv[u]);
   }
   // maxima are exact and order free: non-negative doubles order like their
   // bit patterns, so one integer atomic max per workgroup and component.
@@ -597,6 +617,187 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zbwd_t(Geo G, Bufs B, FftTiles 
   run_fft<R0, R1, R2>(buf, tm, T.pz, -1.0, twl, B.twZ);
   double2* row = B.Nraw[c->ncur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll + (size_t)n * G.Hp;
   for (int b = 0; b < tm; ++b) store_zcol(row, buf + b * Zp, K, Z, m0 + b, threadIdx.x, FFT_THREADS);
+}
+
+// ---------------------------------------------------------------------------
+// A radial plane's whole evaluation in one workgroup, for grids whose six
+// z-transformed fields fit in LDS: the z transforms of u and curl u, the
+// theta transforms, the products and the inverse theta and z transforms, as
+// k_zfwd_t, k_theta_t and k_zbwd_t do them in turn (the same passes and
+// operations, so the same bits), with the intermediates Y kept in LDS
+// instead of making a round trip through HBM.  grid: x = N, z = slot.
+// LDS: the six fields' z columns (field f, mode m at (f M + m) Zp), then the
+// theta transforms of a chunk of PL z-lines (as k_theta_t's tile with tz =
+// PL), then the Z and T twiddles.  The nonlinear terms' z columns, in the
+// linear input layout, overwrite fields 0..2 chunk by chunk: a chunk's
+// entries at linear z lie at or below the padded entries of the chunks
+// still to come (pidx(z) >= z), and its own were read before.
+// ---------------------------------------------------------------------------
+template <int ZR0, int ZR1, int ZR2, int TR0, int TR1, int TR2, int PL>
+__global__ void __launch_bounds__(PLANE_THREADS) k_plane_t(Geo G, Bufs B, FftTiles T, int slot0) {
+  constexpr int NT = PLANE_THREADS;
+  constexpr int Z = plan_len<ZR0, ZR1, ZR2>(), TT = plan_len<TR0, TR1, TR2>();
+  constexpr int K = Z / 3, M = TT / 3, NK = 2 * K - 1;
+  constexpr int Zp = padded_len(Z), Tp = padded_len(TT);
+  static_assert(Z % PL == 0 && PL % 2 == 0, "whole, even chunks of z-lines");
+  const int slot = slot_of(B, slot0);
+  const SlotCtl* c = &B.ctl[slot];
+  if (!c->active) return;
+  const int n = blockIdx.x;
+  const int tid = threadIdx.x;
+  double2* A = smem;
+  double2* Lb = A + (size_t)6 * M * Zp;
+  double2* twz = Lb + (size_t)3 * PL * Tp;
+  double2* twt = twz + Z;
+  for (int i = tid; i < Z; i += NT) twz[i] = B.twZ[i];
+  for (int i = tid; i < TT; i += NT) twt[i] = B.twT[i];
+  // z columns of the six fields (load_zcols for every mode at once)
+  const double2* U = B.U[c->ucur] + (size_t)slot * 3 * G.coll + (size_t)n * G.Hp;
+  const double2* W = B.om + (size_t)slot * 3 * G.coll + (size_t)n * G.Hp;
+  {
+    constexpr int TOT = 6 * M * NK, IT = (TOT + NT - 1) / NT;
+    double2 v[IT];
+    int d[IT];
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int i = tid + u * NT;
+      const bool ok = i < TOT;
+      const int ic = ok ? i : 0;
+      const int f = ic / (M * NK), rem = ic - f * (M * NK);
+      const int b = rem / NK, k = rem - b * NK - (K - 1);
+      const double2* row = f < 3 ? U + (size_t)f * G.coll : W + (size_t)(f - 3) * G.coll;
+      v[u] = coef(row, K, k, b);
+      d[u] = ok ? (f * M + b) * Zp + (k >= 0 ? k : k + Z) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < IT; ++u)
+      if (d[u] >= 0) A[d[u]] = v[u];
+    constexpr int NZ = K + 1;   // j = K .. 2K
+    for (int i = tid; i < 6 * M * NZ; i += NT) {
+      const int col = i / NZ;
+      A[col * Zp + K + (i - col * NZ)] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  run_fft<ZR0, ZR1, ZR2, 6 * M * Z, NT>(A, 6 * M, T.pz, 1.0, twz, B.twZ);
+  const NlPlane pl = NlPlane::make(G, B, slot, n, TT);
+  double mr = 0.0, mt = 0.0, mz = 0.0;
+  for (int z0 = 0; z0 < Z; z0 += PL) {
+    // pack the chunk's lines as k_theta_t does from Y
+    {
+      constexpr int TOT = PL * 3 * M, IT = (TOT + NT - 1) / NT;
+      double2 a[IT], bv[IT];
+      int it[IT];
+#pragma unroll
+      for (int u = 0; u < IT; ++u) {
+        const int i = tid + u * NT;
+        const bool ok = i < TOT;
+        const int ic = ok ? i : 0;
+        const int line = ic / (3 * M), rem = ic - line * 3 * M;
+        const int pr = rem / M, mm = rem - pr * M;
+        const double2* src = A + (size_t)(2 * pr * M + mm) * Zp + pidx(z0 + line);
+        a[u] = src[0];
+        bv[u] = src[(size_t)M * Zp];
+        it[u] = ok ? i : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < IT; ++u) {
+        if (it[u] < 0) continue;
+        const int i = it[u];
+        const int line = i / (3 * M), rem = i - line * 3 * M;
+        const int pr = rem / M, mm = rem - pr * M;
+        double2* L = Lb + (size_t)(pr * PL + line) * Tp;
+        if (mm == 0) {
+          L[0] = make_double2(a[u].x, bv[u].x);
+        } else {
+          L[mm] = make_double2(a[u].x - bv[u].y, a[u].y + bv[u].x);
+          L[TT - mm] = make_double2(a[u].x + bv[u].y, -a[u].y + bv[u].x);
+        }
+      }
+      constexpr int NZ = TT - 2 * M + 1;   // bins M .. T-M
+      for (int i = tid; i < 3 * PL * NZ; i += NT) {
+        const int t = i / NZ;
+        Lb[(size_t)t * Tp + M + (i - t * NZ)] = make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    run_fft<TR0, TR1, TR2, 3 * PL * TT, NT>(Lb, 3 * PL, T.pt, 1.0, twt, B.twT);
+    {
+      constexpr int IT = (PL * TT + NT - 1) / NT;
+      double2 nrt[IT];
+      double nzv[IT];
+#pragma unroll
+      for (int u = 0; u < IT; ++u) {
+        const int i = tid + u * NT;
+        if (i >= PL * TT) continue;
+        const int line = i / TT, th = i - line * TT;
+        const double2* L0 = Lb + (size_t)line * Tp + pidx(th);
+        pl.point(L0[0], L0[(size_t)PL * Tp], L0[(size_t)2 * PL * Tp], twt, th, mr, mt, mz, nrt[u], nzv[u]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < IT; ++u) {
+        const int i = tid + u * NT;
+        if (i >= PL * TT) continue;
+        const int line = i / TT, th = i - line * TT;
+        Lb[(size_t)line * Tp + th] = nrt[u];
+        double* d = (double*)(Lb + ((size_t)PL + (line >> 1)) * Tp + th);
+        d[line & 1] = nzv[u];
+      }
+    }
+    __syncthreads();
+    run_fft<TR0, TR1, TR2, 3 * PL * TT, NT>(Lb, PL + PL / 2, T.pt, -1.0, twt, B.twT);
+    // unpack as k_theta_t does, into the z columns of the nonlinear terms
+    for (int i = tid; i < PL * 3 * M; i += NT) {
+      const int line = i / (3 * M), rem = i - line * 3 * M;
+      const int f = rem / M, mm = rem - f * M;
+      const double2* L = Lb + (f == 2 ? (size_t)PL + (line >> 1) : (size_t)line) * Tp;
+      const double2 X = L[pidx(mm)], Xc = L[pidx((TT - mm) % TT)];
+      const bool re = f == 0 || (f == 2 && (line & 1) == 0);
+      double2 v;
+      if (re) v = make_double2(0.5 * (X.x + Xc.x), 0.5 * (X.y - Xc.y));
+      else v = make_double2(0.5 * (X.y + Xc.y), -0.5 * (X.x - Xc.x));
+      A[(size_t)(f * M + mm) * Zp + z0 + line] = v;
+    }
+    __syncthreads();
+  }
+  // the plane's velocity maxima where they are read (zero otherwise, as the
+  // z kernel leaves them)
+  const bool want_cfl = (B.prm[slot].adaptive && c->mode == MODE_PRED) || c->host_driven;
+  if (!want_cfl) {
+    if (tid < 3) B.cflmax[((size_t)slot * 3 + tid) * G.N + n] = 0ull;
+  } else {
+    __shared__ double wmax[3][NT / 64];
+    for (int off = 32; off >= 1; off >>= 1) {
+      mr = fmax(mr, __shfl_xor(mr, off));
+      mt = fmax(mt, __shfl_xor(mt, off));
+      mz = fmax(mz, __shfl_xor(mz, off));
+    }
+    if ((tid & 63) == 0) {
+      wmax[0][tid >> 6] = mr;
+      wmax[1][tid >> 6] = mt;
+      wmax[2][tid >> 6] = mz;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double v = 0.0;
+      for (int w = 0; w < NT / 64; ++w) v = fmax(v, wmax[tid][w]);
+      B.cflmax[((size_t)slot * 3 + tid) * G.N + n] = (unsigned long long)__double_as_longlong(v);
+    }
+  }
+  run_fft<ZR0, ZR1, ZR2, 6 * M * Z, NT>(A, 3 * M, T.pz, -1.0, twz, B.twZ);
+  // truncation to the retained k (store_zcol for every mode at once)
+  constexpr int PER = K + (M - 1) * NK;   // values per field
+  for (int i = tid; i < 3 * PER; i += NT) {
+    const int f = i / PER, r = i - f * PER;
+    double2* row = B.Nraw[c->ncur] + (size_t)slot * 3 * G.coll + (size_t)f * G.coll + (size_t)n * G.Hp;
+    if (r < K) {
+      row[r] = A[(size_t)(f * M) * Zp + pidx(r)];
+    } else {
+      const int m = 1 + (r - K) / NK, k = (r - K) - (m - 1) * NK - (K - 1);
+      row[K + (m - 1) * NK + k + K - 1] = A[(size_t)(f * M + m) * Zp + pidx(k >= 0 ? k : k + Z)];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -795,6 +996,32 @@ struct PlanKerns {
   FftKerns k;
 };
 }  // namespace
+
+// Plan pairs (z, theta) with a whole-plane kernel: the grids whose six
+// z-transformed fields fit in LDS.
+PlaneKern plane_kernel(const FftPlan& pz, const FftPlan& pt, size_t* smem) {
+  struct Entry {
+    int z[3], t[3], pl;
+    PlaneKern k;
+  };
+  static const Entry table[] = {
+      {{12, 6, 1}, {12, 3, 1}, 24, k_plane_t<12, 6, 1, 12, 3, 1, 24>},   // (K, M) = (24, 12): 132 KiB
+  };
+  auto same = [](const FftPlan& p, const int r[3]) {
+    if (p.nrad < 1 || p.nrad > 3) return false;
+    for (int i = 0; i < 3; ++i)
+      if ((i < p.nrad ? p.rad[i] : 1) != r[i]) return false;
+    return true;
+  };
+  for (const auto& e : table)
+    if (same(pz, e.z) && same(pt, e.t)) {
+      const int Z = pz.L, TT = pt.L, M = TT / 3;
+      *smem = ((size_t)6 * M * padded_len(Z) + (size_t)3 * e.pl * padded_len(TT) + Z + TT) * sizeof(double2);
+      return e.k;
+    }
+  *smem = 0;
+  return nullptr;
+}
 
 FftKerns fft_kernels(const FftPlan& pz, const FftPlan& pt) {
   static const PlanKerns table[] = {

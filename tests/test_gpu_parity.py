@@ -426,6 +426,40 @@ def test_four_pairs_beyond_64_points(opf, N, monkeypatch):
         assert np.array_equal(res[gp][1], res[2][1]), gp
 
 
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_plane_kernel_equals_transform_kernels(opf, adaptive, monkeypatch):
+    """At (K, M) = (24, 12) one workgroup per radial plane runs the whole
+    evaluation with its intermediates in LDS.  It makes the same passes and
+    operations as the three transform kernels, so two slots of different
+    states (one adaptive: the plane's velocity maxima feed the time step)
+    agree with the three-kernel run bit for bit, diagnostics included."""
+    g = load_golden("c0")
+    ic = g["ic"]
+    res = {}
+    for plane in ("1", "0"):
+        monkeypatch.setenv("OPF_PLANE", plane)
+        print(f"whole-plane kernel {plane}, adaptive {adaptive}", flush=True)
+        ctx = _setup(opf, g, capacity=2)
+        ctx.put_state(0, ic[0], ic[1], ic[2])
+        ctx.put_state(1, 1.5 * ic[0], 0.5 * ic[1], ic[2])
+        if adaptive:
+            ctx.adaptive(1, True, maxdt=float(g["dt"]), courant=0.5)
+        ctx.kernel_timing(True)
+        ctx.run(30, diag_every=1)
+        used = ctx.kernel_time("plane")[1]
+        ctx.kernel_timing(False)
+        assert (used > 0) == (plane == "1"), used
+        res[plane] = [(ctx.get_state(s), ctx.diag(s), ctx.timestep(s)) for s in range(2)]
+        ctx.close()
+    monkeypatch.delenv("OPF_PLANE")
+    for s in range(2):
+        for c in range(3):
+            assert np.array_equal(res["1"][s][0][c], res["0"][s][0][c]), (s, c)
+        assert len(res["1"][s][1]) == 30
+        assert np.array_equal(res["1"][s][1], res["0"][s][1]), s
+        assert res["1"][s][2] == res["0"][s][2], s
+
+
 @pytest.mark.parametrize("dt,expect", [(0.1, None), (0.2, 2), (0.4, 3)])
 def test_corrector_decisions_at_large_dt(dt, expect):
     """tim_check_cgce's branches in the device loop at (16, 6, 4): dt = 0.1
