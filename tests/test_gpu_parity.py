@@ -426,24 +426,29 @@ def test_four_pairs_beyond_64_points(opf, N, monkeypatch):
         assert np.array_equal(res[gp][1], res[2][1]), gp
 
 
-@pytest.mark.parametrize("adaptive", [False, True])
-def test_plane_kernel_equals_transform_kernels(opf, adaptive, monkeypatch):
+@pytest.mark.parametrize("case", ["plain", "adaptive", "coriolis+mirror"])
+def test_plane_kernel_equals_transform_kernels(opf, case, monkeypatch):
     """At (K, M) = (24, 12) one workgroup per radial plane runs the whole
     evaluation with its intermediates in LDS.  It makes the same passes and
     operations as the three transform kernels, so two slots of different
-    states (one adaptive: the plane's velocity maxima feed the time step)
-    agree with the three-kernel run bit for bit, diagnostics included."""
+    states agree with the three-kernel run bit for bit, diagnostics
+    included: plain steps; an adaptive slot (the plane's velocity maxima
+    feed the time step); a slot with the Coriolis terms (which read the
+    theta twiddles) beside one with the mirror symmetry imposed."""
     g = load_golden("c0")
     ic = g["ic"]
     res = {}
     for plane in ("1", "0"):
         monkeypatch.setenv("OPF_PLANE", plane)
-        print(f"whole-plane kernel {plane}, adaptive {adaptive}", flush=True)
+        print(f"whole-plane kernel {plane}, case {case}", flush=True)
         ctx = _setup(opf, g, capacity=2)
         ctx.put_state(0, ic[0], ic[1], ic[2])
         ctx.put_state(1, 1.5 * ic[0], 0.5 * ic[1], ic[2])
-        if adaptive:
+        if case == "adaptive":
             ctx.adaptive(1, True, maxdt=float(g["dt"]), courant=0.5)
+        elif case == "coriolis+mirror":
+            ctx.coriolis(0, Ek0=0.01, phi=0.3)
+            ctx.symmetry(1, mirror=True)
         ctx.kernel_timing(True)
         ctx.run(30, diag_every=1)
         used = ctx.kernel_time("plane")[1]
