@@ -73,10 +73,17 @@ def tune(wl, log=print):
     from .engine import Context, host_mesh
     N, K, M, cap, Re, alpha, dt, steps = wl
     os.environ["OPF_TUNING"] = "none"       # the built-in shape, not an older table's
+    states = _states(N, K, M, alpha, cap, host_mesh(N))
     probe = Context(N, K, M, capacity=cap)
     base = probe.tile_config() + (0, 2 if cap >= 4 else 1)
+    # a grid whose evaluation is the whole-plane kernel has no transform
+    # tiles to choose
+    probe.setup(0, Re, alpha, dt)
+    probe.put_state(0, *states[0])
+    probe.kernel_timing(True)
+    probe.run(1, slots=[0])
+    plane = probe.kernel_time("plane")[1] > 0
     probe.close()
-    states = _states(N, K, M, alpha, cap, host_mesh(N))
     cache = {}
 
     def cost(shape):
@@ -103,6 +110,8 @@ def tune(wl, log=print):
              3: lambda d: {0, 4, 16},
              4: lambda d: {1, 2} if cap >= 4 else {1}}
     for i, dflt in ((1, base[1]), (2, base[2]), (3, 0), (4, 1)):
+        if plane and i in (1, 2):
+            continue
         for w in sorted(cands[i](dflt)):
             cand = list(best)
             cand[i] = w
