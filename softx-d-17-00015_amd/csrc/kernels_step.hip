@@ -278,28 +278,37 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
   // column `col` of tim_lumesh_init's band (timestep.f90:67-96): the mesh
   // operator, then the boundary rows of the first and last KL + 1 columns.
   // fetch() reads the mesh values of a column into registers a column ahead
-  // of place(), so the loads overlap an elimination step.
+  // of place(), so the loads overlap an elimination step.  Both run on
+  // every step (with the column clamped, place() writing only when `put`):
+  // a load left pending on one path would make every later step wait for
+  // all the stores before it.
   struct Src {
     double lap[LDA], r2, b0, b1;
   };
   auto fetch = [&](int col, Src& c) {
     for (int q = 0; q < LDA; ++q) c.lap[q] = me[mo.lapb + q * N + col];
     c.r2 = me[mo.rinv2 + col];
-    c.b0 = col <= KL ? me[mo.dr0 + col + (KL + 1) * (1 - S)] : 0.0;
-    c.b1 = col >= N - KL - 1 ? me[mo.dr1 + (KL - N + 1 + col) + (KL + 1) * BC] : 0.0;
+    const int i0 = col <= KL ? col : KL, i1 = col >= N - KL - 1 ? col - (N - KL - 1) : 0;
+    c.b0 = me[mo.dr0 + i0 + (KL + 1) * (1 - S)];
+    c.b1 = me[mo.dr1 + i1 + (KL + 1) * BC];
   };
-  auto place = [&](int col, const Src& c) {
-    for (int row = 0; row < KL; ++row) A(row, col) = 0.0;
-    for (int q = 0; q < LDA; ++q) A(KL + q, col) = c2 * c.lap[q];
+  auto place = [&](int col, const Src& c, bool put) {
+    double v[NLU];
+    for (int row = 0; row < KL; ++row) v[row] = 0.0;
+    for (int q = 0; q < LDA; ++q) v[KL + q] = c2 * c.lap[q];
     const double d = diag_d(c.r2, G.Mp, m, prm.alpha, k, PM);
-    A(2 * KL, col) = (A(2 * KL, col) + c2 * d) + c1;
-    if (col <= KL) A(2 * KL - col, col) = c.b0;
-    if (col >= N - KL - 1) A(2 * KL + N - 1 - col, col) = c.b1;
+    v[2 * KL] = (v[2 * KL] + c2 * d) + c1;
+    for (int row = 0; row < NLU; ++row) {
+      double x = v[row];
+      x = (col <= KL && row == 2 * KL - col) ? c.b0 : x;
+      x = (col >= N - KL - 1 && row == 2 * KL + N - 1 - col) ? c.b1 : x;
+      if (put) A(row, col) = x;
+    }
   };
   auto build = [&](int col) {
     Src c;
     fetch(col, c);
-    place(col, c);
+    place(col, c, true);
   };
   // A finished column leaves as the LAPACK band column (F, pv) and as the
   // solve-ready copy the fused step kernel reads (fw, bw); meanwhile the
@@ -332,55 +341,91 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
     mt[0] = (unsigned char)kue;
     mt[1] = (unsigned char)mxo;
   };
-  if (BC == 1 && k == 0 && m == 0) {   // pressure (0,0) is never factored
-    for (int j = 0; j < N; ++j) {
-      build(j);
-      store(j, 0);
-      pv[j] = 0;
-    }
-    finish();
-    return;
-  }
+  // the pressure matrix of (0,0) is never factored: its lane stores the
+  // columns as built, in the same loop as the others
+  const bool raw = BC == 1 && k == 0 && m == 0;
   for (int col = 0; col < WC && col < N; ++col) build(col);
-  // DGBTF2, KL = KU = 4, KV = 8 (0-based rows)
-  const int KV = 2 * KL;
+  // DGBTF2, KL = KU = 4, KV = 8 (0-based rows).  A step touches columns
+  // j .. j + 8 (the window) and rows KV - 8 .. KV + 4 of them; it reads all
+  // of them that it may change at once and writes every one back, changed
+  // where DGBTF2 changes it and unchanged elsewhere, so the step is
+  // straight-line code (selects instead of branches on the run-time limits
+  // km, ju and the pivot).  Window slots past the matrix's last column hold
+  // columns already stored; rewriting them with their own values is
+  // harmless.  Every element receives DGBTF2's operations in DGBTF2's order.
+  constexpr int KV = 2 * KL;
   int ju = 0;
   Src nx;
   for (int j = 0; j < N; ++j) {
-    if (j + WC < N) fetch(j + WC, nx);
-    const int km = N - 1 - j < KL ? N - 1 - j : KL;
+    const int jn = j + WC < N ? j + WC : N - 1;
+    fetch(jn, nx);
+    const int km = raw ? 0 : (N - 1 - j < KL ? N - 1 - j : KL);
+    double pc[KL + 1];   // rows KV .. KV + KL of column j
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) pc[i] = A(KV + i, j);
     int jp = 0;
-    double best = fabs(A(KV, j));
-    for (int i = 1; i <= km; ++i) {
-      const double v = fabs(A(KV + i, j));
-      if (v > best) { best = v; jp = i; }
+    double best = fabs(pc[0]), piv = pc[0];
+#pragma unroll
+    for (int i = 1; i <= KL; ++i) {
+      const double v = fabs(pc[i]);
+      const bool better = i <= km && v > best;
+      best = better ? v : best;
+      jp = better ? i : jp;
+      piv = better ? pc[i] : piv;
     }
     pv[j] = (unsigned char)jp;
-    if (A(KV + jp, j) == 0.0) {
+    if (!raw && piv == 0.0) {
       atomicMax(&B.ctl[slot].status, 100 + mat);  // singular factor
       return;
     }
     const int lim = j + KL + jp < N - 1 ? j + KL + jp : N - 1;
-    ju = ju > lim ? ju : lim;
-    if (jp != 0)
-      for (int c = 0; c <= ju - j; ++c) {
-        const double t = A(KV + jp - c, j + c);
-        A(KV + jp - c, j + c) = A(KV - c, j + c);
-        A(KV - c, j + c) = t;
+    ju = raw ? ju : (ju > lim ? ju : lim);
+    const int nc = raw ? -1 : ju - j;   // columns j .. j + nc take part (nc <= KV)
+    // row interchange of rows j and j + jp in columns j .. ju
+    double yv[KV + 1];                  // row KV - c of column j + c after it
+    {
+      double ra[KV + 1], rb[KV + 1];
+#pragma unroll
+      for (int c = 0; c <= KV; ++c) {
+        ra[c] = A(KV + jp - c, j + c);
+        rb[c] = A(KV - c, j + c);
       }
-    if (km > 0) {
-      const double rc = 1.0 / A(KV, j);
-      for (int i = 1; i <= km; ++i) A(KV + i, j) = A(KV + i, j) * rc;
-      for (int c = 1; c <= ju - j; ++c) {
-        const double y = A(KV - c, j + c);
-        if (y != 0.0) {
-          const double temp = -y;
-          for (int i = 1; i <= km; ++i) A(KV + i - c, j + c) = A(KV + i - c, j + c) + A(KV + i, j) * temp;
-        }
+#pragma unroll
+      for (int c = 0; c <= KV; ++c) {
+        const bool sw = jp != 0 && c <= nc;
+        A(KV + jp - c, j + c) = sw ? rb[c] : ra[c];
+        A(KV - c, j + c) = sw ? ra[c] : rb[c];
+        yv[c] = sw ? ra[c] : rb[c];
+      }
+    }
+    // multipliers: the pivot column below the diagonal after the
+    // interchange, scaled by 1/pivot (rows 1 .. km)
+    const double rc = 1.0 / (raw ? 1.0 : piv);
+    double l[KL + 1];
+#pragma unroll
+    for (int i = 1; i <= KL; ++i) {
+      const double li = (jp != 0 && i == jp) ? pc[0] : pc[i];
+      l[i] = i <= km ? li * rc : li;
+      A(KV + i, j) = l[i];
+    }
+    // rank-1 update of columns j+1 .. ju, rows below each one's diagonal
+    {
+      double cv[KV + 1][KL + 1];
+#pragma unroll
+      for (int c = 1; c <= KV; ++c)
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) cv[c][i] = A(KV + i - c, j + c);
+#pragma unroll
+      for (int c = 1; c <= KV; ++c) {
+        const double temp = -yv[c];
+        const bool col_on = c <= nc && yv[c] != 0.0;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i)
+          A(KV + i - c, j + c) = (col_on && i <= km) ? cv[c][i] + l[i] * temp : cv[c][i];
       }
     }
     store(j, jp);
-    if (j + WC < N) place(j + WC, nx);
+    place(jn, nx, j + WC < N);
   }
   finish();
 }
