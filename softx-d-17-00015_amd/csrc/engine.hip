@@ -335,7 +335,11 @@ void launch(opf_ctx* c, const char* name, Kern k, dim3 grid, dim3 block, size_t 
 }
 
 dim3 grid_h(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, 1, nslots); }
-dim3 grid_hn(opf_ctx* c, int nslots) { return dim3((c->H + 63) / 64, c->N, nslots); }
+// k_curl's blocks: (64-harmonic block, n) tiles rounded up to 8 XCDs
+dim3 grid_hn(opf_ctx* c, int nslots) {
+  const int ntile = ((c->H + 63) / 64) * c->N;
+  return dim3(8 * ((ntile + 7) / 8), 1, nslots);
+}
 // blocks of k_zfwd_t / k_zbwd_t: their (m tile, n, field) tiles, 6 or 3
 // fields, rounded up to 8 XCDs
 unsigned ztile_blocks(const opf_ctx* c, int nfield) {

@@ -734,10 +734,16 @@ __device__ __forceinline__ void curl_point(const Geo& G, int h, int n, int k, in
 }
 
 // which = 0: curl of U[ucur]; used after put_state and after every step.
-// One thread per (h, n): grid x = harmonics/64, y = N, z = slot.
+// One thread per (h, n).  grid: x = 8 * per blocks over the (64-harmonic
+// block, n) tiles, z = slot; tiles in XCD-aware order (block b takes tile
+// (b % 8) per + b / 8, tiles ordered harmonic block first), so the nine rows
+// of a stencil are read by workgroups on one XCD and come from its L2.
 __global__ void k_curl(Geo G, Bufs B, int slot0, int gated) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = blockIdx.y;
+  const int nhb = (G.H + 63) / 64, ntile = nhb * G.N, per = (ntile + 7) / 8;
+  const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (tile >= ntile) return;
+  const int hb = tile / G.N, n = tile - hb * G.N;
+  const int h = hb * 64 + threadIdx.x;
   const int slot = slot_of(B, slot0);
   const SlotCtl* c = &B.ctl[slot];
   if (!c->active || h >= G.H) return;
