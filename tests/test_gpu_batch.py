@@ -22,7 +22,7 @@ NSTEPS = 30
 def _ic(ctx, alpha, E0, seed):
     from opf import ic
     me = ctx.mesh()
-    return ic.seeded_perturbation(me["r"], me["intrdr"], K, M, alpha, seed, E0)
+    return ic.seeded_perturbation(me["r"], me["intrdr"], ctx.K, ctx.M, alpha, seed, E0)
 
 
 def test_batch_equals_alone_bitwise():
@@ -45,10 +45,10 @@ def test_batch_equals_alone_bitwise():
         assert batch.status(s)["step"] == NSTEPS
 
 
-def _mix_context(opf, graph, compact, slow_dt=0.08, streams=1):
+def _mix_context(opf, graph, compact, slow_dt=0.08, streams=1, grid=(N, K, M)):
     """The MIX slots plus one slot at a large time step, which takes several
     correctors per step and so iterates on alone at the end of every run."""
-    ctx = opf.Context(N, K, M, capacity=len(MIX) + 1)
+    ctx = opf.Context(*grid, capacity=len(MIX) + 1)
     ctx.launch_mode(graph=graph, compact=compact, streams=streams)
     for s, (Re, al, dt, E0, seed) in enumerate(MIX + [(2400.0, 1.25, slow_dt, 1e-3, 21)]):
         ctx.setup(s, Re, al, dt)
@@ -56,17 +56,20 @@ def _mix_context(opf, graph, compact, slow_dt=0.08, streams=1):
     return ctx
 
 
-def test_graph_replay_and_compaction_equal_eager_launches():
+@pytest.mark.parametrize("grid,slow_dt", [((N, K, M), 0.08), ((32, 24, 12), 0.03)])
+def test_graph_replay_and_compaction_equal_eager_launches(grid, slow_dt):
     """The time loop replayed as a captured HIP graph (4 iterations per
     graph, the remainder launched eagerly), and split over two streams, with the run's tail
     launched over the list of slots still iterating, against eager launches
     over the whole range: every slot's state and diagnostics are bit for bit
     equal after each of three runs, with host reads of the states and an
-    eager launch (vel_imposesym) between the runs."""
+    eager launch (vel_imposesym) between the runs.  At (32,24,12) the
+    evaluation is the whole-plane kernel, so it is captured, compacted and
+    split too."""
     import opf
-    eager = _mix_context(opf, 0, 0)
-    fast = _mix_context(opf, 4, 1)
-    split = _mix_context(opf, 0, 1, streams=2)
+    eager = _mix_context(opf, 0, 0, slow_dt, grid=grid)
+    fast = _mix_context(opf, 4, 1, slow_dt, grid=grid)
+    split = _mix_context(opf, 0, 1, slow_dt, streams=2, grid=grid)
     for r in range(3):
         print(f"run {r}: 10 steps, eager and graph", flush=True)
         eager.run(10, diag_every=1)
