@@ -12,6 +12,9 @@
 //            transforms -> Yb[slot][n][z][3][M]
 //   k_zbwd : Yb -> z transform, truncation -> Nraw coll fields
 //
+// For grids whose radial plane fits in LDS, k_plane runs the same three
+// stages for one plane in one workgroup, with Yf and Yb kept in LDS.
+//
 // The padding, the m = 0 Hermitian fill and the coll <-> spec layout change
 // (variables.f90:184-294) are folded into the loads and stores.  Two real
 // theta transforms share one complex FFT (a + ib), so the three c2r
