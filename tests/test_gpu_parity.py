@@ -426,7 +426,7 @@ def test_four_pairs_beyond_64_points(opf, N, monkeypatch):
         assert np.array_equal(res[gp][1], res[2][1]), gp
 
 
-@pytest.mark.parametrize("case", ["plain", "adaptive", "coriolis+mirror"])
+@pytest.mark.parametrize("case", ["plain", "adaptive", "coriolis+mirror", "les+nonnewtonian"])
 def test_plane_kernel_equals_transform_kernels(opf, case, monkeypatch):
     """At (K, M) = (24, 12) one workgroup per radial plane runs the whole
     evaluation with its intermediates in LDS.  It makes the same passes and
@@ -434,7 +434,9 @@ def test_plane_kernel_equals_transform_kernels(opf, case, monkeypatch):
     states agree with the three-kernel run bit for bit, diagnostics
     included: plain steps; an adaptive slot (the plane's velocity maxima
     feed the time step); a slot with the Coriolis terms (which read the
-    theta twiddles) beside one with the mirror symmetry imposed."""
+    theta twiddles) beside one with the mirror symmetry imposed; the LES
+    and non-Newtonian plugins, which add their terms to the N the
+    evaluation leaves."""
     g = load_golden("c0")
     ic = g["ic"]
     res = {}
@@ -449,6 +451,10 @@ def test_plane_kernel_equals_transform_kernels(opf, case, monkeypatch):
         elif case == "coriolis+mirror":
             ctx.coriolis(0, Ek0=0.01, phi=0.3)
             ctx.symmetry(1, mirror=True)
+        elif case == "les+nonnewtonian":
+            ctx.les(0, model=2, C_s=0.1, filter=2, A_plus=25.0)
+            ctx.nonnewtonian(1, rho=1000.0, radius=0.01, mu_0=0.056, mu_inf=0.0056, lam_CY=3e-3, n_CY=0.6,
+                             a_CY=2.0)
         ctx.kernel_timing(True)
         ctx.run(30, diag_every=1)
         used = ctx.kernel_time("plane")[1]
