@@ -508,32 +508,30 @@ __global__ void k_norm_sum(Geo G, const double* __restrict__ b, double* __restri
 // ---------------------------------------------------------------------------
 template <int D>
 __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned char* pv, double* X, int N) {
+  // Every global load is unconditional (column indices clamped into the
+  // matrix; values past the last column are never used), and a ring slot is
+  // refilled only after its last use: a load issued on one path only, or a
+  // slot refilled while its old value is still live (the two then need
+  // different registers and a copy at the loop's back edge), makes each pass
+  // round the loop wait for every load in flight.
   double w[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) w[i] = i < N ? X[i * 64] : 0.0;
   double fl[D][KL];
   int pr[D];
 #pragma unroll
-  for (int u = 0; u < D; ++u)
-    if (u < N - 1) {
+  for (int u = 0; u < D; ++u) {
+    const int c = u < N - 1 ? u : N - 2;
 #pragma unroll
-      for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)u * LUS + 2 * KL + 1 + i];
-      pr[u] = pv[u];
-    }
+    for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)c * LUS + 2 * KL + 1 + i];
+    pr[u] = pv[c];
+  }
   for (int j0 = 0; j0 < N - 1; j0 += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int j = j0 + u;
+      const int off = pr[u];
       if (j < N - 1) {
-        const int off = pr[u];
-        double a[KL];
-#pragma unroll
-        for (int i = 0; i < KL; ++i) a[i] = fl[u][i];
-        if (j + D < N - 1) {
-#pragma unroll
-          for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)(j + D) * LUS + 2 * KL + 1 + i];
-          pr[u] = pv[j + D];
-        }
         const double a0 = w[0];
         double s = w[0];
         s = off == 1 ? w[1] : s;
@@ -549,12 +547,17 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
         const int lm = N - 1 - j < KL ? N - 1 - j : KL;
 #pragma unroll
         for (int i = 1; i <= KL; ++i)
-          if (i <= lm) w[i] = w[i] + a[i - 1] * t;
+          if (i <= lm) w[i] = w[i] + fl[u][i - 1] * t;
         X[j * 64] = w[0];
 #pragma unroll
         for (int i = 0; i < 4; ++i) w[i] = w[i + 1];
-        w[4] = j + 5 < N ? X[(j + 5) * 64] : 0.0;
+        const double xn = X[(j + 5 < N ? j + 5 : N - 1) * 64];
+        w[4] = j + 5 < N ? xn : 0.0;
       }
+      const int jr = j + D < N - 1 ? j + D : N - 2;
+#pragma unroll
+      for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)jr * LUS + 2 * KL + 1 + i];
+      pr[u] = pv[jr];
     }
   }
   double v[9];
@@ -566,33 +569,30 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
   v[8] = w[0];
   double fu[D][2 * KL + 1];
 #pragma unroll
-  for (int u = 0; u < D; ++u)
-    if (N - 1 - u >= 0) {
+  for (int u = 0; u < D; ++u) {
+    const int c = N - 1 - u >= 0 ? N - 1 - u : 0;
 #pragma unroll
-      for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)(N - 1 - u) * LUS + r];
-    }
+    for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)c * LUS + r];
+  }
   for (int j0 = N - 1; j0 >= 0; j0 -= D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int j = j0 - u;
       if (j >= 0) {
-        double f[2 * KL + 1];
-#pragma unroll
-        for (int r = 0; r <= 2 * KL; ++r) f[r] = fu[u][r];
-        if (j - D >= 0) {
-#pragma unroll
-          for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)(j - D) * LUS + r];
-        }
-        v[8] = v[8] / f[8];
+        v[8] = v[8] / fu[u][8];
         const double t = v[8];
         X[j * 64] = t;
 #pragma unroll
         for (int d = 1; d <= 8; ++d)
-          if (j - d >= 0) v[8 - d] = v[8 - d] - t * f[8 - d];
+          if (j - d >= 0) v[8 - d] = v[8 - d] - t * fu[u][8 - d];
 #pragma unroll
         for (int q = 8; q >= 1; --q) v[q] = v[q - 1];
-        v[0] = j - 9 >= 0 ? X[(j - 9) * 64] : 0.0;
+        const double xn = X[(j - 9 >= 0 ? j - 9 : 0) * 64];
+        v[0] = j - 9 >= 0 ? xn : 0.0;
       }
+      const int jr = j - D >= 0 ? j - D : 0;
+#pragma unroll
+      for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)jr * LUS + r];
     }
   }
 }
