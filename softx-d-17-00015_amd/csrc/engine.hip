@@ -373,8 +373,8 @@ void launch_step(opf_ctx* c, int s0, int ns) {
 // vel_matrices for slots s0 .. s0+ns-1 (gated: only slots whose control
 // block asks for it, the tim_new_dt of this step)
 void launch_matrices(opf_ctx* c, int s0, int ns, int gated) {
-  const dim3 gp((c->P + 63) / 64, 4, ns);
-  launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, s0, gated, LuSpec{-1, 0, 0, 0.0, 0.0});
+  launch(c, "factor", k_lu_factor, dim3((c->P + 15) / 16, 4, ns), dim3(64), 0, c->G, c->B, s0, gated,
+         LuSpec{-1, 0, 0, 0.0, 0.0});
   launch(c, "influence", k_influence, dim3((c->P + 63) / 64, 5, ns), dim3(64), 0, c->G, c->B, s0, gated);
   launch(c, "ainv", k_ainv, grid_h(c, ns), dim3(64), 0, c->G, c->B, s0, gated);
 }
@@ -1837,8 +1837,8 @@ extern "C" int opf_tim_lumesh_init(opf_ctx* c, int slot, int which, int PM, int 
     auto ctl = read_ctl(c);
     ctl[slot].status = 0;
     write_ctl(c, ctl);
-    const dim3 gp((c->P + 63) / 64, 4, 1);
-    launch(c, "factor", k_lu_factor, gp, dim3(64), 0, c->G, c->B, slot, 0, LuSpec{which, PM, BC, c1, c2});
+    launch(c, "factor", k_lu_factor, dim3((c->P + 15) / 16, 4, 1), dim3(64), 0, c->G, c->B, slot, 0,
+           LuSpec{which, PM, BC, c1, c2});
     ctl = read_ctl(c);
     c->lu_bc[(size_t)slot * 4 + which] = BC;
     if (ctl[slot].status != 0) {

@@ -237,14 +237,19 @@ __device__ __forceinline__ double diag_d(double r2inv, int Mp, int m, double alp
 }
 
 // ---------------------------------------------------------------------------
-// tim_lumesh_init (timestep.f90:67-96) + DGBTF2, one lane per (pair, matrix),
-// writing the factors in LAPACK band layout and the solve-ready copy (see
-// fw_block / bw_block, meta_block).  grid: x = pairs/64, y = 4 matrices, z = slot.
+// tim_lumesh_init (timestep.f90:67-96) + DGBTF2, four lanes per (pair,
+// matrix), writing the factors in LAPACK band layout and the solve-ready
+// copy (see fw_block / bw_block, meta_block).
+// grid: x = pairs/16, y = 4 matrices, z = slot; 64 threads = 16 systems x 4 lanes.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int gated, LuSpec spec) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int LPS = 4;              // lanes per system
+  constexpr int SPW = 64 / LPS;       // systems per workgroup
+  const int q = threadIdx.x % LPS, sl = threadIdx.x / LPS;
+  const int p = blockIdx.x * SPW + sl;
   const int mat = blockIdx.y;
   const int slot = slot_of(B, slot0);
+  // no barriers below: a lane may leave on its own
   if (p >= G.P || (gated && !B.ctl[slot].refactor) || (spec.mat >= 0 && mat != spec.mat)) return;
   const int N = G.N, P = G.P;
   const int m = p / G.K, k = p % G.K;  // pair (|k|, m)
@@ -264,51 +269,61 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
   double* F = lu_block(B, slot, mat, p, P, N);
   unsigned char* pv = pv_block(B, slot, mat, p, P, N);
   // The elimination at column j touches columns j .. j + KV only, so each
-  // thread keeps a window of KV + 1 band columns in LDS (column c in slot
-  // c mod 9, the threads' values interleaved) instead of working on the
+  // system keeps a window of KV + 1 band columns in LDS (column c in slot
+  // c mod 9, the systems' values interleaved) instead of working on the
   // factor array in global memory, where every access of the serial
   // elimination waited a full cache round trip.  A column leaves the window
-  // finished (stored once, 128 contiguous bytes) and the next enters built
-  // from the mesh.  The arithmetic is DGBTF2's, operation for operation.
+  // finished (stored once) and the next enters built from the mesh.  The
+  // four lanes of a system share a step's work, each on its own data with
+  // the same instructions: lane q takes the window columns j + c with
+  // c = q, q + 4 (and 8 for lane 0), the rows q, q + 4, q + 8 (and 12) of a
+  // new column, and a quarter of a finished one; all four find the pivot.
+  // Lanes pass values to each other through the window: a wave's LDS
+  // operations complete in the order issued, and lds_fence() keeps the
+  // compiler from moving an access across a hand-over point.  The
+  // arithmetic is DGBTF2's, operation for operation.
   constexpr int WC = 2 * KL + 1;   // window columns
-  __shared__ double Wd[WC * NLU * 64];
-  const int tl = threadIdx.x;
-  auto A = [&](int row, int col) -> double& { return Wd[((col % WC) * NLU + row) * 64 + tl]; };
+  __shared__ double Wd[WC * NLU * SPW];
+  auto W = [&](int ws, int row) -> double& { return Wd[(ws * NLU + row) * SPW + sl]; };
+  auto lds_fence = [] { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
   const int S = (m * G.Mp + (PM < 0 ? -PM : PM)) % 2;
   // column `col` of tim_lumesh_init's band (timestep.f90:67-96): the mesh
   // operator, then the boundary rows of the first and last KL + 1 columns.
-  // fetch() reads the mesh values of a column into registers a column ahead
-  // of place(), so the loads overlap an elimination step.  Both run on
-  // every step (with the column clamped, place() writing only when `put`):
-  // a load left pending on one path would make every later step wait for
-  // all the stores before it.
+  // Lane q builds rows brow(t) = q + 4t (t < 3) and row 12 (t = 3, all four
+  // lanes, the same value).  fetch() reads the mesh values of a column into
+  // registers a column ahead of place(), so the loads overlap an
+  // elimination step.  Both run on every step (with the column clamped,
+  // place() writing only when `put`): a load left pending on one path would
+  // make every later step wait for all the stores before it.  Each lane
+  // loads its own rows' values and reads its own rows of a finished column,
+  // so no lane ever selects among values by q (the compiler turns such
+  // selects into an indexed stack array).
+  auto brow = [&](int t) { return t < 3 ? LPS * t + q : NLU - 1; };
   struct Src {
-    double lap[LDA], r2, b0, b1;
+    double lap[4], r2, b0, b1;
   };
   auto fetch = [&](int col, Src& c) {
-    for (int q = 0; q < LDA; ++q) c.lap[q] = me[mo.lapb + q * N + col];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int a = brow(t) - KL;
+      c.lap[t] = me[mo.lapb + (a < 0 ? 0 : a) * N + col];
+    }
     c.r2 = me[mo.rinv2 + col];
     const int i0 = col <= KL ? col : KL, i1 = col >= N - KL - 1 ? col - (N - KL - 1) : 0;
     c.b0 = me[mo.dr0 + i0 + (KL + 1) * (1 - S)];
     c.b1 = me[mo.dr1 + i1 + (KL + 1) * BC];
   };
-  auto place = [&](int col, const Src& c, bool put) {
-    double v[NLU];
-    for (int row = 0; row < KL; ++row) v[row] = 0.0;
-    for (int q = 0; q < LDA; ++q) v[KL + q] = c2 * c.lap[q];
+  auto place = [&](int ws, int col, const Src& c, bool put) {
     const double d = diag_d(c.r2, G.Mp, m, prm.alpha, k, PM);
-    v[2 * KL] = (v[2 * KL] + c2 * d) + c1;
-    for (int row = 0; row < NLU; ++row) {
-      double x = v[row];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int row = brow(t);
+      double x = row < KL ? 0.0 : c2 * c.lap[t];
+      x = row == 2 * KL ? (x + c2 * d) + c1 : x;
       x = (col <= KL && row == 2 * KL - col) ? c.b0 : x;
       x = (col >= N - KL - 1 && row == 2 * KL + N - 1 - col) ? c.b1 : x;
-      if (put) A(row, col) = x;
+      if (put) W(ws, row) = x;
     }
-  };
-  auto build = [&](int col) {
-    Src c;
-    fetch(col, c);
-    place(col, c, true);
   };
   // A finished column leaves as the LAPACK band column (F, pv) and as the
   // solve-ready copy the fused step kernel reads (fw, bw); meanwhile the
@@ -318,51 +333,58 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
   // r = 1/u0 (correctly rounded) instead of dividing by the diagonal.
   double2* fw = fw_block(B, slot, mat, p, P, N);
   double2* bw = bw_block(B, slot, mat, p, P, N);
-  int kue = 1, mxo = 0;
-  auto store = [&](int col, int pj) {
-    double cv[NLU];
-    for (int row = 0; row < NLU; ++row) cv[row] = A(row, col);
-    double* Fc = F + (size_t)col * LUS;
-    for (int row = 0; row < NLU; ++row) Fc[row] = cv[row];
-    for (int row = NLU; row < LUS; ++row) Fc[row] = 0.0;
-    fw[col * FW_Q + 0] = make_double2((double)pj, cv[9]);
-    fw[col * FW_Q + 1] = make_double2(cv[10], cv[11]);
-    fw[col * FW_Q + 2] = make_double2(cv[12], 0.0);
-    const double u0 = cv[2 * KL];
-    bw[col] = make_double2(u0, u0 != 0.0 ? 1.0 / u0 : 0.0);
-    for (int q = 1; q < BW_Q; ++q)
-      bw[(size_t)q * N + col] = make_double2(cv[2 * KL - (2 * q - 1)], cv[2 * KL - 2 * q]);
-    for (int d = 1; d <= 2 * KL; ++d)
-      if (col - d >= 0 && cv[2 * KL - d] != 0.0) kue = d > kue ? d : kue;
+  int kue = 1, mxo = 0;   // this lane's share of the meta block
+  auto rd = [&](int ws, int row) { return W(ws, row < NLU ? row : NLU - 1); };
+  auto store = [&](int ws, int col, int pj) {
+    // LAPACK column: rows 4q .. 4q + 3 from lane q (rows 13 .. 15 zero)
+    double x[LPS];
+#pragma unroll
+    for (int r = 0; r < LPS; ++r) {
+      const int row = LPS * q + r;
+      const double v = rd(ws, row);
+      x[r] = row < NLU ? v : 0.0;
+      // superdiagonal d = 8 - row of U in column col
+      if (row < 2 * KL && col - (2 * KL - row) >= 0 && v != 0.0) kue = 2 * KL - row > kue ? 2 * KL - row : kue;
+    }
+    reinterpret_cast<double4*>(F + (size_t)col * LUS)[q] = make_double4(x[0], x[1], x[2], x[3]);
+    // solve-ready copy: fw pair q (q < 3) = (pj, u9), (u10, u11), (u12, 0);
+    // bw pair q = (u8, 1/u8), (u7, u6), (u5, u4), (u3, u2) and, from lane 0,
+    // pair 4 = (u1, u0)
+    const double fa = rd(ws, q == 1 ? 10 : 12), fb = rd(ws, q == 0 ? 9 : 11);
+    if (q < FW_Q) fw[col * FW_Q + q] = make_double2(q == 0 ? (double)pj : fa, q == 2 ? 0.0 : fb);
+    static_assert(BW_Q == LPS + 1, "bw pairs: one per lane and a fifth from lane 0");
+    const double ba = rd(ws, q == 0 ? 2 * KL : 2 * KL + 1 - 2 * q), bb = rd(ws, 2 * KL - 2 * q);
+    bw[(size_t)q * N + col] = make_double2(ba, q == 0 ? (ba != 0.0 ? 1.0 / ba : 0.0) : bb);
+    if (q == 0) bw[(size_t)LPS * N + col] = make_double2(rd(ws, 1), rd(ws, 0));
     mxo = pj > mxo ? pj : mxo;
   };
-  auto finish = [&] {
-    unsigned char* mt = meta_block(B, slot, mat, p, P);
-    mt[0] = (unsigned char)kue;
-    mt[1] = (unsigned char)mxo;
-  };
-  // the pressure matrix of (0,0) is never factored: its lane stores the
+  // the pressure matrix of (0,0) is never factored: its lanes store the
   // columns as built, in the same loop as the others
   const bool raw = BC == 1 && k == 0 && m == 0;
-  for (int col = 0; col < WC && col < N; ++col) build(col);
+  for (int col = 0; col < WC && col < N; ++col) {
+    Src c;
+    fetch(col, c);
+    place(col, col, c, true);
+  }
   // DGBTF2, KL = KU = 4, KV = 8 (0-based rows).  A step touches columns
-  // j .. j + 8 (the window) and rows KV - 8 .. KV + 4 of them; it reads all
-  // of them that it may change at once and writes every one back, changed
-  // where DGBTF2 changes it and unchanged elsewhere, so the step is
-  // straight-line code (selects instead of branches on the run-time limits
-  // km, ju and the pivot).  Window slots past the matrix's last column hold
-  // columns already stored; rewriting them with their own values is
-  // harmless.  Every element receives DGBTF2's operations in DGBTF2's order.
+  // j .. j + 8 (the window) and rows KV - 8 .. KV + 4 of them; each lane
+  // reads the elements of its columns that the step may change at once and
+  // writes every one back, changed where DGBTF2 changes it and unchanged
+  // elsewhere, so the step is straight-line code (selects instead of
+  // branches on the run-time limits km, ju and the pivot).  Window slots
+  // past the matrix's last column hold columns already stored; rewriting
+  // them with their own values is harmless.
   constexpr int KV = 2 * KL;
   int ju = 0;
   Src nx;
-  for (int j = 0; j < N; ++j) {
+  for (int j = 0, wj = 0; j < N; ++j, wj = wj + 1 == WC ? 0 : wj + 1) {
     const int jn = j + WC < N ? j + WC : N - 1;
     fetch(jn, nx);
+    lds_fence();   // the previous step's columns and new column are written
     const int km = raw ? 0 : (N - 1 - j < KL ? N - 1 - j : KL);
     double pc[KL + 1];   // rows KV .. KV + KL of column j
 #pragma unroll
-    for (int i = 0; i <= KL; ++i) pc[i] = A(KV + i, j);
+    for (int i = 0; i <= KL; ++i) pc[i] = W(wj, KV + i);
     int jp = 0;
     double best = fabs(pc[0]), piv = pc[0];
 #pragma unroll
@@ -373,31 +395,14 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
       jp = better ? i : jp;
       piv = better ? pc[i] : piv;
     }
-    pv[j] = (unsigned char)jp;
+    if (q == 0) pv[j] = (unsigned char)jp;
     if (!raw && piv == 0.0) {
-      atomicMax(&B.ctl[slot].status, 100 + mat);  // singular factor
+      if (q == 0) atomicMax(&B.ctl[slot].status, 100 + mat);  // singular factor
       return;
     }
     const int lim = j + KL + jp < N - 1 ? j + KL + jp : N - 1;
     ju = raw ? ju : (ju > lim ? ju : lim);
     const int nc = raw ? -1 : ju - j;   // columns j .. j + nc take part (nc <= KV)
-    // row interchange of rows j and j + jp in columns j .. ju
-    double yv[KV + 1];                  // row KV - c of column j + c after it
-    {
-      double ra[KV + 1], rb[KV + 1];
-#pragma unroll
-      for (int c = 0; c <= KV; ++c) {
-        ra[c] = A(KV + jp - c, j + c);
-        rb[c] = A(KV - c, j + c);
-      }
-#pragma unroll
-      for (int c = 0; c <= KV; ++c) {
-        const bool sw = jp != 0 && c <= nc;
-        A(KV + jp - c, j + c) = sw ? rb[c] : ra[c];
-        A(KV - c, j + c) = sw ? ra[c] : rb[c];
-        yv[c] = sw ? ra[c] : rb[c];
-      }
-    }
     // multipliers: the pivot column below the diagonal after the
     // interchange, scaled by 1/pivot (rows 1 .. km)
     const double rc = 1.0 / (raw ? 1.0 : piv);
@@ -406,28 +411,41 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
     for (int i = 1; i <= KL; ++i) {
       const double li = (jp != 0 && i == jp) ? pc[0] : pc[i];
       l[i] = i <= km ? li * rc : li;
-      A(KV + i, j) = l[i];
     }
-    // rank-1 update of columns j+1 .. ju, rows below each one's diagonal
-    {
-      double cv[KV + 1][KL + 1];
+    // this lane's columns j + c: the row interchange of rows j and j + jp,
+    // then, for c = 0, the multipliers and, for c > 0, the rank-1 update of
+    // the rows below the diagonal
 #pragma unroll
-      for (int c = 1; c <= KV; ++c)
+    for (int t = 0; t < 3; ++t) {
+      const int c = q + LPS * t;
+      if (t == 2 && q != 0) break;   // c = 8 is lane 0's alone
+      const int ws = wj + c >= WC ? wj + c - WC : wj + c;
+      const double ra = W(ws, KV + jp - c), rb = W(ws, KV - c);
+      const bool sw = jp != 0 && c <= nc;
+      W(ws, KV + jp - c) = sw ? rb : ra;
+      W(ws, KV - c) = sw ? ra : rb;
+      const double temp = -(sw ? ra : rb);   // row KV - c after the interchange
+      const bool col_on = c <= nc && temp != 0.0;
+      double cv[KL + 1];
 #pragma unroll
-        for (int i = 1; i <= KL; ++i) cv[c][i] = A(KV + i - c, j + c);
+      for (int i = 1; i <= KL; ++i) cv[i] = W(ws, KV + i - c);
 #pragma unroll
-      for (int c = 1; c <= KV; ++c) {
-        const double temp = -yv[c];
-        const bool col_on = c <= nc && yv[c] != 0.0;
-#pragma unroll
-        for (int i = 1; i <= KL; ++i)
-          A(KV + i - c, j + c) = (col_on && i <= km) ? cv[c][i] + l[i] * temp : cv[c][i];
-      }
+      for (int i = 1; i <= KL; ++i)
+        W(ws, KV + i - c) = c == 0 ? l[i] : ((col_on && i <= km) ? cv[i] + l[i] * temp : cv[i]);
     }
-    store(j, jp);
-    place(jn, nx, j + WC < N);
+    lds_fence();   // column j is final
+    store(wj, j, jp);
+    lds_fence();   // every lane has read column j before its slot is refilled
+    place(wj, jn, nx, j + WC < N);
   }
-  finish();
+  // the four lanes of a system are neighbours in the wave
+  kue = max(kue, __shfl_xor(kue, 1));
+  kue = max(kue, __shfl_xor(kue, 2));
+  if (q == 0) {
+    unsigned char* mt = meta_block(B, slot, mat, p, P);
+    mt[0] = (unsigned char)kue;
+    mt[1] = (unsigned char)mxo;
+  }
 }
 
 // ---------------------------------------------------------------------------
