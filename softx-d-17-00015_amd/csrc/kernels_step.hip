@@ -334,6 +334,27 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
   double2* fw = fw_block(B, slot, mat, p, P, N);
   double2* bw = bw_block(B, slot, mat, p, P, N);
   int kue = 1, mxo = 0;   // this lane's share of the meta block
+  // The backward copy's planes hold one 16-byte pair per column (the pivot
+  // rows one byte), so writing each column as it is finished would fill a
+  // 128-byte line in eight pieces far apart in time, and lines left partly
+  // written in the cache cost write bandwidth.  Each lane collects its own
+  // entries for 8 columns (planes q and, lane 0, 4 in LDS; lane 0 the pivot
+  // offsets in a register) and writes them together: flush(c0, n, npv).
+  constexpr int SG = 8;
+  __shared__ double2 Sb[SG * BW_Q * SPW];
+  auto sb = [&](int cc, int qq) -> double2& { return Sb[(cc * BW_Q + qq) * SPW + sl]; };
+  unsigned long long pvs = 0;
+  auto flush = [&](int c0, int n, int npv) {
+#pragma unroll
+    for (int cc = 0; cc < SG; ++cc)
+      if (cc < n) {
+        bw[(size_t)q * N + c0 + cc] = sb(cc, q);
+        if (q == 0) bw[(size_t)LPS * N + c0 + cc] = sb(cc, LPS);
+      }
+    if (q == 0)
+      for (int cc = 0; cc < npv; ++cc) pv[c0 + cc] = (unsigned char)(pvs >> (8 * cc));
+    pvs = 0;
+  };
   auto rd = [&](int ws, int row) { return W(ws, row < NLU ? row : NLU - 1); };
   auto store = [&](int ws, int col, int pj) {
     // LAPACK column: rows 4q .. 4q + 3 from lane q (rows 13 .. 15 zero)
@@ -347,15 +368,15 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
       if (row < 2 * KL && col - (2 * KL - row) >= 0 && v != 0.0) kue = 2 * KL - row > kue ? 2 * KL - row : kue;
     }
     reinterpret_cast<double4*>(F + (size_t)col * LUS)[q] = make_double4(x[0], x[1], x[2], x[3]);
-    // solve-ready copy: fw pair q (q < 3) = (pj, u9), (u10, u11), (u12, 0);
-    // bw pair q = (u8, 1/u8), (u7, u6), (u5, u4), (u3, u2) and, from lane 0,
-    // pair 4 = (u1, u0)
+    // solve-ready copy, in window rows r: fw pair q (q < 3) = (pj, r9),
+    // (r10, r11), (r12, 0); bw pair q = (r8, 1/r8), (r7, r6), (r5, r4),
+    // (r3, r2) and, from lane 0, pair 4 = (r1, r0)
     const double fa = rd(ws, q == 1 ? 10 : 12), fb = rd(ws, q == 0 ? 9 : 11);
     if (q < FW_Q) fw[col * FW_Q + q] = make_double2(q == 0 ? (double)pj : fa, q == 2 ? 0.0 : fb);
     static_assert(BW_Q == LPS + 1, "bw pairs: one per lane and a fifth from lane 0");
     const double ba = rd(ws, q == 0 ? 2 * KL : 2 * KL + 1 - 2 * q), bb = rd(ws, 2 * KL - 2 * q);
-    bw[(size_t)q * N + col] = make_double2(ba, q == 0 ? (ba != 0.0 ? 1.0 / ba : 0.0) : bb);
-    if (q == 0) bw[(size_t)LPS * N + col] = make_double2(rd(ws, 1), rd(ws, 0));
+    sb(col % SG, q) = make_double2(ba, q == 0 ? (ba != 0.0 ? 1.0 / ba : 0.0) : bb);
+    if (q == 0) sb(col % SG, LPS) = make_double2(rd(ws, 1), rd(ws, 0));
     mxo = pj > mxo ? pj : mxo;
   };
   // the pressure matrix of (0,0) is never factored: its lanes store the
@@ -395,8 +416,9 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
       jp = better ? i : jp;
       piv = better ? pc[i] : piv;
     }
-    if (q == 0) pv[j] = (unsigned char)jp;
+    pvs |= (unsigned long long)jp << (8 * (j % SG));
     if (!raw && piv == 0.0) {
+      flush(j - j % SG, j % SG, j % SG + 1);
       if (q == 0) atomicMax(&B.ctl[slot].status, 100 + mat);  // singular factor
       return;
     }
@@ -437,6 +459,9 @@ __global__ void __launch_bounds__(64) k_lu_factor(Geo G, Bufs B, int slot0, int 
     store(wj, j, jp);
     lds_fence();   // every lane has read column j before its slot is refilled
     place(wj, jn, nx, j + WC < N);
+    // after place(): its wait for the mesh loads then does not include
+    // these stores (loads and stores share one in-order counter)
+    if (j % SG == SG - 1 || j == N - 1) flush(j - j % SG, j % SG + 1, j % SG + 1);
   }
   // the four lanes of a system are neighbours in the wave
   kue = max(kue, __shfl_xor(kue, 1));
