@@ -546,11 +546,14 @@ __global__ void k_norm_sum(Geo G, const double* __restrict__ b, double* __restri
 // lu_solve<double> for one lane whose vector lives in LDS (element n at
 // X[n * 64], the workgroup's lanes interleaved) and whose factor columns and
 // pivot offsets are read D columns ahead into a register ring, so a solve
-// waits on LDS rather than on a cache round trip per column.  The
-// operations and their order are lu_solve's.
+// waits on LDS rather than on a cache round trip per column.  The factors
+// come from the solve-ready copy (FW: pivot offset and multipliers, 48
+// contiguous bytes per column; BW: U's diagonal and superdiagonals in five
+// planes), half the bytes of the LAPACK columns' two 128-byte lines.  The
+// operations and their order are lu_solve's (the diagonal divides).
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned char* pv, double* X, int N) {
+__device__ __forceinline__ void lu_solve_lds(const double2* FW, const double2* BW, double* X, int N) {
   // Every global load is unconditional (column indices clamped into the
   // matrix; values past the last column are never used), and a ring slot is
   // refilled only after its last use: a load issued on one path only, or a
@@ -562,13 +565,28 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
   for (int i = 0; i < 5; ++i) w[i] = i < N ? X[i * 64] : 0.0;
   double fl[D][KL];
   int pr[D];
+  // column c of L: (pivot offset, l1) (l2, l3) (l4, 0)
+  auto lcol = [&](int c, double (&f)[KL], int& o) {
+    const double2 a = FW[(size_t)c * FW_Q], b = FW[(size_t)c * FW_Q + 1], e = FW[(size_t)c * FW_Q + 2];
+    f[0] = a.y;
+    f[1] = b.x;
+    f[2] = b.y;
+    f[3] = e.x;
+    o = (int)a.x;
+  };
+  // column c of U, band rows 0..8 (row 8 - d: superdiagonal d): plane 0
+  // holds (u0, 1/u0), plane q >= 1 (u_{2q-1}, u_{2q})
+  auto ucol = [&](int c, double (&f)[2 * KL + 1]) {
+    f[2 * KL] = ((const double*)(BW + c))[0];
 #pragma unroll
-  for (int u = 0; u < D; ++u) {
-    const int c = u < N - 1 ? u : N - 2;
+    for (int q = 1; q < BW_Q; ++q) {
+      const double2 v = BW[(size_t)q * N + c];
+      f[2 * KL + 1 - 2 * q] = v.x;
+      f[2 * KL - 2 * q] = v.y;
+    }
+  };
 #pragma unroll
-    for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)c * LUS + 2 * KL + 1 + i];
-    pr[u] = pv[c];
-  }
+  for (int u = 0; u < D; ++u) lcol(u < N - 1 ? u : N - 2, fl[u], pr[u]);
   for (int j0 = 0; j0 < N - 1; j0 += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -597,10 +615,7 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
         const double xn = X[(j + 5 < N ? j + 5 : N - 1) * 64];
         w[4] = j + 5 < N ? xn : 0.0;
       }
-      const int jr = j + D < N - 1 ? j + D : N - 2;
-#pragma unroll
-      for (int i = 0; i < KL; ++i) fl[u][i] = F[(size_t)jr * LUS + 2 * KL + 1 + i];
-      pr[u] = pv[jr];
+      lcol(j + D < N - 1 ? j + D : N - 2, fl[u], pr[u]);
     }
   }
   double v[9];
@@ -612,11 +627,7 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
   v[8] = w[0];
   double fu[D][2 * KL + 1];
 #pragma unroll
-  for (int u = 0; u < D; ++u) {
-    const int c = N - 1 - u >= 0 ? N - 1 - u : 0;
-#pragma unroll
-    for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)c * LUS + r];
-  }
+  for (int u = 0; u < D; ++u) ucol(N - 1 - u >= 0 ? N - 1 - u : 0, fu[u]);
   for (int j0 = N - 1; j0 >= 0; j0 -= D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -633,9 +644,7 @@ __device__ __forceinline__ void lu_solve_lds(const double* F, const unsigned cha
         const double xn = X[(j - 9 >= 0 ? j - 9 : 0) * 64];
         v[0] = j - 9 >= 0 ? xn : 0.0;
       }
-      const int jr = j - D >= 0 ? j - D : 0;
-#pragma unroll
-      for (int r = 0; r <= 2 * KL; ++r) fu[u][r] = F[(size_t)jr * LUS + r];
+      ucol(j - D >= 0 ? j - D : 0, fu[u]);
     }
   }
 }
@@ -660,8 +669,8 @@ __global__ void __launch_bounds__(64) k_influence(Geo G, Bufs B, int slot0, int 
   double* U = B.Uinf + (size_t)slot * 6 * N * P + p;
   __shared__ double Xs[MAXN * 64];
   double* x = Xs + threadIdx.x;
-  auto LUp = [&](int mat) { return (const double*)lu_block(B, slot, mat, p, P, N); };
-  auto PVp = [&](int mat) { return (const unsigned char*)pv_block(B, slot, mat, p, P, N); };
+  auto FWp = [&](int mat) { return (const double2*)fw_block(B, slot, mat, p, P, N); };
+  auto BWp = [&](int mat) { return (const double2*)bw_block(B, slot, mat, p, P, N); };
   // right-hand side and factor of this lane's solve (one inlined solve for
   // every case, so the vector stays in LDS and the factor loads in flight)
   const SlotParams prm = B.prm[slot];
@@ -677,7 +686,7 @@ __global__ void __launch_bounds__(64) k_influence(Geo G, Bufs B, int slot0, int 
     else v = (n == 0 || n == N - 1) ? 0.0 : 1.0;
     x[n * 64] = v;
   }
-  if (solve) lu_solve_lds<8>(LUp(mat), PVp(mat), x, N);
+  if (solve) lu_solve_lds<8>(FWp(mat), BWp(mat), x, N);
   if (which < 3) {
     double* u = U + (size_t)which * N * P;
     for (int n = 0; n < N; ++n) u[(size_t)n * P] = x[n * 64];
