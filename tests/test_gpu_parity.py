@@ -81,6 +81,23 @@ def test_lumesh_init_of_any_operator(opf, name):
             assert np.all(x[:, 0] == 0.0)
 
 
+def test_lumesh_init_of_a_singular_operator_is_reported(opf):
+    """tim_lumesh_init with c1 = c2 = 0 leaves only the axis and boundary rows,
+    so DGBTF2 meets a zero pivot in the second column of every pair; the
+    reference stops there (timestep.f90:92, dgbtrf info /= 0), the call
+    returns OPF_ERR_SINGULAR.  The slot's status is cleared, and the regular
+    coefficients rebuild the factor set bit for bit."""
+    g = load_golden("c0")
+    ctx = _setup(opf, g)
+    before = ctx.lumesh_get(0, 2)
+    with pytest.raises(opf.OpfError) as err:
+        ctx.lumesh_init(0, 2, 0, 0, 0.0, 0.0)
+    assert err.value.code == 3 and "tim_lumesh_init" in str(err.value)
+    ctx.lumesh_init(0, 2, 0, 0, 1.0 / float(g["dt"]), -0.51 / float(g["Re"]))
+    after = ctx.lumesh_get(0, 2)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+
+
 @pytest.mark.parametrize("name", ["mini", "c0"])
 def test_lumesh_invert_matches_oracle(opf, name):
     g = load_golden(name)
